@@ -1,0 +1,87 @@
+// Shared internals of libappfis_hip.so (gfx950 / CDNA4, wave64).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <hip/hip_fp16.h>
+#include <hip/hip_bf16.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <cstring>
+#include <string>
+
+#include "../../include/appfis_hip.h"
+
+struct appfis_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev_start = nullptr;
+    hipEvent_t ev_stop = nullptr;
+    // device scratch reused across calls (reductions, flood scalars); grown on demand
+    void* scratch = nullptr;
+    size_t scratch_bytes = 0;
+    // pinned host staging for small results
+    void* pinned = nullptr;
+    size_t pinned_bytes = 0;
+    int num_cu = 0;
+};
+
+namespace appfis {
+
+void set_error(const char* fmt, ...);
+int hip_fail(hipError_t e, const char* what, const char* file, int line);
+int invalid(const char* fmt, ...);
+
+// Ensure ctx->scratch holds at least `bytes`; returns status.
+int ensure_scratch(appfis_ctx* ctx, size_t bytes);
+int ensure_pinned(appfis_ctx* ctx, size_t bytes);
+
+inline size_t dtype_size(int dtype)
+{
+    switch (dtype) {
+    case APPFIS_U8: return 1;
+    case APPFIS_I32: return 4;
+    case APPFIS_F32: return 4;
+    case APPFIS_F64: return 8;
+    case APPFIS_F16: return 2;
+    case APPFIS_BF16: return 2;
+    case APPFIS_I64: return 8;
+    default: return 0;
+    }
+}
+
+// Validates ndim in 1..3, dims > 0, returns padded extents in d[3] (missing axes = 1).
+int check_shape(int ndim, const int* dims, int* d);
+
+inline unsigned ceil_div(size_t a, size_t b) { return (unsigned)((a + b - 1) / b); }
+
+// splitmix64 (Steele, Lea, Flood 2014): r(seed, n) = mix(seed + (n+1)*golden)
+__host__ __device__ inline uint64_t mix64(uint64_t z)
+{
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+    return z ^ (z >> 31);
+}
+__host__ __device__ inline uint64_t rng(uint64_t seed, uint64_t n)
+{
+    return mix64(seed + (n + 1) * 0x9E3779B97F4A7C15ULL);
+}
+
+}  // namespace appfis
+
+#define APPFIS_HIP_CHECK(expr)                                                      \
+    do {                                                                            \
+        hipError_t _e = (expr);                                                     \
+        if (_e != hipSuccess) return appfis::hip_fail(_e, #expr, __FILE__, __LINE__); \
+    } while (0)
+
+#define APPFIS_LAUNCH_CHECK()                                                       \
+    do {                                                                            \
+        hipError_t _e = hipGetLastError();                                          \
+        if (_e != hipSuccess) return appfis::hip_fail(_e, "kernel launch", __FILE__, __LINE__); \
+    } while (0)
+
+#define APPFIS_REQUIRE(cond, ...)                                                   \
+    do {                                                                            \
+        if (!(cond)) return appfis::invalid(__VA_ARGS__);                           \
+    } while (0)

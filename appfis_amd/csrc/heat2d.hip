@@ -1,0 +1,172 @@
+// HeatDiffusion 2D 5-point explicit sweep (config form of the reference functor
+// HeatDiffusion/src/heat_appfis.hpp:166-174, driven like heat_appfis.hpp:86-91 on a
+// DUAL grid: sweep, then Grid::Swap).
+//
+//   u' = u + c*((((uN + uS) + uE) + uW) - 4u)      one rounding per operation, no FMA
+//
+// PLAIN : one sweep per launch, one output per thread, x along the wave.
+// CHOSEN: temporal blocking.  A block loads an (TX+2T) x (TY+2T) window of the field
+//         into LDS and advances it T sweeps in place (each sweep: compute into
+//         registers, barrier, write back, barrier), shrinking the computed region by one
+//         cell per sweep, then writes the TX x TY centre.  Dirichlet ghost cells are
+//         loaded and never updated, exactly as in the one-sweep form, so every interior
+//         value is bit-identical to PLAIN.  HBM/MALL traffic per sweep drops by ~T.
+#include "common.hpp"
+
+#include <vector>
+
+using namespace appfis;
+
+namespace {
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_heat2d_plain(const T* __restrict__ u, T* __restrict__ v, int nx, int ny,
+                                                      T c)
+{
+    const int W = nx + 2;
+    const int i = blockIdx.x * 64 + threadIdx.x + 1;
+    const int j = blockIdx.y * 4 + threadIdx.y + 1;
+    if (i > nx || j > ny) return;
+    const size_t o = (size_t)i + (size_t)W * (size_t)j;
+    const T uc = u[o];
+    T s = u[o - W] + u[o + W];
+    s = s + u[o + 1];
+    s = s + u[o - 1];
+    s = s - (T)4 * uc;
+    v[o] = uc + c * s;
+}
+
+// Temporal blocking: TX x TY output tile, TS sweeps per launch, 256 threads.
+template <typename T, int TX, int TY, int TS>
+__global__ __launch_bounds__(256) void k_heat2d_tb(const T* __restrict__ u, T* __restrict__ v, int nx, int ny, T c,
+                                                   int steps)
+{
+    constexpr int RX = TX + 2 * TS, RY = TY + 2 * TS;
+    constexpr int NCELL = RX * RY;
+    constexpr int PER = (NCELL + 255) / 256;
+    __shared__ T tile[RY][RX];
+    const int W = nx + 2, H = ny + 2;
+    // window origin in padded coordinates (may be negative / past the array)
+    const int gx0 = blockIdx.x * TX + 1 - TS;
+    const int gy0 = blockIdx.y * TY + 1 - TS;
+    const int tid = threadIdx.x;
+    for (int p = tid; p < NCELL; p += 256) {
+        int ly = p / RX, lx = p - ly * RX;
+        int gx = gx0 + lx, gy = gy0 + ly;
+        T val = (T)0;
+        if (gx >= 0 && gx < W && gy >= 0 && gy < H) val = u[(size_t)gx + (size_t)W * (size_t)gy];
+        tile[ly][lx] = val;
+    }
+    __syncthreads();
+    T nv[PER];
+    for (int s = 0; s < steps; s++) {
+        // cells at distance >= s+1 from the window edge can be advanced to sweep s+1
+        const int lo = s + 1, hx = RX - s - 1, hy = RY - s - 1;
+#pragma unroll
+        for (int q = 0; q < PER; q++) {
+            int p = tid + q * 256;
+            int ly = p / RX, lx = p - ly * RX;
+            int gx = gx0 + lx, gy = gy0 + ly;
+            bool active = p < NCELL && lx >= lo && lx < hx && ly >= lo && ly < hy && gx >= 1 && gx <= nx &&
+                          gy >= 1 && gy <= ny;
+            if (active) {
+                const T uc = tile[ly][lx];
+                T sum = tile[ly - 1][lx] + tile[ly + 1][lx];
+                sum = sum + tile[ly][lx + 1];
+                sum = sum + tile[ly][lx - 1];
+                sum = sum - (T)4 * uc;
+                nv[q] = uc + c * sum;
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int q = 0; q < PER; q++) {
+            int p = tid + q * 256;
+            int ly = p / RX, lx = p - ly * RX;
+            int gx = gx0 + lx, gy = gy0 + ly;
+            bool active = p < NCELL && lx >= lo && lx < hx && ly >= lo && ly < hy && gx >= 1 && gx <= nx &&
+                          gy >= 1 && gy <= ny;
+            if (active) tile[ly][lx] = nv[q];
+        }
+        __syncthreads();
+    }
+    // write the TX x TY centre (interior cells only)
+    for (int p = tid; p < TX * TY; p += 256) {
+        int ly = p / TX, lx = p - ly * TX;
+        int gx = blockIdx.x * TX + 1 + lx, gy = blockIdx.y * TY + 1 + ly;
+        if (gx <= nx && gy <= ny) v[(size_t)gx + (size_t)W * (size_t)gy] = tile[ly + TS][lx + TS];
+    }
+}
+
+template <typename T>
+int run_plain(appfis_ctx* ctx, T* b0, T* b1, int nx, int ny, T c, int iters)
+{
+    dim3 block(64, 4), grid(ceil_div(nx, 64), ceil_div(ny, 4));
+    T* buf[2] = {b0, b1};
+    for (int it = 0; it < iters; it++) {
+        hipLaunchKernelGGL(k_heat2d_plain<T>, grid, block, 0, ctx->stream, buf[it & 1], buf[(it + 1) & 1], nx, ny, c);
+        APPFIS_LAUNCH_CHECK();
+    }
+    return APPFIS_OK;
+}
+
+// Split `iters` sweeps into launches of at most TS sweeps such that the number of
+// launches has the parity of `iters`: the field then ends in bufs[iters % 2], as
+// with one sweep per launch.
+inline int plan_launches(int iters, int ts, int* counts, int max_launches)
+{
+    int n = (iters + ts - 1) / ts;
+    if ((n & 1) != (iters & 1)) n++;
+    if (n > iters) n = iters;
+    if (n > max_launches) return -1;
+    int base = iters / n, extra = iters % n;
+    for (int l = 0; l < n; l++) counts[l] = base + (l < extra ? 1 : 0);
+    return n;
+}
+
+template <typename T, int TX, int TY, int TS>
+int run_tb(appfis_ctx* ctx, T* b0, T* b1, int nx, int ny, T c, int iters)
+{
+    T* buf[2] = {b0, b1};
+    dim3 grid(ceil_div(nx, TX), ceil_div(ny, TY));
+    std::vector<int> counts((size_t)iters);
+    int n = plan_launches(iters, TS, counts.data(), iters);
+    if (n <= 0) return invalid("heat2d: cannot split %d sweeps", iters);
+    for (int l = 0; l < n; l++) {
+        hipLaunchKernelGGL((k_heat2d_tb<T, TX, TY, TS>), grid, dim3(256), 0, ctx->stream, buf[l & 1],
+                           buf[(l + 1) & 1], nx, ny, c, counts[l]);
+        APPFIS_LAUNCH_CHECK();
+    }
+    return APPFIS_OK;
+}
+
+}  // namespace
+
+extern "C" int appfis_heat2d(appfis_ctx* ctx, int dtype, void* buf0, void* buf1, int nx, int ny, double c, int iters,
+                             int variant, int tsteps)
+{
+    APPFIS_REQUIRE(ctx && buf0 && buf1, "NULL argument");
+    APPFIS_REQUIRE(nx > 0 && ny > 0, "interior must be non-empty (nx=%d ny=%d)", nx, ny);
+    APPFIS_REQUIRE((double)(nx + 2) * (ny + 2) < 2147483647.0, "grid exceeds the int index range");
+    APPFIS_REQUIRE(iters >= 0, "iters must be >= 0");
+    APPFIS_REQUIRE(buf0 != buf1, "the two buffers must differ");
+    if (iters == 0) return APPFIS_OK;
+    if (variant == APPFIS_PLAIN) {
+        if (dtype == APPFIS_F64) return run_plain<double>(ctx, (double*)buf0, (double*)buf1, nx, ny, c, iters);
+        if (dtype == APPFIS_F32) return run_plain<float>(ctx, (float*)buf0, (float*)buf1, nx, ny, (float)c, iters);
+        return invalid("heat2d: dtype must be F64 or F32");
+    }
+    if (variant == APPFIS_CHOSEN) {
+        int ts = tsteps <= 0 ? 8 : tsteps;
+        if (dtype == APPFIS_F64) {
+            if (ts <= 4) return run_tb<double, 32, 32, 4>(ctx, (double*)buf0, (double*)buf1, nx, ny, c, iters);
+            return run_tb<double, 32, 32, 8>(ctx, (double*)buf0, (double*)buf1, nx, ny, c, iters);
+        }
+        if (dtype == APPFIS_F32) {
+            if (ts <= 4) return run_tb<float, 32, 32, 4>(ctx, (float*)buf0, (float*)buf1, nx, ny, (float)c, iters);
+            return run_tb<float, 64, 32, 8>(ctx, (float*)buf0, (float*)buf1, nx, ny, (float)c, iters);
+        }
+        return invalid("heat2d: dtype must be F64 or F32");
+    }
+    return invalid("heat2d: unknown variant %d", variant);
+}

@@ -1,0 +1,354 @@
+// Himeno 19-point Jacobi sweep (reference functor Himeno/src/himeno_appfis.hpp:178-198,
+// driven like himeno_appfis.hpp:104-113 on a DUAL p grid) with the config's residual
+//   gosa = sum over the interior of ss*ss      after every sweep.
+//
+// Layout: 14 dense arrays of dims (nx, ny, nz) including the 1-cell boundary shell.
+//
+// Kernel shape (2.5D k-march): a block owns full x rows (TX = 256 threads along x) and
+// RY rows per thread, and marches a chunk of ZC planes in k.  p is staged through a
+// 3-plane LDS ring (planes k-1, k, k+1 with their 1-cell x/y halo), so each p value is
+// read from memory about once; the 12 coefficient arrays and wrk1 are pure coalesced
+// streams.  gosa: per-thread FP64 accumulation, wave64 shuffle tree, one LDS stage, one
+// partial per block, and the last block to arrive (atomic ticket) sums the partials in
+// block order into gosa[it] -- deterministic, on the device, no host round trip.
+//
+// PLAIN (F32 or F64): the reference operation order, no FMA contraction.
+// CHOSEN (defect correction, F64 problem): p = B + q with a FP64 base B and a FP32
+//   correction q.  Because the sweep is affine, ss(B + q) = ss(B) + L q with
+//   L q = (a3 * A q - q) * bnd, A the 18-neighbour weighted sum.  Every `outer` sweeps a
+//   FP64 rebase sweep forms P = B + q, computes ss(P) exactly in FP64 (gosa from it),
+//   and restarts with B = P, r = (float)ss(P), q = (float)(omega*ss(P)).  In between,
+//   FP32 sweeps advance q with ss = r + L q using FP32 copies of the coefficients, so
+//   the cancellation ss << p never happens in FP32.  The FP32 coefficient copies are
+//   made once per call, inside the call.
+#include "common.hpp"
+
+#include <vector>
+
+using namespace appfis;
+
+namespace {
+
+constexpr int TX = 256;  // threads along x (one block spans 256 interior columns)
+constexpr int RY = 2;    // rows per thread
+constexpr int ZC = 32;   // planes per block
+constexpr int LW = TX + 2, LH = RY + 2;
+
+struct Coef {
+    const void* a0;
+    const void* a1;
+    const void* a2;
+    const void* a3;
+    const void* b0;
+    const void* b1;
+    const void* b2;
+    const void* c0;
+    const void* c1;
+    const void* c2;
+    const void* bnd;
+    const void* wrk1;  // PLAIN: wrk1;  inner sweep: r (FP32 residual of the base)
+};
+
+__device__ inline double block_sum_last(double v, double* partials, unsigned* ticket, double* out, double* lds)
+{
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    __shared__ bool last;
+    const unsigned nblk = gridDim.x * gridDim.y * gridDim.z;
+    const unsigned bid = blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z);
+    if (lane == 0) lds[wid] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double s = 0.0;
+        for (int w = 0; w < TX / 64; w++) s += lds[w];
+        partials[bid] = s;
+        __threadfence();
+        unsigned prev = atomicAdd(ticket, 1u);
+        last = (prev == nblk - 1);
+    }
+    __syncthreads();
+    if (!last) return 0.0;
+    __threadfence();
+    double acc = 0.0;
+    for (unsigned p = threadIdx.x; p < nblk; p += TX)
+        acc += __hip_atomic_load(partials + p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o, 64);
+    __syncthreads();
+    if (lane == 0) lds[wid] = acc;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double s = 0.0;
+        for (int w = 0; w < TX / 64; w++) s += lds[w];
+        *out = s;
+        *ticket = 0u;
+    }
+    return 0.0;
+}
+
+// MODE 0 (PLAIN):  ss = (s0*a3 - p)*bnd with s0 including wrk1; pn = p + omega*ss
+// MODE 1 (inner):  p is the correction q, wrk1 slot holds r;
+//                  ss = r + (s0*a3 - q)*bnd with s0 excluding wrk1; qn = q + omega*ss
+template <typename T, int MODE>
+__global__ __launch_bounds__(TX) void k_himeno(const T* __restrict__ p, T* __restrict__ pn, Coef cf, int nx, int ny,
+                                               int nz, T omega, double* partials, unsigned* ticket, double* gosa_out)
+{
+    __shared__ T ring[3][LH][LW];
+    __shared__ double red[TX / 64];
+    const int tx = threadIdx.x;
+    const int i = blockIdx.x * TX + tx + 1;          // interior x of this thread
+    const int j0 = blockIdx.y * RY + 1;              // first interior row of the block
+    const int k0 = blockIdx.z * ZC + 1;
+    const int k1 = min(k0 + ZC, nz - 1);
+    const size_t sy = (size_t)nx, sz = (size_t)nx * (size_t)ny;
+    const bool xin = i < nx - 1;
+    const T* a0 = (const T*)cf.a0; const T* a1 = (const T*)cf.a1; const T* a2 = (const T*)cf.a2;
+    const T* a3 = (const T*)cf.a3;
+    const T* b0 = (const T*)cf.b0; const T* b1 = (const T*)cf.b1; const T* b2 = (const T*)cf.b2;
+    const T* c0 = (const T*)cf.c0; const T* c1 = (const T*)cf.c1; const T* c2 = (const T*)cf.c2;
+    const T* bnd = (const T*)cf.bnd; const T* w1 = (const T*)cf.wrk1;
+
+    // stage plane k of p (block rows j0-1 .. j0+RY, columns blockIdx.x*TX .. +TX+1)
+    auto stage = [&](int k, int slot) {
+        const int xb = blockIdx.x * TX;
+        for (int q = tx; q < LW * LH; q += TX) {
+            int ly = q / LW, lx = q - ly * LW;
+            int gx = xb + lx, gy = j0 - 1 + ly;
+            T v = (T)0;
+            if (gx < nx && gy < ny) v = p[(size_t)gx + sy * gy + sz * k];
+            ring[slot][ly][lx] = v;
+        }
+    };
+    stage(k0 - 1, (k0 - 1) % 3);
+    stage(k0, k0 % 3);
+    double g = 0.0;
+    for (int k = k0; k < k1; k++) {
+        stage(k + 1, (k + 1) % 3);
+        __syncthreads();
+        const int sm = (k - 1) % 3, s0s = k % 3, sp = (k + 1) % 3;
+#pragma unroll
+        for (int r = 0; r < RY; r++) {
+            const int jj = j0 + r;
+            const int ly = r + 1, lx = tx + 1;
+            if (xin && jj < ny - 1) {
+                const size_t o = (size_t)i + sy * jj + sz * k;
+                const T pc = ring[s0s][ly][lx];
+                T s0 = a0[o] * ring[s0s][ly][lx + 1] + a1[o] * ring[s0s][ly + 1][lx] + a2[o] * ring[sp][ly][lx]
+                       + b0[o] * (ring[s0s][ly + 1][lx + 1] - ring[s0s][ly - 1][lx + 1] - ring[s0s][ly + 1][lx - 1]
+                                  + ring[s0s][ly - 1][lx - 1])
+                       + b1[o] * (ring[sp][ly + 1][lx] - ring[sp][ly - 1][lx] - ring[sm][ly + 1][lx]
+                                  + ring[sm][ly - 1][lx])
+                       + b2[o] * (ring[sp][ly][lx + 1] - ring[sp][ly][lx - 1] - ring[sm][ly][lx + 1]
+                                  + ring[sm][ly][lx - 1])
+                       + c0[o] * ring[s0s][ly][lx - 1] + c1[o] * ring[s0s][ly - 1][lx] + c2[o] * ring[sm][ly][lx];
+                T ss;
+                if constexpr (MODE == 0) {
+                    s0 = s0 + w1[o];
+                    ss = (s0 * a3[o] - pc) * bnd[o];
+                } else {
+                    ss = w1[o] + (s0 * a3[o] - pc) * bnd[o];
+                }
+                pn[o] = pc + omega * ss;
+                g += (double)ss * (double)ss;
+            }
+        }
+        __syncthreads();
+    }
+    block_sum_last(g, partials, ticket, gosa_out, red);
+}
+
+// CHOSEN rebase sweep (FP64): P = B + q; ss = ss(P) in FP64 with the FP64 coefficients;
+// writes B = P, r = (float)ss, qn = (float)(omega*ss); gosa from the FP64 ss.
+__global__ __launch_bounds__(TX) void k_himeno_rebase(const double* __restrict__ B, const float* __restrict__ q,
+                                                      double* __restrict__ Bn, float* __restrict__ r,
+                                                      float* __restrict__ qn, Coef cf, int nx, int ny, int nz,
+                                                      double omega, double* partials, unsigned* ticket,
+                                                      double* gosa_out)
+{
+    __shared__ double ring[3][LH][LW];
+    __shared__ double red[TX / 64];
+    const int tx = threadIdx.x;
+    const int i = blockIdx.x * TX + tx + 1;
+    const int j0 = blockIdx.y * RY + 1;
+    const int k0 = blockIdx.z * ZC + 1;
+    const int k1 = min(k0 + ZC, nz - 1);
+    const size_t sy = (size_t)nx, sz = (size_t)nx * (size_t)ny;
+    const bool xin = i < nx - 1;
+    const double* a0 = (const double*)cf.a0; const double* a1 = (const double*)cf.a1;
+    const double* a2 = (const double*)cf.a2; const double* a3 = (const double*)cf.a3;
+    const double* b0 = (const double*)cf.b0; const double* b1 = (const double*)cf.b1;
+    const double* b2 = (const double*)cf.b2;
+    const double* c0 = (const double*)cf.c0; const double* c1 = (const double*)cf.c1;
+    const double* c2 = (const double*)cf.c2;
+    const double* bnd = (const double*)cf.bnd; const double* w1 = (const double*)cf.wrk1;
+
+    auto stage = [&](int k, int slot) {
+        const int xb = blockIdx.x * TX;
+        for (int t = tx; t < LW * LH; t += TX) {
+            int ly = t / LW, lx = t - ly * LW;
+            int gx = xb + lx, gy = j0 - 1 + ly;
+            double v = 0.0;
+            if (gx < nx && gy < ny) {
+                size_t o = (size_t)gx + sy * gy + sz * k;
+                v = __dadd_rn(B[o], (double)q[o]);
+            }
+            ring[slot][ly][lx] = v;
+        }
+    };
+    stage(k0 - 1, (k0 - 1) % 3);
+    stage(k0, k0 % 3);
+    double g = 0.0;
+    for (int k = k0; k < k1; k++) {
+        stage(k + 1, (k + 1) % 3);
+        __syncthreads();
+        const int sm = (k - 1) % 3, s0s = k % 3, sp = (k + 1) % 3;
+#pragma unroll
+        for (int rr = 0; rr < RY; rr++) {
+            const int jj = j0 + rr;
+            const int ly = rr + 1, lx = tx + 1;
+            if (xin && jj < ny - 1) {
+                const size_t o = (size_t)i + sy * jj + sz * k;
+                const double pc = ring[s0s][ly][lx];
+                double s0 = a0[o] * ring[s0s][ly][lx + 1] + a1[o] * ring[s0s][ly + 1][lx] + a2[o] * ring[sp][ly][lx]
+                            + b0[o] * (ring[s0s][ly + 1][lx + 1] - ring[s0s][ly - 1][lx + 1]
+                                       - ring[s0s][ly + 1][lx - 1] + ring[s0s][ly - 1][lx - 1])
+                            + b1[o] * (ring[sp][ly + 1][lx] - ring[sp][ly - 1][lx] - ring[sm][ly + 1][lx]
+                                       + ring[sm][ly - 1][lx])
+                            + b2[o] * (ring[sp][ly][lx + 1] - ring[sp][ly][lx - 1] - ring[sm][ly][lx + 1]
+                                       + ring[sm][ly][lx - 1])
+                            + c0[o] * ring[s0s][ly][lx - 1] + c1[o] * ring[s0s][ly - 1][lx]
+                            + c2[o] * ring[sm][ly][lx] + w1[o];
+                const double ss = (s0 * a3[o] - pc) * bnd[o];
+                Bn[o] = pc;
+                r[o] = (float)ss;
+                qn[o] = (float)(omega * ss);
+                g += ss * ss;
+            }
+        }
+        __syncthreads();
+    }
+    block_sum_last(g, partials, ticket, gosa_out, red);
+}
+
+__global__ void k_f64_to_f32(const double* __restrict__ in, float* __restrict__ out, size_t n)
+{
+    size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (; t < n; t += stride) out[t] = (float)in[t];
+}
+
+// P = B + q on the interior (final field of the defect-correction form)
+__global__ void k_add_correction(const double* __restrict__ B, const float* __restrict__ q, double* __restrict__ out,
+                                 size_t n)
+{
+    size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (; t < n; t += stride) out[t] = __dadd_rn(B[t], (double)q[t]);
+}
+
+dim3 grid_of(int nx, int ny, int nz)
+{
+    return dim3(ceil_div(nx - 2, TX), ceil_div(ny - 2, RY), ceil_div(nz - 2, ZC));
+}
+
+}  // namespace
+
+extern "C" int appfis_himeno(appfis_ctx* ctx, int dtype, void* const* arrays, int nx, int ny, int nz, double omega,
+                             int iters, int variant, int outer, double* gosa_host)
+{
+    APPFIS_REQUIRE(ctx && arrays, "NULL argument");
+    for (int a = 0; a < 14; a++) APPFIS_REQUIRE(arrays[a], "arrays[%d] is NULL", a);
+    APPFIS_REQUIRE(arrays[0] != arrays[1], "p0 and p1 must differ");
+    APPFIS_REQUIRE(nx >= 3 && ny >= 3 && nz >= 3, "himeno needs at least 3 points per axis (shell included)");
+    APPFIS_REQUIRE((double)nx * ny * nz < 2147483647.0, "grid exceeds the int index range");
+    APPFIS_REQUIRE(iters >= 0, "iters must be >= 0");
+    if (iters == 0) return APPFIS_OK;
+    const dim3 grid = grid_of(nx, ny, nz);
+    const size_t nblk = (size_t)grid.x * grid.y * grid.z;
+    // scratch: ticket | partials[nblk] | gosa[iters]
+    const size_t off_part = 256, off_gosa = off_part + ((nblk * 8 + 255) / 256) * 256;
+    int st = ensure_scratch(ctx, off_gosa + (size_t)iters * 8 + 256);
+    if (st) return st;
+    unsigned* ticket = (unsigned*)ctx->scratch;
+    double* partials = (double*)((char*)ctx->scratch + off_part);
+    double* gosa_dev = (double*)((char*)ctx->scratch + off_gosa);
+    Coef cf{arrays[2], arrays[3], arrays[4], arrays[5], arrays[6], arrays[7], arrays[8],
+            arrays[9], arrays[10], arrays[11], arrays[12], arrays[13]};
+
+    if (variant == APPFIS_PLAIN) {
+        for (int it = 0; it < iters; it++) {
+            void* pin = arrays[it & 1];
+            void* pout = arrays[(it + 1) & 1];
+            if (dtype == APPFIS_F32) {
+                hipLaunchKernelGGL((k_himeno<float, 0>), grid, dim3(TX), 0, ctx->stream, (const float*)pin,
+                                   (float*)pout, cf, nx, ny, nz, (float)omega, partials, ticket, gosa_dev + it);
+            } else if (dtype == APPFIS_F64) {
+                hipLaunchKernelGGL((k_himeno<double, 0>), grid, dim3(TX), 0, ctx->stream, (const double*)pin,
+                                   (double*)pout, cf, nx, ny, nz, omega, partials, ticket, gosa_dev + it);
+            } else {
+                return invalid("himeno PLAIN: dtype must be F32 or F64");
+            }
+            APPFIS_LAUNCH_CHECK();
+        }
+    } else if (variant == APPFIS_CHOSEN) {
+        APPFIS_REQUIRE(dtype == APPFIS_F64, "himeno CHOSEN (defect correction) takes the F64 problem");
+        const int K = outer > 0 ? outer : 8;
+        const size_t n = (size_t)nx * ny * nz;
+        // work arrays: 11 FP32 coefficient copies, r, q (x2), B (x2 FP64)
+        std::vector<void*> work;
+        auto alloc = [&](size_t bytes) -> void* {
+            void* p = nullptr;
+            if (hipMalloc(&p, bytes) != hipSuccess) return nullptr;
+            work.push_back(p);
+            return p;
+        };
+        auto release = [&]() {
+            hipStreamSynchronize(ctx->stream);
+            for (void* p : work) hipFree(p);
+        };
+        float* c32[11];
+        for (int a = 0; a < 11; a++) c32[a] = (float*)alloc(n * 4);
+        float* r32 = (float*)alloc(n * 4);
+        float* q[2] = {(float*)alloc(n * 4), (float*)alloc(n * 4)};
+        double* B[2] = {(double*)alloc(n * 8), (double*)alloc(n * 8)};
+        for (void* p : work)
+            if (!p) { release(); return invalid("himeno CHOSEN: device allocation failed"); }
+        unsigned cb = (unsigned)std::min<size_t>(ceil_div(n, 256), 65536);
+        for (int a = 0; a < 11; a++)
+            hipLaunchKernelGGL(k_f64_to_f32, dim3(cb), dim3(256), 0, ctx->stream, (const double*)arrays[2 + a], c32[a],
+                               n);
+        // start: B = p (the whole array incl. shell), q = 0
+        hipMemcpyAsync(B[0], arrays[0], n * 8, hipMemcpyDeviceToDevice, ctx->stream);
+        hipMemcpyAsync(B[1], arrays[0], n * 8, hipMemcpyDeviceToDevice, ctx->stream);
+        hipMemsetAsync(q[0], 0, n * 4, ctx->stream);
+        hipMemsetAsync(q[1], 0, n * 4, ctx->stream);
+        Coef c32s{c32[0], c32[1], c32[2], c32[3], c32[4], c32[5], c32[6], c32[7], c32[8], c32[9], c32[10], r32};
+        int bi = 0, qi = 0;
+        for (int it = 0; it < iters; it++) {
+            if (it % K == 0) {
+                hipLaunchKernelGGL(k_himeno_rebase, grid, dim3(TX), 0, ctx->stream, B[bi], q[qi], B[bi ^ 1], r32,
+                                   q[qi ^ 1], cf, nx, ny, nz, omega, partials, ticket, gosa_dev + it);
+                bi ^= 1;
+            } else {
+                hipLaunchKernelGGL((k_himeno<float, 1>), grid, dim3(TX), 0, ctx->stream, q[qi], q[qi ^ 1], c32s, nx,
+                                   ny, nz, (float)omega, partials, ticket, gosa_dev + it);
+            }
+            qi ^= 1;
+            hipError_t e = hipGetLastError();
+            if (e != hipSuccess) { release(); return hip_fail(e, "himeno launch", __FILE__, __LINE__); }
+        }
+        // final field P = B + q into p[iters % 2]; the shell is copied unchanged from p0
+        void* out = arrays[iters & 1];
+        if (out != arrays[0]) hipMemcpyAsync(out, arrays[0], n * 8, hipMemcpyDeviceToDevice, ctx->stream);
+        hipLaunchKernelGGL(k_add_correction, dim3(cb), dim3(256), 0, ctx->stream, B[bi], q[qi], (double*)out, n);
+        hipError_t e = hipGetLastError();
+        release();
+        if (e != hipSuccess) return hip_fail(e, "himeno finish", __FILE__, __LINE__);
+    } else {
+        return invalid("himeno: unknown variant %d", variant);
+    }
+    if (gosa_host) {
+        APPFIS_HIP_CHECK(hipMemcpyAsync(gosa_host, gosa_dev, (size_t)iters * 8, hipMemcpyDeviceToHost, ctx->stream));
+        APPFIS_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+    }
+    return APPFIS_OK;
+}

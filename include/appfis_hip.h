@@ -1,0 +1,218 @@
+/*
+ * appfis_hip.h -- C ABI of libappfis_hip.so, the MI355X (gfx950) backend of the
+ * APPFIS iterative-stencil runtime.
+ *
+ * Conventions
+ *   - Every entry point returns an int status (APPFIS_OK == 0).  Nothing ever
+ *     calls exit(); the message of the last failure on the calling host thread
+ *     is available from appfis_last_error().
+ *   - Arrays are dense, row-major, x fastest, exactly like APPFIS::Array
+ *     (Appfis/Array.hpp:579-590 of the reference):  idx = i + dimX*(j + dimY*k).
+ *     "dims" are the padded extents (interior + 2*ghost per axis).
+ *   - Device pointers come from appfis_malloc (or any hipMalloc'd memory).
+ *   - All kernels are enqueued on the context's single HIP stream and are
+ *     asynchronous unless a host result is returned (then the call waits).
+ *   - One context per host thread; a context is not thread-safe.
+ *   - Faces follow APPFIS::DIRECTION (reference Appfis/Env.hpp:30):
+ *     LEFT=0 RIGHT=1 UP=2 DOWN=3 FRONT=4 BACK=5.  In 2D UP/DOWN are the y faces;
+ *     in 3D UP/DOWN are the z faces and FRONT/BACK the y faces.
+ *   - Reduction ops follow APPFIS::REDUCTION (reference Appfis/Env.hpp:32):
+ *     MAX=0 MIN=1 SUM=2.
+ */
+#ifndef APPFIS_HIP_H
+#define APPFIS_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define APPFIS_ABI_VERSION 1
+
+/* status codes */
+#define APPFIS_OK 0
+#define APPFIS_ERR_INVALID 1     /* bad argument (shape, dtype, face, null pointer) */
+#define APPFIS_ERR_HIP 2         /* a HIP runtime call failed */
+#define APPFIS_ERR_UNSUPPORTED 3 /* valid request that this build does not implement */
+#define APPFIS_ERR_NOMEM 4       /* device allocation failed */
+
+/* element types */
+#define APPFIS_U8 0
+#define APPFIS_I32 1
+#define APPFIS_F32 2
+#define APPFIS_F64 3
+#define APPFIS_F16 4
+#define APPFIS_BF16 5
+#define APPFIS_I64 6
+
+/* reductions (APPFIS::REDUCTION) */
+#define APPFIS_MAX 0
+#define APPFIS_MIN 1
+#define APPFIS_SUM 2
+
+/* kernel variants of the hand-written config kernels */
+#define APPFIS_PLAIN 0   /* one sweep per launch, reference dtype and operation order */
+#define APPFIS_CHOSEN 1  /* temporal blocking / packed / mixed-precision form */
+
+typedef struct appfis_ctx appfis_ctx;
+
+/* ---- library / context (replaces Topology::Initialize device+stream setup,
+ *      reference Appfis/Topology.hpp:124-151) ---------------------------------- */
+int appfis_abi_version(void);
+const char* appfis_status_string(int status);
+const char* appfis_last_error(void);
+int appfis_device_count(int* count);
+int appfis_ctx_create(int device, appfis_ctx** out);
+int appfis_ctx_destroy(appfis_ctx* ctx);
+int appfis_ctx_device(appfis_ctx* ctx, int* device);
+void* appfis_ctx_stream(appfis_ctx* ctx); /* the hipStream_t, for header-side launches */
+int appfis_sync(appfis_ctx* ctx);
+int appfis_device_name(appfis_ctx* ctx, char* buf, int len);
+
+/* ---- memory (replaces Grid::CreateSubGridsGPU / CopySubGridsGPUToCPU,
+ *      reference Appfis/Grid.hpp:403-479) --------------------------------------- */
+int appfis_malloc(appfis_ctx* ctx, size_t bytes, void** out); /* zero-filled, like Array::AllocateMemory (Array.hpp:339-342) */
+int appfis_free(appfis_ctx* ctx, void* ptr);
+int appfis_memcpy_h2d(appfis_ctx* ctx, void* dst, const void* src, size_t bytes);
+int appfis_memcpy_d2h(appfis_ctx* ctx, void* dst, const void* src, size_t bytes);
+int appfis_memcpy_d2d(appfis_ctx* ctx, void* dst, const void* src, size_t bytes);
+int appfis_memset(appfis_ctx* ctx, void* dst, int value, size_t bytes);
+
+/* ---- timing on the context stream (hipEvents) ---------------------------------- */
+int appfis_timer_start(appfis_ctx* ctx);
+int appfis_timer_stop(appfis_ctx* ctx, float* ms); /* waits for the stop event */
+
+/* ---- hipGraph capture of the context stream -------------------------------------- */
+int appfis_graph_begin(appfis_ctx* ctx);
+int appfis_graph_end(appfis_ctx* ctx, void** graph_exec);
+int appfis_graph_launch(appfis_ctx* ctx, void* graph_exec, int times);
+int appfis_graph_destroy(appfis_ctx* ctx, void* graph_exec);
+
+/* ---- seeded synthetic inputs ------------------------------------------------------
+ * r(seed, n) = mix64(seed + (n+1)*0x9E3779B97F4A7C15)  (splitmix64, Steele et al. 2014),
+ * n = interior linear index (x fastest, interior extents, no ghost).
+ * uniform: lo + (hi-lo)*u, u = (r>>11)*2^-53 for F64, (r>>40)*2^-24 for F32/F16/BF16
+ *          (then rounded to the storage type); ghost cells are not written.
+ * bernoulli: cell = (r < p*2^64) ? 1 : 0 for U8. */
+int appfis_fill_uniform(appfis_ctx* ctx, void* data, int dtype, int ndim, const int* dims, int ghost,
+                        uint64_t seed, double lo, double hi);
+int appfis_fill_bernoulli_u8(appfis_ctx* ctx, uint8_t* data, int ndim, const int* dims, int ghost,
+                             uint64_t seed, double p);
+int appfis_fill_value(appfis_ctx* ctx, void* data, int dtype, size_t count, double value);
+
+/* ---- boundary fill (replaces Grid::SetBoundary, reference Appfis/Grid.hpp:558-678,
+ *      and Grid::SetBoundaryCopy, Grid.hpp:682-830) -------------------------------
+ * face = -1 applies every face in the reference order (LEFT, RIGHT, UP, DOWN[, FRONT, BACK]).
+ * value points at one host element of dtype. */
+int appfis_set_boundary(appfis_ctx* ctx, void* data, int dtype, int ndim, const int* dims, int ghost,
+                        int face, const void* value);
+int appfis_set_boundary_copy(appfis_ctx* ctx, void* data, int dtype, int ndim, const int* dims,
+                             int ghost, int face);
+
+/* ---- halo exchange at MPI world size 1 (replaces Communication::Exchange/ExchangeGPU,
+ *      reference Appfis/Communication.hpp:613-748, with the face pack/unpack of
+ *      Runtime.hpp:917-1173).  periodic != 0: the ghost layers of every axis are
+ *      refreshed from the opposite interior layers, axis X first then Y then Z, each
+ *      axis over the full extent of the others, so edges and corners come out as in
+ *      the reference CPU path.  periodic == 0: no neighbour exists; nothing moves. */
+int appfis_exchange(appfis_ctx* ctx, void* data, int dtype, int ndim, const int* dims, int ghost,
+                    int periodic);
+
+/* ---- reductions over a box (replaces Execute{1,2,3}DReduce, reference
+ *      Appfis/Runtime.hpp:237-255,598-913 and Runtime_CUDA.hpp:554-740) ---------------
+ * box = {startX,endX,startY,endY,startZ,endZ} (half-open, like APPFIS::Config).
+ * appfis_reduce: result has the array's dtype (written to host memory `result`);
+ *   MAX starts from numeric_limits<T>::min() and MIN from numeric_limits<T>::max(),
+ *   as the reference does.  SUM is accumulated in a wide type (int64 for integers,
+ *   compensated FP64 for floats) and rounded to T once at the end.
+ * appfis_reduce_sum_wide: SUM only, result in int64 (U8/I32) or double (floats).
+ * appfis_reduce_device: result written to device memory `dev_result` (dtype of the
+ *   array for MAX/MIN; int64 or double for SUM); no host synchronisation. */
+int appfis_reduce(appfis_ctx* ctx, const void* data, int dtype, int ndim, const int* dims,
+                  const int* box, int op, void* result);
+int appfis_reduce_sum_wide(appfis_ctx* ctx, const void* data, int dtype, int ndim, const int* dims,
+                           const int* box, void* result);
+int appfis_reduce_device(appfis_ctx* ctx, const void* data, int dtype, int ndim, const int* dims,
+                         const int* box, int op, void* dev_result);
+
+/* ---- HeatDiffusion 2D (config form of the reference functor, HeatDiffusion/src/
+ *      heat_appfis.hpp:166-174, driven as in heat_appfis.hpp:86-91) -----------------
+ * u' = u + c*(uN + uS + uE + uW - 4u) over the nx*ny interior, ghost 1, double
+ * buffered.  bufs[0] holds the field at entry; after `iters` sweeps the field is in
+ * bufs[iters % 2] (the Grid::Swap parity).  dtype F64 or F32.
+ * variant PLAIN: one sweep per launch.  CHOSEN: `tsteps` sweeps per launch with an
+ * LDS-resident tile (temporal blocking); tsteps <= 0 picks the tuned depth. */
+int appfis_heat2d(appfis_ctx* ctx, int dtype, void* buf0, void* buf1, int nx, int ny, double c,
+                  int iters, int variant, int tsteps);
+
+/* ---- GameOfLife 2D, uint8, periodic, rule B3/S23 (config form of GameOfLife/src/
+ *      game_appfis.hpp:121-135 plus the periodic exchange of Communication.hpp:613-678).
+ * bufs as for heat2d (ghost 1, padded (nx+2)*(ny+2)).  The ghost ring of the
+ * written buffer is refreshed every generation.  PLAIN: one generation per launch;
+ * CHOSEN: bit-packed field, `tsteps` generations per launch. */
+int appfis_gol(appfis_ctx* ctx, uint8_t* buf0, uint8_t* buf1, int nx, int ny, int gens,
+               int variant, int tsteps);
+
+/* ---- FDTD3d radius-4 star (config form of FDTD3d/src/fdtd3d_appfis.hpp:129-143):
+ * u' = c[0]*u + sum_{r=1..4} c[r]*(6 neighbours at distance r); ghost 4 held at 0.
+ * dtype F32 / F64 (arithmetic in that type) or F16 / BF16 (storage only, FP32
+ * arithmetic).  Field in bufs[steps % 2] at exit. */
+int appfis_fdtd3d(appfis_ctx* ctx, int dtype, void* buf0, void* buf1, int nx, int ny, int nz,
+                  const double* coeff5, int steps, int variant);
+
+/* ---- Himeno 19-point Jacobi (Himeno/src/himeno_appfis.hpp:178-198) with the
+ * residual gosa = sum ss^2 over the interior after every sweep.
+ * arrays (14 device pointers, all of `dtype`, dims (nx,ny,nz) padded incl. the
+ * 1-cell boundary shell):  p0, p1, a0, a1, a2, a3, b0, b1, b2, c0, c1, c2, bnd, wrk1.
+ * Field in p[iters % 2] at exit.  gosa_host (iters doubles) receives each sweep's
+ * gosa.  variant PLAIN: dtype F32 or F64, reference operation order.
+ * variant CHOSEN (defect correction): dtype F64 inputs; FP32 inner sweeps on the
+ * correction, FP64 outer residual every `outer` sweeps (<=0: tuned).  */
+int appfis_himeno(appfis_ctx* ctx, int dtype, void* const* arrays, int nx, int ny, int nz,
+                  double omega, int iters, int variant, int outer, double* gosa_host);
+
+/* ---- Triton shallow-water flood step chain (Triton/src/triton_appfis.hpp:316-386
+ * with the functors of triton_appfis.hpp:567-1112) plus uniform rainfall.
+ * One scenario: fields (device, padded (nx+2)*(ny+2), dtype F64 or F32):
+ *   h, qx, qy (state, updated in place), z (dem, ghost = wall height), n2 (Manning n^2).
+ * params: see appfis_flood_params.  Runs `steps` time steps; writes per-step
+ * simulated time and FP64 Kahan volume (sum h * dx^2 over the interior) to the host
+ * arrays time_host / volume_host when not NULL. */
+typedef struct appfis_flood_params {
+    double dx;            /* cell size (m) */
+    double courant;       /* CFL number */
+    double hextra;        /* wet/dry depth threshold (m) */
+    double dt_fallback;   /* time step used while every cell is dry (arglist.time_step) */
+    double print_interval;/* dt is clamped to the next multiple of this (<=0: never) */
+    double rain_rate;     /* uniform rainfall (m/s), added as h += rain*dt after the update */
+    double start_time;    /* simulated time at entry (s) */
+} appfis_flood_params;
+
+int appfis_flood_run(appfis_ctx* ctx, int dtype, void* h, void* qx, void* qy, const void* z,
+                     const void* n2, int nx, int ny, const appfis_flood_params* params, int steps,
+                     int variant, double* time_host, double* volume_host);
+
+/* Ensemble: nscen independent scenarios of different sizes advanced together, one
+ * batched launch per phase over a tile work-list, the step loop captured in a
+ * hipGraph.  Pointer arrays are host arrays of device pointers (one per scenario).
+ * time_host / volume_host: nscen*steps doubles (scenario-major) or NULL. */
+int appfis_flood_ensemble_run(appfis_ctx* ctx, int dtype, int nscen, const int* nx, const int* ny,
+                              void* const* h, void* const* qx, void* const* qy,
+                              const void* const* z, const void* const* n2,
+                              const appfis_flood_params* params, int steps,
+                              double* time_host, double* volume_host);
+
+/* Synthetic DEM of the flood ensemble: sum of `nhills` Gaussian hills plus a planar
+ * slope along x, written to the interior of z (F64 or F32); ghost set to `wall`.
+ * Hill k (k < nhills): r_j = r(seed, j) for j = 4k..4k+3 ->
+ *   amplitude 5 + 45*u0 (m), sigma 200 + 1800*u1 (m), centre (u2*L, u3*L), L = nx*dx. */
+int appfis_flood_dem(appfis_ctx* ctx, int dtype, void* z, int nx, int ny, double dx, uint64_t seed,
+                     int nhills, double slope, double wall);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* APPFIS_HIP_H */

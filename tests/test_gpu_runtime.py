@@ -1,0 +1,119 @@
+"""GPU parity of the runtime entry points (C ABI) against the oracle.
+
+seeded inputs, Grid::SetBoundary / SetBoundaryCopy, the single-rank Exchange and the
+Execute*Reduce box reductions: bit-exact wherever the result is integer/byte work or a
+pure data movement; compensated sums within a few ulps of math.fsum.
+"""
+import math
+
+import numpy as np
+import pytest
+
+import oracle
+from appfis_amd import hip
+
+pytestmark = pytest.mark.gpu
+
+NPDT = {hip.U8: np.uint8, hip.I32: np.int32, hip.F32: np.float32, hip.F64: np.float64, hip.F16: np.float16}
+
+
+def test_abi_version_and_roundtrip(ctx):
+    assert hip.lib().appfis_abi_version() == 1
+    a = np.arange(1000, dtype=np.float64)
+    d = ctx.from_numpy(a)
+    assert np.array_equal(d.download(), a)
+
+
+@pytest.mark.parametrize("dt", [hip.F64, hip.F32, hip.F16])
+@pytest.mark.parametrize("shape,ghost", [((37, 29), 1), ((9, 13, 17), 4), ((100,), 2)])
+def test_fill_uniform_matches_oracle(ctx, dt, shape, ghost):
+    d = ctx.array(shape, dt)
+    ctx.fill_uniform(d, ghost, 1234, -0.5, 2.0)
+    ref = np.zeros(shape, dtype=NPDT[dt])
+    oracle.fill_uniform(ref, ghost, 1234, -0.5, 2.0)
+    assert np.array_equal(d.download().view(np.uint8), ref.view(np.uint8))
+
+
+def test_fill_uniform_bf16_and_bernoulli(ctx):
+    d = ctx.array((12, 20, 18), hip.BF16)
+    ctx.fill_uniform(d, 4, 77)
+    ref = np.zeros((12, 20, 18), dtype=np.uint16)
+    oracle.fill_uniform(ref, 4, 77, bf16=True)
+    assert np.array_equal(d.download(), ref)
+    g = ctx.array((130, 258), hip.U8)
+    ctx.fill_bernoulli(g, 1, 1002, 0.25)
+    r = np.zeros((130, 258), dtype=np.uint8)
+    oracle.fill_bernoulli(r, 1, 1002, 0.25)
+    assert np.array_equal(g.download(), r)
+
+
+@pytest.mark.parametrize("dt", [hip.U8, hip.I32, hip.F64, hip.F16])
+@pytest.mark.parametrize("shape,ghost", [((11, 14), 1), ((10, 12, 9), 2), ((16,), 3), ((13, 9, 11), 1)])
+def test_boundary_and_exchange_match_oracle(ctx, dt, shape, ghost):
+    rs = np.random.RandomState(sum(shape) + ghost)
+    base = rs.randint(0, 100, size=shape).astype(NPDT[dt])
+    nd = len(shape)
+    for face in [-1] + list(range(2 * nd)):
+        d = ctx.from_numpy(base, dt)
+        ctx.set_boundary(d, ghost, face, 7)
+        r = base.copy()
+        oracle.set_boundary(r, ghost, face, 7)
+        assert np.array_equal(d.download(), r), ("set_boundary", face)
+        d = ctx.from_numpy(base, dt)
+        ctx.set_boundary_copy(d, ghost, face)
+        r = base.copy()
+        oracle.set_boundary_copy(r, ghost, face)
+        assert np.array_equal(d.download(), r), ("set_boundary_copy", face)
+    if all(s - 2 * ghost >= 2 * ghost for s in shape):
+        d = ctx.from_numpy(base, dt)
+        ctx.exchange(d, ghost, True)
+        r = base.copy()
+        oracle.exchange(r, ghost, True)
+        assert np.array_equal(d.download(), r)
+
+
+def test_exchange_rejects_undersized_subgrid(ctx):
+    d = ctx.array((5, 5), hip.F64)
+    with pytest.raises(hip.AppfisError):
+        ctx.exchange(d, 2, True)
+
+
+@pytest.mark.parametrize("dt", [hip.U8, hip.I32, hip.F32, hip.F64])
+def test_reduce_matches_oracle(ctx, dt):
+    rs = np.random.RandomState(dt)
+    shape = (21, 37, 45)
+    if dt in (hip.U8, hip.I32):
+        a = rs.randint(0, 3 if dt == hip.U8 else 1000, size=shape).astype(NPDT[dt])
+        if dt == hip.I32:
+            a -= 500
+    else:
+        a = (rs.rand(*shape) - 0.3).astype(NPDT[dt])
+    d = ctx.from_numpy(a, dt)
+    for box in ([1, 44, 1, 36, 1, 20], [0, 45, 0, 37, 0, 21], [5, 6, 3, 30, 2, 3]):
+        for op in (hip.MAX, hip.MIN):
+            assert ctx.reduce(d, box, op) == oracle.reduce(a, box, op), (box, op)
+        sub = a[box[4]:box[5], box[2]:box[3], box[0]:box[1]]
+        w = ctx.reduce_sum_wide(d, box)
+        if dt in (hip.U8, hip.I32):
+            assert w == int(sub.astype(np.int64).sum())
+            # the T-typed SUM wraps like the reference's accumulation in T
+            bits = 8 if dt == hip.U8 else 32
+            wrapped = int(sub.astype(np.int64).sum()) % (1 << bits)
+            assert int(ctx.reduce(d, box, hip.SUM)) % (1 << bits) == wrapped
+        else:
+            exact = math.fsum(sub.astype(np.float64).ravel())
+            assert abs(w - exact) <= 4 * np.spacing(abs(exact))
+
+
+def test_reduce_max_quirk_and_large(ctx):
+    a = -np.ones((64, 64), dtype=np.float32)
+    d = ctx.from_numpy(a)
+    assert ctx.reduce(d, [1, 63, 1, 63], hip.MAX) == np.finfo(np.float32).tiny
+    # a box larger than one grid-stride pass, repeated calls reuse the ticket
+    big = np.zeros((257, 257, 130), dtype=np.float64)
+    oracle.fill_uniform(big, 1, 99)
+    db = ctx.from_numpy(big)
+    box = [1, 129, 1, 256, 1, 256]
+    exact = math.fsum(big[1:256, 1:256, 1:129].ravel())
+    for _ in range(3):
+        assert abs(ctx.reduce_sum_wide(db, box) - exact) <= 4 * np.spacing(exact)

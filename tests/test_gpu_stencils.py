@@ -1,0 +1,135 @@
+"""GPU parity of the hand-written config kernels against the oracle.
+
+Heat2D, GameOfLife and FDTD3d are integer work or fixed-order floating point with no
+FMA contraction on either side, so the bar is bit-exact: the PLAIN and CHOSEN forms
+must reproduce the oracle's bytes.  Full config sizes are covered through the oracle
+for a few sweeps plus size-independent properties for the full sweep count.
+"""
+import numpy as np
+import pytest
+
+import oracle
+from appfis_amd import hip
+
+pytestmark = pytest.mark.gpu
+
+FDTD_COEFF = [0.5, 0.125, 0.0625, 0.125 / 3, 0.03125]
+
+
+def _heat_pair(ctx, nx, ny, dt, seed=1001):
+    a = np.zeros((ny + 2, nx + 2), dtype=np.float64 if dt == hip.F64 else np.float32)
+    oracle.fill_uniform(a, 1, seed)
+    return a, ctx.from_numpy(a, dt), ctx.array(a.shape, dt)
+
+
+@pytest.mark.parametrize("dt", [hip.F64, hip.F32])
+@pytest.mark.parametrize("nx,ny,iters", [(37, 29, 11), (64, 64, 16), (130, 67, 9), (1, 5, 3)])
+@pytest.mark.parametrize("variant", [hip.PLAIN, hip.CHOSEN])
+def test_heat2d_bit_exact(ctx, dt, nx, ny, iters, variant):
+    a, d0, d1 = _heat_pair(ctx, nx, ny, dt)
+    ctx.heat2d(d0, d1, nx, ny, 0.1, iters, variant)
+    ref = oracle.heat2d(a.copy(), np.zeros_like(a), nx, ny, 0.1, iters)
+    got = (d0, d1)[iters % 2].download()
+    assert np.array_equal(got, ref)
+
+
+@pytest.mark.parametrize("tsteps", [1, 3, 4, 8])
+def test_heat2d_chosen_depths(ctx, tsteps):
+    nx, ny, iters = 200, 150, 23
+    a, d0, d1 = _heat_pair(ctx, nx, ny, hip.F64)
+    ctx.heat2d(d0, d1, nx, ny, 0.1, iters, hip.CHOSEN, tsteps)
+    ref = oracle.heat2d(a.copy(), np.zeros_like(a), nx, ny, 0.1, iters)
+    assert np.array_equal((d0, d1)[iters % 2].download(), ref)
+
+
+def test_heat2d_full_config(ctx):
+    # 1024^2 FP64, 500 sweeps: the whole config against the oracle
+    nx = ny = 1024
+    a, d0, d1 = _heat_pair(ctx, nx, ny, hip.F64)
+    ref = oracle.heat2d(a.copy(), np.zeros_like(a), nx, ny, 0.1, 500)
+    for variant in (hip.PLAIN, hip.CHOSEN):
+        d0.upload(a)
+        ctx.heat2d(d0, d1, nx, ny, 0.1, 500, variant)
+        got = d0.download()
+        assert np.array_equal(got, ref), variant
+    print(f"[heat2d] checksum {ref.sum():.17g}")
+
+
+def _gol_pair(ctx, nx, ny, seed=1002, p=0.25):
+    a = np.zeros((ny + 2, nx + 2), dtype=np.uint8)
+    oracle.fill_bernoulli(a, 1, seed, p)
+    return a, ctx.from_numpy(a), ctx.array(a.shape, hip.U8)
+
+
+@pytest.mark.parametrize("nx,ny,gens", [(14, 9, 5), (62, 40, 13), (64, 64, 10), (94, 33, 7), (126, 77, 21),
+                                        (253, 101, 6), (1000, 19, 4), (510, 257, 31)])
+def test_gol_plain_bit_exact(ctx, nx, ny, gens):
+    # widths cover W = nx + 2 = 0 .. 15 mod 16 across the cases
+    a, d0, d1 = _gol_pair(ctx, nx, ny)
+    ctx.gol(d0, d1, nx, ny, gens, hip.PLAIN)
+    ref = oracle.gol_u8(a.copy(), np.zeros_like(a), nx, ny, gens)
+    assert np.array_equal((d0, d1)[gens % 2].download(), ref)
+
+
+@pytest.mark.parametrize("nx", list(range(14, 30)))
+def test_gol_plain_all_residues(ctx, nx):
+    ny, gens = 23, 9
+    a, d0, d1 = _gol_pair(ctx, nx, ny, seed=nx)
+    ctx.gol(d0, d1, nx, ny, gens, hip.PLAIN)
+    ref = oracle.gol_u8(a.copy(), np.zeros_like(a), nx, ny, gens)
+    assert np.array_equal((d0, d1)[gens % 2].download(), ref)
+    # the other buffer holds generation gens-1 with its ring exchanged
+    prev = oracle.gol_u8(a.copy(), np.zeros_like(a), nx, ny, gens - 1)
+    assert np.array_equal((d0, d1)[(gens + 1) % 2].download(), prev)
+
+
+def test_gol_full_config_prefix(ctx):
+    # 8192^2 config: the first 12 generations against the oracle bit for bit
+    nx = ny = 8192
+    a, d0, d1 = _gol_pair(ctx, nx, ny)
+    ctx.gol(d0, d1, nx, ny, 12, hip.PLAIN)
+    ref = oracle.gol_u8(a.copy(), np.zeros_like(a), nx, ny, 12)
+    assert np.array_equal(d0.download(), ref)
+
+
+def _fdtd_pair(ctx, n, dt, seed=1004):
+    npdt = {hip.F32: np.float32, hip.F64: np.float64, hip.F16: np.float16, hip.BF16: np.uint16}[dt]
+    shape = tuple(x + 8 for x in n[::-1])
+    a = np.zeros(shape, dtype=npdt)
+    oracle.fill_uniform(a, 4, seed, bf16=(dt == hip.BF16))
+    return a, ctx.from_numpy(a, dt), ctx.array(shape, dt)
+
+
+@pytest.mark.parametrize("dt", [hip.F32, hip.F64, hip.F16, hip.BF16])
+@pytest.mark.parametrize("n,steps", [((37, 29, 23), 4), ((64, 64, 64), 3), ((70, 17, 130), 2)])
+def test_fdtd3d_bit_exact(ctx, dt, n, steps):
+    nx, ny, nz = n
+    coeff = FDTD_COEFF if dt not in (hip.F16,) else [c / 2.0625 for c in FDTD_COEFF]
+    a, d0, d1 = _fdtd_pair(ctx, n, dt)
+    ctx.fdtd3d(d0, d1, nx, ny, nz, coeff, steps)
+    ref = oracle.fdtd3d(a.copy(), np.zeros_like(a), nx, ny, nz, coeff, steps, bf16=(dt == hip.BF16))
+    got = (d0, d1)[steps % 2].download()
+    assert np.array_equal(got.view(np.uint8), ref.view(np.uint8))
+
+
+def test_fdtd3d_full_config(ctx):
+    # 512^3 FP32: 2 sweeps bit-exact against the oracle, then the full 100 sweeps checked
+    # against the FP64 kernel on normalised fields (u / 2.0625^n)
+    n = (512, 512, 512)
+    a, d0, d1 = _fdtd_pair(ctx, n, hip.F32)
+    ctx.fdtd3d(d0, d1, *n, FDTD_COEFF, 2)
+    ref = oracle.fdtd3d(a.copy(), np.zeros_like(a), *n, FDTD_COEFF, 2)
+    assert np.array_equal(d0.download(), ref)
+    del ref
+    d0.upload(a)
+    ctx.fdtd3d(d0, d1, *n, FDTD_COEFF, 100)
+    f32 = d0.download().astype(np.float64)
+    e0 = ctx.from_numpy(a.astype(np.float64))
+    e1 = ctx.array(a.shape, hip.F64)
+    ctx.fdtd3d(e0, e1, *n, FDTD_COEFF, 100)
+    f64 = e0.download()
+    scale = 2.0625 ** 100
+    err = np.abs(f32 - f64) / scale
+    rel_l2 = np.linalg.norm((f32 - f64).ravel()) / np.linalg.norm(f64.ravel())
+    print(f"[fdtd3d] FP32 vs FP64 after 100 steps: max-abs(normalised) {err.max():.3e} rel-L2 {rel_l2:.3e}")
+    assert rel_l2 < 1e-5
