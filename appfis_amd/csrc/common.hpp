@@ -20,6 +20,10 @@ struct appfis_ctx {
     // device scratch reused across calls (reductions, flood scalars); grown on demand
     void* scratch = nullptr;
     size_t scratch_bytes = 0;
+    // large device workspace of the chosen forms (mixed-precision copies, flood
+    // intermediates); grown on demand, kept until the context is destroyed
+    void* work = nullptr;
+    size_t work_bytes = 0;
     // pinned host staging for small results
     void* pinned = nullptr;
     size_t pinned_bytes = 0;
@@ -34,6 +38,7 @@ int invalid(const char* fmt, ...);
 
 // Ensure ctx->scratch holds at least `bytes`; returns status.
 int ensure_scratch(appfis_ctx* ctx, size_t bytes);
+int ensure_work(appfis_ctx* ctx, size_t bytes);
 int ensure_pinned(appfis_ctx* ctx, size_t bytes);
 
 inline size_t dtype_size(int dtype)

@@ -52,6 +52,20 @@ int ensure_scratch(appfis_ctx* ctx, size_t bytes)
     return APPFIS_OK;
 }
 
+int ensure_work(appfis_ctx* ctx, size_t bytes)
+{
+    if (ctx->work_bytes >= bytes) return APPFIS_OK;
+    if (ctx->work) {
+        APPFIS_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+        APPFIS_HIP_CHECK(hipFree(ctx->work));
+        ctx->work = nullptr;
+        ctx->work_bytes = 0;
+    }
+    APPFIS_HIP_CHECK(hipMalloc(&ctx->work, bytes));
+    ctx->work_bytes = bytes;
+    return APPFIS_OK;
+}
+
 int ensure_pinned(appfis_ctx* ctx, size_t bytes)
 {
     if (ctx->pinned_bytes >= bytes) return APPFIS_OK;
@@ -141,14 +155,21 @@ int appfis_ctx_create(int device, appfis_ctx** out)
 int appfis_ctx_destroy(appfis_ctx* ctx)
 {
     if (!ctx) return APPFIS_OK;
-    hipSetDevice(ctx->device);
-    if (ctx->stream) hipStreamSynchronize(ctx->stream);
-    if (ctx->scratch) hipFree(ctx->scratch);
-    if (ctx->pinned) hipHostFree(ctx->pinned);
-    if (ctx->ev_start) hipEventDestroy(ctx->ev_start);
-    if (ctx->ev_stop) hipEventDestroy(ctx->ev_stop);
-    if (ctx->stream) hipStreamDestroy(ctx->stream);
+    // best-effort teardown: report the first failure but release everything
+    hipError_t first = hipSuccess;
+    auto note = [&](hipError_t e) {
+        if (first == hipSuccess && e != hipSuccess) first = e;
+    };
+    note(hipSetDevice(ctx->device));
+    if (ctx->stream) note(hipStreamSynchronize(ctx->stream));
+    if (ctx->scratch) note(hipFree(ctx->scratch));
+    if (ctx->work) note(hipFree(ctx->work));
+    if (ctx->pinned) note(hipHostFree(ctx->pinned));
+    if (ctx->ev_start) note(hipEventDestroy(ctx->ev_start));
+    if (ctx->ev_stop) note(hipEventDestroy(ctx->ev_stop));
+    if (ctx->stream) note(hipStreamDestroy(ctx->stream));
     delete ctx;
+    if (first != hipSuccess) return hip_fail(first, "context teardown", __FILE__, __LINE__);
     return APPFIS_OK;
 }
 
@@ -257,7 +278,8 @@ int appfis_graph_end(appfis_ctx* ctx, void** graph_exec)
     APPFIS_HIP_CHECK(hipStreamEndCapture(ctx->stream, &g));
     hipGraphExec_t ge = nullptr;
     hipError_t e = hipGraphInstantiate(&ge, g, nullptr, nullptr, 0);
-    hipGraphDestroy(g);
+    hipError_t e2 = hipGraphDestroy(g);
+    if (e == hipSuccess) e = e2;
     if (e != hipSuccess) return hip_fail(e, "hipGraphInstantiate", __FILE__, __LINE__);
     *graph_exec = (void*)ge;
     return APPFIS_OK;

@@ -1,25 +1,688 @@
-// Triton flood step chain.
+// Triton shallow-water flood model: the step chain of Triton/src/triton_appfis.hpp:
+// 316-386 with the functors of triton_appfis.hpp:567-1112, plus the config's uniform
+// rainfall (added as h += rain*dt right after the update, where the reference adds its
+// point sources), fused into two kernels per step:
+//
+//   K1 (flux + update + rain): one thread per interior cell evaluates the Roe flux of
+//       its four edges (X edges (i,i+1) and (i-1,i), Y edges (j,j-1) and (j+1,j)),
+//       accumulates rhs*0 / rhs*1 in the order of the reference's X-then-Y functors,
+//       applies the update with implicit Manning friction and the rain, and writes
+//       the new state to the other buffer of a ping-pong pair (the reference's reset
+//       pass and six rhs arrays never touch memory).
+//   K2 (wet/dry fix + CFL + volume): the momentum fix of triton_appfis.hpp:1095-1112 in
+//       place, then the CFL candidate of triton_appfis.hpp:567-586 from the fixed state
+//       (wave/block min, one atomicMin per block) and a compensated FP64 sum of h.  The
+//       last block of a scenario (atomic ticket) finalises on the device what the
+//       reference does on the host (triton_appfis.hpp:322-335, 381-385): simtime += dt,
+//       the print_interval bookkeeping, the next dt (fallback when every cell is dry,
+//       clamp to the next print boundary), and records time / volume / dt of the step.
+//
+// No host round trip per step: the reference's Execute2DReduce(MIN) + D2H + 3 host
+// syncs per step become device-side state, so a chunk of steps can be captured in a
+// hipGraph and replayed.  Ensembles: every phase is one launch over a tile work-list
+// spanning all scenarios (64 x 4 cells per 256-thread block, scenario found by binary
+// search over tile offsets), with per-scenario dt and reductions.
+//
+// REAL = double (reference dtype) or float (FP32 bulk form; the volume is still a
+// compensated FP64 sum).  No FMA contraction: the operation order is the functors'.
 #include "common.hpp"
 
+#include <cfloat>
+#include <vector>
+
+using namespace appfis;
+
+namespace {
+
+constexpr int BX = 64, BY = 4, BT = BX * BY;
+constexpr int CHUNK = 20;  // steps per captured graph (even: buffer parity returns)
+
+template <typename R> struct RealLim;
+template <> struct RealLim<double> {
+    __host__ __device__ static double max() { return DBL_MAX; }
+};
+template <> struct RealLim<float> {
+    __host__ __device__ static float max() { return FLT_MAX; }
+};
+
+struct ScenState {
+    double dt;        // dt of the step about to run
+    double simtime;   // simulated time (s)
+    unsigned long long dtmin;  // ordered bits of the running CFL minimum (R-typed value)
+    int print_id;
+    int step;         // steps completed in this call
+    unsigned ticket;  // blocks of this scenario done in the current K2
+    int pad;
+};
+
+template <typename R>
+struct Scen {
+    R* h[2];
+    R* qx[2];
+    R* qy[2];
+    const R* z;
+    const R* n2;
+    int nx, ny;
+    int tiles_x;
+    int tile0;         // first global tile of this scenario
+    int ntiles;
+    ScenState* st;
+    double* partials;  // 2 doubles per tile (compensated volume partial)
+    double* rec_time;  // per-step records (nsteps each) or null
+    double* rec_vol;
+    double* rec_dt;
+};
+
+struct Params {
+    double dx, cn, hextra, dt_fallback, print_interval, rain;
+    int nsteps_rec;  // length of the record arrays
+};
+
+template <typename R> __device__ inline unsigned long long min_bits(R v);
+template <> __device__ inline unsigned long long min_bits<double>(double v)
+{
+    return (unsigned long long)__double_as_longlong(v);
+}
+template <> __device__ inline unsigned long long min_bits<float>(float v) { return (unsigned long long)__float_as_uint(v); }
+template <typename R> __device__ inline R from_bits(unsigned long long b);
+template <> __device__ inline double from_bits<double>(unsigned long long b) { return __longlong_as_double((long long)b); }
+template <> __device__ inline float from_bits<float>(unsigned long long b) { return __uint_as_float((unsigned)b); }
+
+// Roe flux with entropy fix, bed-slope source and wet/dry treatment -- the shared body
+// of the X functor (triton_appfis.hpp:608-824) and the Y functor (827-1043).
+template <typename R>
+__device__ inline bool roe(R h1, R h2, R qx1, R qy1, R z1, R qx2, R qy2, R z2, R sqh1, R sqh2, R nx_, R ny_, R dx,
+                           R dt, R hextra, R d1[3], R d2[3])
+{
+    const R G = (R)9.81, SQRTG = (R)3.132091953, EPS12 = (R)1e-12;
+    if (!(h1 > (R)0.0 || h2 > (R)0.0)) return false;
+    R alpha[3], beta[3], eigen[3], eigenE[3], eigev[3][3];
+    R u1, u2, v1, v2;
+    if (h1 >= hextra) { u1 = qx1 / h1; v1 = qy1 / h1; }
+    else { u1 = (R)0.0; v1 = (R)0.0; }
+    if (h2 >= hextra) { u2 = qx2 / h2; v2 = qy2 / h2; }
+    else { u2 = (R)0.0; v2 = (R)0.0; }
+    const R h12 = (R)0.5 * (h1 + h2);
+    const R dh = h2 - h1;
+    const R c12 = sqrt(G * h12);
+    const R u12 = (u1 * sqh1 + u2 * sqh2) / (sqh1 + sqh2);
+    const R v12 = (v1 * sqh1 + v2 * sqh2) / (sqh1 + sqh2);
+    const R un = u12 * nx_ + v12 * ny_;
+    eigen[0] = un - c12;
+    eigen[1] = un;
+    eigen[2] = un + c12;
+#pragma unroll
+    for (int i = 0; i < 3; i++) eigenE[i] = (R)0.0;
+    R e1 = u1 * nx_ + v1 * ny_ - SQRTG * sqh1;
+    R e2 = u2 * nx_ + v2 * ny_ - SQRTG * sqh2;
+    if (e1 < (R)0.0 && e2 > (R)0.0) {
+        eigenE[0] = eigen[0] - e1 * (e2 - eigen[0]) / (e2 - e1);
+        eigen[0] = e1 * (e2 - eigen[0]) / (e2 - e1);
+    }
+    e1 = u1 * nx_ + v1 * ny_ + SQRTG * sqh1;
+    e2 = u2 * nx_ + v2 * ny_ + SQRTG * sqh2;
+    if (e1 < (R)0.0 && e2 > (R)0.0) {
+        eigenE[2] = eigen[2] - e2 * (eigen[2] - e1) / (e2 - e1);
+        eigen[2] = e2 * (eigen[2] - e1) / (e2 - e1);
+    }
+    eigev[0][0] = (R)1.0;
+    eigev[0][1] = u12 - c12 * nx_;
+    eigev[0][2] = v12 - c12 * ny_;
+    eigev[1][0] = (R)0.0;
+    eigev[1][1] = -c12 * ny_;
+    eigev[1][2] = c12 * nx_;
+    eigev[2][0] = (R)1.0;
+    eigev[2][1] = u12 + c12 * nx_;
+    eigev[2][2] = v12 + c12 * ny_;
+    alpha[0] = (R)0.5 * (dh - (((qx2 - qx1) * nx_ + (qy2 - qy1) * ny_) - un * dh) / c12);
+    alpha[1] = (((qy2 - qy1) - v12 * dh) * nx_ - ((qx2 - qx1) - u12 * dh) * ny_) / c12;
+    alpha[2] = (R)0.5 * (dh + (((qx2 - qx1) * nx_ + (qy2 - qy1) * ny_) - un * dh) / c12);
+    const R dz = z2 - z1;
+    R l1 = z1 + h1;
+    R l2 = z2 + h2;
+    R dzp = dz;
+    R hp;
+    if (dz >= (R)0.0) {
+        hp = h1;
+        if (l1 < z2) dzp = h1;
+    } else {
+        hp = h2;
+        if (l2 < z1) dzp = -h2;
+    }
+    beta[0] = (R)0.5 * G * (hp - (R)0.5 * fabs(dzp)) * dzp / c12;
+    hp = h1 + alpha[0];
+    if (eigen[0] * eigen[2] < (R)0.0 && hp > (R)0.0 && h1 > (R)0.0 && h2 > (R)0.0) {
+        beta[0] = fmax(beta[0], alpha[0] * eigen[0] - h1 * dx * (R)0.5 / dt);
+        beta[0] = fmin(beta[0], -alpha[2] * eigen[2] + h2 * dx * (R)0.5 / dt);
+    }
+    beta[1] = (R)0.0;
+    beta[2] = -beta[0];
+#pragma unroll
+    for (int i = 0; i < 3; i++) { d1[i] = (R)0.0; d2[i] = (R)0.0; }
+    l1 = hp - beta[0] / eigen[0];
+    l2 = hp + beta[2] / eigen[2];
+    if (l2 < -EPS12 && h2 < EPS12) {
+        d1[0] += (eigen[0] * alpha[0] - beta[0]) + (eigen[2] * alpha[2] - beta[2]);
+    } else if (l1 < -EPS12 && h1 < EPS12) {
+        d2[0] += (eigen[0] * alpha[0] - beta[0]) + (eigen[2] * alpha[2] - beta[2]);
+    } else {
+#pragma unroll
+        for (int i = 0; i < 3; i++) {
+            if (eigen[i] > (R)0.0) {
+                d2[0] += (eigen[i] * alpha[i] - beta[i]) * eigev[i][0];
+                d2[1] += (eigen[i] * alpha[i] - beta[i]) * eigev[i][1];
+                d2[2] += (eigen[i] * alpha[i] - beta[i]) * eigev[i][2];
+            } else {
+                d1[0] += (eigen[i] * alpha[i] - beta[i]) * eigev[i][0];
+                d1[1] += (eigen[i] * alpha[i] - beta[i]) * eigev[i][1];
+                d1[2] += (eigen[i] * alpha[i] - beta[i]) * eigev[i][2];
+            }
+        }
+    }
+#pragma unroll
+    for (int i = 0; i < 3; i++) {
+        if (eigenE[i] > (R)0.0) {
+            d2[0] += (eigenE[i] * alpha[i]) * eigev[i][0];
+            d2[1] += (eigenE[i] * alpha[i]) * eigev[i][1];
+            d2[2] += (eigenE[i] * alpha[i]) * eigev[i][2];
+        } else {
+            d1[0] += (eigenE[i] * alpha[i]) * eigev[i][0];
+            d1[1] += (eigenE[i] * alpha[i]) * eigev[i][1];
+            d1[2] += (eigenE[i] * alpha[i]) * eigev[i][2];
+        }
+    }
+#pragma unroll
+    for (int i = 0; i < 3; i++) {
+        if (fabs(d1[i]) < EPS12) d1[i] = (R)0.0;
+        if (fabs(d2[i]) < EPS12) d2[i] = (R)0.0;
+    }
+    return true;
+}
+
+template <typename R> __device__ inline R sqrt_pos(R h) { return h > (R)0.0 ? sqrt(h) : (R)0.0; }
+
+// scenario of a global tile: binary search over tile0 (uniform per block)
+template <typename R>
+__device__ inline int find_scen(const Scen<R>* sc, int nscen, int tile)
+{
+    int lo = 0, hi = nscen - 1;
+    while (lo < hi) {
+        int mid = (lo + hi + 1) >> 1;
+        if (sc[mid].tile0 <= tile) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
+template <typename R>
+__global__ __launch_bounds__(BT) void k_flux_update(const Scen<R>* __restrict__ sc, int nscen, int par, Params P)
+{
+    const int tile = blockIdx.x;
+    const int s = find_scen(sc, nscen, tile);
+    const Scen<R> S = sc[s];
+    const int t = tile - S.tile0;
+    const int tyi = t / S.tiles_x, txi = t - tyi * S.tiles_x;
+    const int i = txi * BX + (threadIdx.x & (BX - 1)) + 1;
+    const int j = tyi * BY + (threadIdx.x >> 6) + 1;
+    if (i > S.nx || j > S.ny) return;
+    const int W = S.nx + 2;
+    const R* h = S.h[par];
+    const R* qx = S.qx[par];
+    const R* qy = S.qy[par];
+    const R* z = S.z;
+    const R dt = (R)S.st->dt;
+    const R dx = (R)P.dx, hextra = (R)P.hextra, rain = (R)P.rain;
+    const size_t c = (size_t)i + (size_t)W * (size_t)j;
+    const size_t e = c + 1, w = c - 1, dn = c - W, up = c + W;  // up = j+1, dn = j-1
+    const R hc = h[c], he = h[e], hw = h[w], hd = h[dn], hu = h[up];
+    const R qxc = qx[c], qyc = qy[c], zc = z[c];
+    const R sc_ = sqrt_pos(hc);
+    R rh0 = (R)0.0, rqx0 = (R)0.0, rqy0 = (R)0.0, rh1 = (R)0.0, rqx1 = (R)0.0, rqy1 = (R)0.0;
+    R d1[3], d2[3];
+    // X edge (i, i+1): this cell is side 1 -> rhs*0 (overwrite, triton_appfis.hpp:817-819)
+    if (roe(hc, he, qxc, qyc, zc, qx[e], qy[e], z[e], sc_, sqrt_pos(he), (R)1.0, (R)0.0, dx, dt, hextra, d1, d2)) {
+        rh0 = dt * d1[0] / dx;
+        rqx0 = dt * d1[1] / dx;
+        rqy0 = dt * d1[2] / dx;
+    }
+    // X edge (i-1, i): this cell is side 2 -> rhs*1 (overwrite, triton_appfis.hpp:821-823)
+    if (roe(hw, hc, qx[w], qy[w], z[w], qxc, qyc, zc, sqrt_pos(hw), sc_, (R)1.0, (R)0.0, dx, dt, hextra, d1, d2)) {
+        rh1 = dt * d2[0] / dx;
+        rqx1 = dt * d2[1] / dx;
+        rqy1 = dt * d2[2] / dx;
+    }
+    // Y edge jj = j (id1 = j, id2 = j-1): this cell is side 1 -> rhs*0 += (1034-1036)
+    if (roe(hc, hd, qxc, qyc, zc, qx[dn], qy[dn], z[dn], sc_, sqrt_pos(hd), (R)0.0, (R)1.0, dx, dt, hextra, d1, d2)) {
+        rh0 = rh0 + dt * d1[0] / dx;
+        rqx0 = rqx0 + dt * d1[1] / dx;
+        rqy0 = rqy0 + dt * d1[2] / dx;
+    }
+    // Y edge jj = j+1 (id1 = j+1, id2 = j): this cell is side 2 -> rhs*1 += (1038-1040)
+    if (roe(hu, hc, qx[up], qy[up], z[up], qxc, qyc, zc, sqrt_pos(hu), sc_, (R)0.0, (R)1.0, dx, dt, hextra, d1, d2)) {
+        rh1 = rh1 + dt * d2[0] / dx;
+        rqx1 = rqx1 + dt * d2[1] / dx;
+        rqy1 = rqy1 + dt * d2[2] / dx;
+    }
+    // update with implicit Manning friction (triton_appfis.hpp:1046-1092)
+    const R G = (R)9.81, EPS12 = (R)1e-12;
+    R hij, qxij, qyij;
+    const R hn = hc;
+    hij = hn - rh0 - rh1;
+    if (hij < (R)0.0) hij = (R)0.0;
+    if (hij < hextra) {
+        qxij = (R)0.0;
+        qyij = (R)0.0;
+    } else {
+        const R mx = qxc - rqx0 - rqx1;
+        const R my = qyc - rqy0 - rqy1;
+        const R modM = sqrt(mx * mx + my * my);
+        const R nn = S.n2[c];
+        if (nn > EPS12 && hn >= hextra && modM > EPS12) {
+            const R tt = dt * G * nn * modM / (hn * hn * cbrt(hn));
+            qxij = (R)-0.5 * (mx - mx * sqrt((R)1.0 + (R)4.0 * tt)) / tt;
+            qyij = (R)-0.5 * (my - my * sqrt((R)1.0 + (R)4.0 * tt)) / tt;
+        } else {
+            qxij = mx;
+            qyij = my;
+        }
+    }
+    const int o = par ^ 1;
+    S.h[o][c] = hij + rain * dt;
+    S.qx[o][c] = qxij;
+    S.qy[o][c] = qyij;
+}
+
+__device__ inline void two_sum(double& s, double& cmp, double x)
+{
+    double t = __dadd_rn(s, x);
+    double bp = __dsub_rn(t, s);
+    cmp = __dadd_rn(cmp, __dadd_rn(__dsub_rn(s, __dsub_rn(t, bp)), __dsub_rn(x, bp)));
+    s = t;
+}
+
+// MODE 0: initial CFL only (no fix, no time advance, no record); MODE 1: full step end
+template <typename R, int MODE>
+__global__ __launch_bounds__(BT) void k_fix_cfl(Scen<R>* __restrict__ sc, int nscen, int par, Params P)
+{
+    __shared__ R smin[BT / 64];
+    __shared__ double ssum[BT / 64][2];
+    __shared__ bool last;
+    const int tile = blockIdx.x;
+    const int s = find_scen(sc, nscen, tile);
+    const Scen<R> S = sc[s];
+    const int t = tile - S.tile0;
+    const int tyi = t / S.tiles_x, txi = t - tyi * S.tiles_x;
+    const int i = txi * BX + (threadIdx.x & (BX - 1)) + 1;
+    const int j = tyi * BY + (threadIdx.x >> 6) + 1;
+    const R G = (R)9.81, EPS12 = (R)1e-12;
+    const R hextra = (R)P.hextra, dx = (R)P.dx, cn = (R)P.cn;
+    R cfl = RealLim<R>::max();
+    double vs = 0.0, vc = 0.0;
+    if (i <= S.nx && j <= S.ny) {
+        const int W = S.nx + 2;
+        const size_t c = (size_t)i + (size_t)W * (size_t)j;
+        R* h = S.h[par];
+        R* qx = S.qx[par];
+        R* qy = S.qy[par];
+        const R* z = S.z;
+        const R hij = h[c];
+        R qxv = qx[c], qyv = qy[c];
+        if (MODE == 1 && hij > hextra) {
+            const R zij = z[c];
+            if (((hij + zij < z[c + 1]) && (h[c + 1] < EPS12)) || ((hij + zij < z[c - 1]) && (h[c - 1] < EPS12))) {
+                qxv = (R)0.0;
+                qx[c] = qxv;
+            }
+            if (((hij + zij < z[c - W]) && (h[c - W] < EPS12)) || ((hij + zij < z[c + W]) && (h[c + W] < EPS12))) {
+                qyv = (R)0.0;
+                qy[c] = qyv;
+            }
+        }
+        if (hij > hextra) {
+            const R dtx = (dx / (fabs(qxv / hij) + sqrt(G * hij)));
+            const R dty = (dx / (fabs(qyv / hij) + sqrt(G * hij)));
+            cfl = dtx < dty ? cn * dtx : cn * dty;
+        }
+        vs = (double)hij;
+    }
+    // block min of the CFL candidates, compensated block sum of h
+    for (int o = 32; o > 0; o >>= 1) {
+        R u = __shfl_xor(cfl, o, 64);
+        cfl = u < cfl ? u : cfl;
+        double us = __shfl_xor(vs, o, 64), uc = __shfl_xor(vc, o, 64);
+        two_sum(vs, vc, us);
+        vc = __dadd_rn(vc, uc);
+    }
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    if (lane == 0) {
+        smin[wid] = cfl;
+        ssum[wid][0] = vs;
+        ssum[wid][1] = vc;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        R m = smin[0];
+        double bs = ssum[0][0], bc = ssum[0][1];
+        for (int w = 1; w < BT / 64; w++) {
+            m = smin[w] < m ? smin[w] : m;
+            two_sum(bs, bc, ssum[w][0]);
+            bc = __dadd_rn(bc, ssum[w][1]);
+        }
+        if (m < RealLim<R>::max()) atomicMin(&S.st->dtmin, min_bits<R>(m));
+        S.partials[2 * t] = bs;
+        S.partials[2 * t + 1] = bc;
+        __threadfence();
+        unsigned prev = atomicAdd(&S.st->ticket, 1u);
+        last = (prev == (unsigned)S.ntiles - 1);
+    }
+    __syncthreads();
+    if (!last) return;
+    __threadfence();
+    // last block of this scenario: volume, dt bookkeeping, records
+    double as = 0.0, ac = 0.0;
+    for (int p = threadIdx.x; p < S.ntiles; p += BT) {
+        two_sum(as, ac, __hip_atomic_load(S.partials + 2 * p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+        ac = __dadd_rn(ac, __hip_atomic_load(S.partials + 2 * p + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+    }
+    for (int o = 32; o > 0; o >>= 1) {
+        double us = __shfl_xor(as, o, 64), uc = __shfl_xor(ac, o, 64);
+        two_sum(as, ac, us);
+        ac = __dadd_rn(ac, uc);
+    }
+    __syncthreads();
+    if (lane == 0) {
+        ssum[wid][0] = as;
+        ssum[wid][1] = ac;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double bs = ssum[0][0], bc = ssum[0][1];
+        for (int w = 1; w < BT / 64; w++) {
+            two_sum(bs, bc, ssum[w][0]);
+            bc = __dadd_rn(bc, ssum[w][1]);
+        }
+        ScenState* st = S.st;
+        const unsigned long long mb = __hip_atomic_load(&st->dtmin, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        const double PI = P.print_interval;
+        double simtime = st->simtime;
+        int pid = st->print_id;
+        if (MODE == 1) {
+            const double dtn = st->dt;
+            simtime += dtn;
+            if (PI > 0.0 && simtime >= PI * (pid + 1)) pid++;
+            const int stp = st->step;
+            if (stp < P.nsteps_rec) {
+                const double vol = __dmul_rn(__dmul_rn(__dadd_rn(bs, bc), P.dx), P.dx);
+                if (S.rec_time) S.rec_time[stp] = simtime;
+                if (S.rec_vol) S.rec_vol[stp] = vol;
+                if (S.rec_dt) S.rec_dt[stp] = dtn;
+            }
+            st->step = stp + 1;
+            st->simtime = simtime;
+            st->print_id = pid;
+        }
+        // next dt (triton_appfis.hpp:322-335)
+        const R m = from_bits<R>(mb);
+        double gdt = (double)m;
+        if (gdt >= (double)RealLim<R>::max() - 1.0) gdt = P.dt_fallback;
+        if (PI > 0.0 && simtime + gdt > PI * (pid + 1)) gdt = PI * (pid + 1) - simtime;
+        st->dt = (double)(R)gdt;
+        st->dtmin = min_bits<R>(RealLim<R>::max());
+        st->ticket = 0u;
+    }
+}
+
+__global__ void k_init_state(ScenState* st, int n, double start_time, unsigned long long maxbits)
+{
+    int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= n) return;
+    st[s].dt = 0.0;
+    st[s].simtime = start_time;
+    st[s].dtmin = maxbits;
+    st[s].print_id = 0;
+    st[s].step = 0;
+    st[s].ticket = 0u;
+    st[s].pad = 0;
+}
+
+template <typename R>
+int run_flood(appfis_ctx* ctx, int nscen, const int* nx, const int* ny, void* const* h, void* const* qx,
+              void* const* qy, const void* const* z, const void* const* n2, const appfis_flood_params* prm, int steps,
+              bool graph, double* time_host, double* volume_host, double* dt_host)
+{
+    for (int s = 0; s < nscen; s++) {
+        APPFIS_REQUIRE(nx[s] >= 1 && ny[s] >= 1, "scenario %d: empty interior", s);
+        APPFIS_REQUIRE((double)(nx[s] + 2) * (ny[s] + 2) < 2147483647.0, "scenario %d exceeds the int index range", s);
+        APPFIS_REQUIRE(h[s] && qx[s] && qy[s] && z[s] && n2[s], "scenario %d: NULL field", s);
+    }
+    APPFIS_REQUIRE(prm->dx > 0.0, "dx must be positive");
+    // tile work-list
+    std::vector<Scen<R>> hs((size_t)nscen);
+    long long total_tiles = 0;
+    size_t field_bytes = 0;
+    for (int s = 0; s < nscen; s++) {
+        int tx = (int)ceil_div(nx[s], BX), ty = (int)ceil_div(ny[s], BY);
+        hs[s].nx = nx[s];
+        hs[s].ny = ny[s];
+        hs[s].tiles_x = tx;
+        hs[s].tile0 = (int)total_tiles;
+        hs[s].ntiles = tx * ty;
+        total_tiles += (long long)tx * ty;
+        field_bytes += ((size_t)(nx[s] + 2) * (ny[s] + 2) * sizeof(R) + 255) / 256 * 256;
+    }
+    APPFIS_REQUIRE(total_tiles < (1LL << 31), "too many tiles");
+    const int rec = steps > 0 ? steps : 1;
+    // workspace: second buffers (h, qx, qy) | states | partials | records | descriptor table
+    const size_t st_bytes = ((size_t)nscen * sizeof(ScenState) + 255) / 256 * 256;
+    const size_t part_bytes = ((size_t)total_tiles * 16 + 255) / 256 * 256;
+    const size_t rec_bytes = ((size_t)nscen * rec * 8 + 255) / 256 * 256;
+    const size_t tab_bytes = ((size_t)nscen * sizeof(Scen<R>) + 255) / 256 * 256;
+    int stt = ensure_work(ctx, 3 * field_bytes + st_bytes + part_bytes + 3 * rec_bytes + tab_bytes);
+    if (stt) return stt;
+    char* w = (char*)ctx->work;
+    size_t off = 0;
+    for (int s = 0; s < nscen; s++) {
+        const size_t fb = (size_t)(nx[s] + 2) * (ny[s] + 2) * sizeof(R);
+        const size_t fa = (fb + 255) / 256 * 256;
+        hs[s].h[0] = (R*)h[s];
+        hs[s].qx[0] = (R*)qx[s];
+        hs[s].qy[0] = (R*)qy[s];
+        hs[s].h[1] = (R*)(w + off);
+        hs[s].qx[1] = (R*)(w + off + field_bytes);
+        hs[s].qy[1] = (R*)(w + off + 2 * field_bytes);
+        off += fa;
+        // the second buffers start as full copies (ghost cells included)
+        APPFIS_HIP_CHECK(hipMemcpyAsync(hs[s].h[1], h[s], fb, hipMemcpyDeviceToDevice, ctx->stream));
+        APPFIS_HIP_CHECK(hipMemcpyAsync(hs[s].qx[1], qx[s], fb, hipMemcpyDeviceToDevice, ctx->stream));
+        APPFIS_HIP_CHECK(hipMemcpyAsync(hs[s].qy[1], qy[s], fb, hipMemcpyDeviceToDevice, ctx->stream));
+        hs[s].z = (const R*)z[s];
+        hs[s].n2 = (const R*)n2[s];
+    }
+    char* base = w + 3 * field_bytes;
+    ScenState* st = (ScenState*)base;
+    double* partials = (double*)(base + st_bytes);
+    double* rt = (double*)(base + st_bytes + part_bytes);
+    double* rv = (double*)(base + st_bytes + part_bytes + rec_bytes);
+    double* rd = (double*)(base + st_bytes + part_bytes + 2 * rec_bytes);
+    Scen<R>* tab = (Scen<R>*)(base + st_bytes + part_bytes + 3 * rec_bytes);
+    for (int s = 0; s < nscen; s++) {
+        hs[s].st = st + s;
+        hs[s].partials = partials + 2 * (size_t)hs[s].tile0;
+        hs[s].rec_time = rt + (size_t)s * rec;
+        hs[s].rec_vol = rv + (size_t)s * rec;
+        hs[s].rec_dt = rd + (size_t)s * rec;
+    }
+    APPFIS_HIP_CHECK(hipMemcpyAsync(tab, hs.data(), (size_t)nscen * sizeof(Scen<R>), hipMemcpyHostToDevice, ctx->stream));
+    APPFIS_HIP_CHECK(hipStreamSynchronize(ctx->stream));  // the host table is a local
+    Params P{prm->dx, prm->courant, prm->hextra, prm->dt_fallback, prm->print_interval, prm->rain_rate, rec};
+    unsigned long long maxbits;
+    if (sizeof(R) == 8) {
+        double m = DBL_MAX;
+        memcpy(&maxbits, &m, 8);
+    } else {
+        float m = FLT_MAX;
+        unsigned u;
+        memcpy(&u, &m, 4);
+        maxbits = u;
+    }
+    hipLaunchKernelGGL(k_init_state, dim3(ceil_div(nscen, 64)), dim3(64), 0, ctx->stream, st, nscen, prm->start_time,
+                       maxbits);
+    APPFIS_LAUNCH_CHECK();
+    const dim3 grid((unsigned)total_tiles);
+    // dt of the first step from the initial state
+    hipLaunchKernelGGL((k_fix_cfl<R, 0>), grid, dim3(BT), 0, ctx->stream, tab, nscen, 0, P);
+    APPFIS_LAUNCH_CHECK();
+    auto step_pair = [&](int par) -> int {
+        hipLaunchKernelGGL((k_flux_update<R>), grid, dim3(BT), 0, ctx->stream, tab, nscen, par, P);
+        APPFIS_LAUNCH_CHECK();
+        hipLaunchKernelGGL((k_fix_cfl<R, 1>), grid, dim3(BT), 0, ctx->stream, tab, nscen, par ^ 1, P);
+        APPFIS_LAUNCH_CHECK();
+        return APPFIS_OK;
+    };
+    int done = 0;
+    if (graph && steps >= 2 * CHUNK) {
+        APPFIS_HIP_CHECK(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+        int cst = APPFIS_OK;
+        for (int k = 0; k < CHUNK && cst == APPFIS_OK; k++) cst = step_pair(k & 1);
+        hipGraph_t g = nullptr;
+        hipError_t e = hipStreamEndCapture(ctx->stream, &g);
+        if (cst != APPFIS_OK) return cst;
+        if (e != hipSuccess) return hip_fail(e, "flood graph capture", __FILE__, __LINE__);
+        hipGraphExec_t ge = nullptr;
+        e = hipGraphInstantiate(&ge, g, nullptr, nullptr, 0);
+        hipError_t e2 = hipGraphDestroy(g);
+        if (e != hipSuccess) return hip_fail(e, "flood graph instantiate", __FILE__, __LINE__);
+        if (e2 != hipSuccess) return hip_fail(e2, "flood graph destroy", __FILE__, __LINE__);
+        const int reps = steps / CHUNK;
+        for (int r = 0; r < reps; r++) {
+            e = hipGraphLaunch(ge, ctx->stream);
+            if (e != hipSuccess) {
+                (void)hipGraphExecDestroy(ge);
+                return hip_fail(e, "flood graph launch", __FILE__, __LINE__);
+            }
+        }
+        APPFIS_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+        APPFIS_HIP_CHECK(hipGraphExecDestroy(ge));
+        done = reps * CHUNK;
+    }
+    for (int k = done; k < steps; k++) {
+        int cst = step_pair(k & 1);
+        if (cst) return cst;
+    }
+    // result in buffer steps % 2: bring it home when that is the work copy
+    if (steps & 1) {
+        for (int s = 0; s < nscen; s++) {
+            const size_t fb = (size_t)(nx[s] + 2) * (ny[s] + 2) * sizeof(R);
+            APPFIS_HIP_CHECK(hipMemcpyAsync(h[s], hs[s].h[1], fb, hipMemcpyDeviceToDevice, ctx->stream));
+            APPFIS_HIP_CHECK(hipMemcpyAsync(qx[s], hs[s].qx[1], fb, hipMemcpyDeviceToDevice, ctx->stream));
+            APPFIS_HIP_CHECK(hipMemcpyAsync(qy[s], hs[s].qy[1], fb, hipMemcpyDeviceToDevice, ctx->stream));
+        }
+    }
+    if (steps > 0) {
+        const size_t rb = (size_t)nscen * rec * 8;
+        if (time_host) APPFIS_HIP_CHECK(hipMemcpyAsync(time_host, rt, rb, hipMemcpyDeviceToHost, ctx->stream));
+        if (volume_host) APPFIS_HIP_CHECK(hipMemcpyAsync(volume_host, rv, rb, hipMemcpyDeviceToHost, ctx->stream));
+        if (dt_host) APPFIS_HIP_CHECK(hipMemcpyAsync(dt_host, rd, rb, hipMemcpyDeviceToHost, ctx->stream));
+        if (time_host || volume_host || dt_host) APPFIS_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+    }
+    return APPFIS_OK;
+}
+
+// Synthetic DEM (see include/appfis_hip.h): one thread per padded cell.
+template <typename R>
+__global__ void k_dem(R* z, int nx, int ny, double dx, const double* hills, int nhills, double slope, double wall)
+{
+    const int W = nx + 2, H = ny + 2;
+    size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (size_t)W * H) return;
+    const int j = (int)(t / (size_t)W), i = (int)(t - (size_t)j * W);
+    if (i == 0 || j == 0 || i == W - 1 || j == H - 1) {
+        z[t] = (R)wall;
+        return;
+    }
+    const double x = ((double)i - 0.5) * dx, y = ((double)j - 0.5) * dx;
+    double v = 0.0;
+    for (int k = 0; k < nhills; k++) {
+        const double ddx = x - hills[4 * k + 2], ddy = y - hills[4 * k + 3], s = hills[4 * k + 1];
+        v = v + hills[4 * k] * exp(-(ddx * ddx + ddy * ddy) / (2.0 * s * s));
+    }
+    z[t] = (R)(v + slope * x);
+}
+
+}  // namespace
+
 extern "C" {
+
 int appfis_flood_run(appfis_ctx* ctx, int dtype, void* h, void* qx, void* qy, const void* z, const void* n2, int nx,
                      int ny, const appfis_flood_params* params, int steps, int variant, double* time_host,
                      double* volume_host)
 {
-    appfis::set_error("flood: not built yet");
-    return APPFIS_ERR_UNSUPPORTED;
+    APPFIS_REQUIRE(ctx && params, "NULL argument");
+    APPFIS_REQUIRE(steps >= 0, "steps must be >= 0");
+    APPFIS_REQUIRE(variant == APPFIS_PLAIN || variant == APPFIS_CHOSEN, "flood: unknown variant %d", variant);
+    void* hh[1] = {h};
+    void* qq[1] = {qx};
+    void* rr[1] = {qy};
+    const void* zz[1] = {z};
+    const void* nn[1] = {n2};
+    const bool graph = variant == APPFIS_CHOSEN;
+    if (dtype == APPFIS_F64)
+        return run_flood<double>(ctx, 1, &nx, &ny, hh, qq, rr, zz, nn, params, steps, graph, time_host, volume_host,
+                                 nullptr);
+    if (dtype == APPFIS_F32)
+        return run_flood<float>(ctx, 1, &nx, &ny, hh, qq, rr, zz, nn, params, steps, graph, time_host, volume_host,
+                                nullptr);
+    return invalid("flood: dtype must be F64 or F32");
 }
+
 int appfis_flood_ensemble_run(appfis_ctx* ctx, int dtype, int nscen, const int* nx, const int* ny, void* const* h,
                               void* const* qx, void* const* qy, const void* const* z, const void* const* n2,
                               const appfis_flood_params* params, int steps, double* time_host, double* volume_host)
 {
-    appfis::set_error("flood: not built yet");
-    return APPFIS_ERR_UNSUPPORTED;
+    APPFIS_REQUIRE(ctx && params && nx && ny && h && qx && qy && z && n2, "NULL argument");
+    APPFIS_REQUIRE(nscen >= 1, "nscen must be >= 1");
+    APPFIS_REQUIRE(steps >= 0, "steps must be >= 0");
+    if (dtype == APPFIS_F64)
+        return run_flood<double>(ctx, nscen, nx, ny, h, qx, qy, z, n2, params, steps, true, time_host, volume_host,
+                                 nullptr);
+    if (dtype == APPFIS_F32)
+        return run_flood<float>(ctx, nscen, nx, ny, h, qx, qy, z, n2, params, steps, true, time_host, volume_host,
+                                nullptr);
+    return invalid("flood: dtype must be F64 or F32");
 }
+
 int appfis_flood_dem(appfis_ctx* ctx, int dtype, void* z, int nx, int ny, double dx, uint64_t seed, int nhills,
                      double slope, double wall)
 {
-    appfis::set_error("flood: not built yet");
-    return APPFIS_ERR_UNSUPPORTED;
+    APPFIS_REQUIRE(ctx && z, "NULL argument");
+    APPFIS_REQUIRE(nx >= 1 && ny >= 1 && nhills >= 0 && nhills <= 4096, "bad DEM arguments");
+    std::vector<double> hills((size_t)(nhills > 0 ? nhills : 1) * 4);
+    const double L = (double)nx * dx;
+    for (int k = 0; k < nhills; k++) {
+        double u[4];
+        for (int m = 0; m < 4; m++) u[m] = (double)(rng(seed, 4 * (uint64_t)k + m) >> 11) * (1.0 / 9007199254740992.0);
+        hills[4 * k + 0] = 5.0 + 45.0 * u[0];
+        hills[4 * k + 1] = 200.0 + 1800.0 * u[1];
+        hills[4 * k + 2] = u[2] * L;
+        hills[4 * k + 3] = u[3] * L;
+    }
+    int st = ensure_scratch(ctx, 4096 + hills.size() * 8);
+    if (st) return st;
+    double* dh = (double*)((char*)ctx->scratch + 4096);
+    APPFIS_HIP_CHECK(hipMemcpyAsync(dh, hills.data(), hills.size() * 8, hipMemcpyHostToDevice, ctx->stream));
+    const size_t n = (size_t)(nx + 2) * (ny + 2);
+    if (dtype == APPFIS_F64)
+        hipLaunchKernelGGL(k_dem<double>, dim3(ceil_div(n, 256)), dim3(256), 0, ctx->stream, (double*)z, nx, ny, dx, dh,
+                           nhills, slope, wall);
+    else if (dtype == APPFIS_F32)
+        hipLaunchKernelGGL(k_dem<float>, dim3(ceil_div(n, 256)), dim3(256), 0, ctx->stream, (float*)z, nx, ny, dx, dh,
+                           nhills, slope, wall);
+    else
+        return invalid("flood_dem: dtype must be F64 or F32");
+    APPFIS_LAUNCH_CHECK();
+    // the host vector dies on return: make the upload complete first
+    APPFIS_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+    return APPFIS_OK;
 }
-}
+
+}  // extern "C"

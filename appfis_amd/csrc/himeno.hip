@@ -293,34 +293,28 @@ extern "C" int appfis_himeno(appfis_ctx* ctx, int dtype, void* const* arrays, in
         APPFIS_REQUIRE(dtype == APPFIS_F64, "himeno CHOSEN (defect correction) takes the F64 problem");
         const int K = outer > 0 ? outer : 8;
         const size_t n = (size_t)nx * ny * nz;
-        // work arrays: 11 FP32 coefficient copies, r, q (x2), B (x2 FP64)
-        std::vector<void*> work;
-        auto alloc = [&](size_t bytes) -> void* {
-            void* p = nullptr;
-            if (hipMalloc(&p, bytes) != hipSuccess) return nullptr;
-            work.push_back(p);
-            return p;
-        };
-        auto release = [&]() {
-            hipStreamSynchronize(ctx->stream);
-            for (void* p : work) hipFree(p);
-        };
+        // workspace (kept by the context across calls): 11 FP32 coefficient copies, r,
+        // q x2 (FP32) and B x2 (FP64), each 256-byte aligned
+        const size_t n4 = ((n * 4 + 255) / 256) * 256, n8 = ((n * 8 + 255) / 256) * 256;
+        st = ensure_work(ctx, 14 * n4 + 2 * n8);
+        if (st) return st;
+        char* w = (char*)ctx->work;
         float* c32[11];
-        for (int a = 0; a < 11; a++) c32[a] = (float*)alloc(n * 4);
-        float* r32 = (float*)alloc(n * 4);
-        float* q[2] = {(float*)alloc(n * 4), (float*)alloc(n * 4)};
-        double* B[2] = {(double*)alloc(n * 8), (double*)alloc(n * 8)};
-        for (void* p : work)
-            if (!p) { release(); return invalid("himeno CHOSEN: device allocation failed"); }
-        unsigned cb = (unsigned)std::min<size_t>(ceil_div(n, 256), 65536);
-        for (int a = 0; a < 11; a++)
+        for (int a = 0; a < 11; a++) c32[a] = (float*)(w + (size_t)a * n4);
+        float* r32 = (float*)(w + 11 * n4);
+        float* q[2] = {(float*)(w + 12 * n4), (float*)(w + 13 * n4)};
+        double* B[2] = {(double*)(w + 14 * n4), (double*)(w + 14 * n4 + n8)};
+        const unsigned cb = (unsigned)std::min<size_t>(ceil_div(n, 256), 65536);
+        for (int a = 0; a < 11; a++) {
             hipLaunchKernelGGL(k_f64_to_f32, dim3(cb), dim3(256), 0, ctx->stream, (const double*)arrays[2 + a], c32[a],
                                n);
-        // start: B = p (the whole array incl. shell), q = 0
-        hipMemcpyAsync(B[0], arrays[0], n * 8, hipMemcpyDeviceToDevice, ctx->stream);
-        hipMemcpyAsync(B[1], arrays[0], n * 8, hipMemcpyDeviceToDevice, ctx->stream);
-        hipMemsetAsync(q[0], 0, n * 4, ctx->stream);
-        hipMemsetAsync(q[1], 0, n * 4, ctx->stream);
+            APPFIS_LAUNCH_CHECK();
+        }
+        // start: B = p (whole array, shell included), q = 0
+        APPFIS_HIP_CHECK(hipMemcpyAsync(B[0], arrays[0], n * 8, hipMemcpyDeviceToDevice, ctx->stream));
+        APPFIS_HIP_CHECK(hipMemcpyAsync(B[1], arrays[0], n * 8, hipMemcpyDeviceToDevice, ctx->stream));
+        APPFIS_HIP_CHECK(hipMemsetAsync(q[0], 0, n * 4, ctx->stream));
+        APPFIS_HIP_CHECK(hipMemsetAsync(q[1], 0, n * 4, ctx->stream));
         Coef c32s{c32[0], c32[1], c32[2], c32[3], c32[4], c32[5], c32[6], c32[7], c32[8], c32[9], c32[10], r32};
         int bi = 0, qi = 0;
         for (int it = 0; it < iters; it++) {
@@ -332,17 +326,13 @@ extern "C" int appfis_himeno(appfis_ctx* ctx, int dtype, void* const* arrays, in
                 hipLaunchKernelGGL((k_himeno<float, 1>), grid, dim3(TX), 0, ctx->stream, q[qi], q[qi ^ 1], c32s, nx,
                                    ny, nz, (float)omega, partials, ticket, gosa_dev + it);
             }
+            APPFIS_LAUNCH_CHECK();
             qi ^= 1;
-            hipError_t e = hipGetLastError();
-            if (e != hipSuccess) { release(); return hip_fail(e, "himeno launch", __FILE__, __LINE__); }
         }
-        // final field P = B + q into p[iters % 2]; the shell is copied unchanged from p0
-        void* out = arrays[iters & 1];
-        if (out != arrays[0]) hipMemcpyAsync(out, arrays[0], n * 8, hipMemcpyDeviceToDevice, ctx->stream);
-        hipLaunchKernelGGL(k_add_correction, dim3(cb), dim3(256), 0, ctx->stream, B[bi], q[qi], (double*)out, n);
-        hipError_t e = hipGetLastError();
-        release();
-        if (e != hipSuccess) return hip_fail(e, "himeno finish", __FILE__, __LINE__);
+        // final field P = B + q (every element; the shell has q = 0) into p[iters % 2]
+        hipLaunchKernelGGL(k_add_correction, dim3(cb), dim3(256), 0, ctx->stream, B[bi], q[qi],
+                           (double*)arrays[iters & 1], n);
+        APPFIS_LAUNCH_CHECK();
     } else {
         return invalid("himeno: unknown variant %d", variant);
     }

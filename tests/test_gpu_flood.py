@@ -1,0 +1,133 @@
+"""GPU parity of the Triton flood step chain (fused flux/update/rain + fix/CFL/volume).
+
+FP64 without friction touches only + - * / sqrt, all correctly rounded on both sides,
+so h, qx, qy, the dt sequence and the simulated time are bit-exact against the oracle.
+With Manning friction the GPU's cbrt may differ from glibc's by an ulp, so FP64 is held
+to FLOOD_F64_RTOL.  The FP32 bulk form is checked against the FP64 oracle with the
+stated FLOOD_F32_H_RTOL (relative L2 of h) and its FP64 Kahan volume against the
+mass balance.  The batched ensemble (tile work-list, graph-captured step chunks) must
+reproduce single-scenario runs bit for bit.
+"""
+import numpy as np
+import pytest
+
+import oracle
+from appfis_amd import hip
+
+pytestmark = pytest.mark.gpu
+
+FLOOD_F64_RTOL = 1e-9
+FLOOD_F32_H_RTOL = 1e-3
+DX, CN, HEXTRA, DT_FALLBACK, RAIN = 10.0, 0.5, 1e-3, 1.0, 50e-3 / 3600.0
+
+
+def scenario(n, seed, dtype=np.float64, wet=0.0):
+    z = oracle.flood_dem(n, n, DX, seed)
+    h = np.zeros_like(z)
+    if wet:
+        h[1:-1, 1:-1] = np.maximum(np.percentile(z[1:-1, 1:-1], 40) - z[1:-1, 1:-1], 0.0) + wet
+    n2 = np.full_like(z, 0.035 ** 2)
+    return [a.astype(dtype) for a in (h, np.zeros_like(z), np.zeros_like(z), z, n2)]
+
+
+def params(print_interval=0.0):
+    return hip.FloodParams(DX, CN, HEXTRA, DT_FALLBACK, print_interval, RAIN, 0.0)
+
+
+def gpu_run(ctx, arrs, steps, variant=hip.PLAIN, print_interval=0.0):
+    dt = hip.F64 if arrs[0].dtype == np.float64 else hip.F32
+    d = [ctx.from_numpy(a, dt) for a in arrs]
+    n = arrs[0].shape[0] - 2
+    t, v = ctx.flood_run(d[0], d[1], d[2], d[3], d[4], n, n, params(print_interval), steps, variant)
+    return [x.download() for x in d[:3]], t, v
+
+
+def cpu_run(arrs, steps, print_interval=0.0):
+    a = [x.copy() for x in arrs]
+    n = a[0].shape[0] - 2
+    t, v, dt = oracle.flood_run(a[0], a[1], a[2], a[3], a[4], n, n, DX, CN, HEXTRA, DT_FALLBACK, print_interval,
+                                RAIN, 0.0, steps)
+    return a[:3], t, v
+
+
+@pytest.mark.parametrize("n,wet,steps", [(48, 0.0, 60), (100, 0.2, 40), (67, 0.5, 25)])
+def test_flood_f64_frictionless_bit_exact(ctx, n, wet, steps):
+    arrs = scenario(n, 3000 + n, wet=wet)
+    arrs[4][:] = 0.0
+    g, tg, vg = gpu_run(ctx, arrs, steps)
+    c, tc, vc = cpu_run(arrs, steps)
+    for a, b in zip(g, c):
+        assert np.array_equal(a, b)
+    assert np.array_equal(tg, tc)
+    assert np.allclose(vg, vc, rtol=1e-13, atol=0)
+
+
+@pytest.mark.parametrize("n,wet,steps", [(64, 0.0, 80), (90, 0.3, 50)])
+def test_flood_f64_with_friction(ctx, n, wet, steps):
+    arrs = scenario(n, 3100 + n, wet=wet)
+    g, tg, vg = gpu_run(ctx, arrs, steps)
+    c, tc, vc = cpu_run(arrs, steps)
+    scale = np.max(np.abs(c[0])) + 1e-30
+    assert np.max(np.abs(g[0] - c[0])) <= FLOOD_F64_RTOL * scale
+    qs = max(np.max(np.abs(c[1])), np.max(np.abs(c[2])), 1e-30)
+    assert np.max(np.abs(g[1] - c[1])) <= FLOOD_F64_RTOL * qs * 10
+    assert np.allclose(tg, tc, rtol=1e-12)
+    assert np.allclose(vg, vc, rtol=1e-10)
+
+
+def test_flood_graph_form_matches_direct_launches(ctx):
+    arrs = scenario(80, 3201, wet=0.1)
+    steps = 45  # two captured chunks of 20 plus 5 direct steps
+    a, ta, va = gpu_run(ctx, arrs, steps, hip.PLAIN)
+    b, tb, vb = gpu_run(ctx, arrs, steps, hip.CHOSEN)
+    for x, y in zip(a, b):
+        assert np.array_equal(x, y)
+    assert np.array_equal(ta, tb) and np.array_equal(va, vb)
+
+
+def test_flood_print_interval_clamp(ctx):
+    arrs = scenario(40, 3301, wet=0.3)
+    arrs[4][:] = 0.0
+    g, tg, _ = gpu_run(ctx, arrs, 30, print_interval=7.0)
+    c, tc, _ = cpu_run(arrs, 30, print_interval=7.0)
+    assert np.array_equal(tg, tc)
+    assert np.any(np.isclose(tg, 7.0, rtol=0, atol=1e-12))
+
+
+def test_flood_f32_against_f64(ctx):
+    n, steps = 128, 200
+    a64 = scenario(n, 3401)
+    c, tc, vc = cpu_run(a64, steps)
+    a32 = [x.astype(np.float32) for x in a64]
+    g, tg, vg = gpu_run(ctx, a32, steps, hip.CHOSEN)
+    rel = np.linalg.norm(g[0].astype(np.float64) - c[0]) / np.linalg.norm(c[0])
+    vmass = RAIN * n * n * DX * DX * tg
+    print(f"[flood f32] rel-L2(h) {rel:.3e}; volume vs rain*A*t {np.max(np.abs(vg - vmass) / vmass):.3e}")
+    assert rel < FLOOD_F32_H_RTOL
+    assert np.max(np.abs(vg - vmass) / vmass) < 1e-5
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+def test_flood_ensemble_matches_single_runs(ctx, dtype):
+    sizes = [33, 64, 17, 100, 48]
+    steps = 44
+    dt = hip.F64 if dtype == np.float64 else hip.F32
+    scen, singles = [], []
+    for k, n in enumerate(sizes):
+        arrs = scenario(n, 3500 + k, dtype, wet=0.2 if k % 2 else 0.0)
+        d = [ctx.from_numpy(a, dt) for a in arrs]
+        scen.append({"nx": n, "ny": n, "h": d[0], "qx": d[1], "qy": d[2], "z": d[3], "n2": d[4]})
+        singles.append(gpu_run(ctx, arrs, steps, hip.PLAIN))
+    t, v = ctx.flood_ensemble_run(scen, params(), steps, dt)
+    for k, s in enumerate(scen):
+        for x, y in zip((s["h"], s["qx"], s["qy"]), singles[k][0]):
+            assert np.array_equal(x.download(), y), k
+        assert np.array_equal(t[k], singles[k][1]) and np.array_equal(v[k], singles[k][2])
+
+
+def test_flood_dem_device_matches_oracle(ctx):
+    z = ctx.array((258, 258), hip.F64)
+    ctx.flood_dem(z, 256, 256, DX, 77)
+    ref = oracle.flood_dem(256, 256, DX, 77)
+    # device exp vs glibc exp: a few ulps
+    assert np.allclose(z.download(), ref, rtol=1e-14, atol=1e-12)

@@ -1,0 +1,165 @@
+"""Pins of the oracle's Triton flood step chain (CPU only).
+
+The Roe solver of Triton/src/triton_appfis.hpp has no closed form, so the oracle's
+restatement is fixed by what the mathematics does fix, chosen so that a dropped or
+mis-signed term fails at least one check:
+  - lake at rest stays at rest (C-property of the bed-slope source beta, 711-751)
+  - a dry domain without rain stays dry and uses the fallback dt (322-329)
+  - closed domain with rain: V(t) = V0 + rain * A * t (mass conservation of the flux
+    form; the walls are the 1e6 DEM ghost of triton_appfis.hpp:195)
+  - mirror symmetry of a symmetric problem, and the X functor against the Y functor
+    through a transpose (qx <-> -qy, the y axis points to decreasing row index)
+  - Ritter's analytic dry-bed dam break (wave speeds and depth profile)
+"""
+import numpy as np
+import pytest
+
+import oracle
+
+DX = 10.0
+
+
+def run(h, qx, qy, z, n2, steps, rain=0.0, courant=0.5, hextra=1e-3, dt_fallback=1.0, print_interval=0.0,
+        dx=DX):
+    ny, nx = h.shape[0] - 2, h.shape[1] - 2
+    t, v, dt = oracle.flood_run(h, qx, qy, z, n2, nx, ny, dx, courant, hextra, dt_fallback, print_interval, rain,
+                                0.0, steps)
+    return t, v, dt
+
+
+def fields(nx, ny, dtype=np.float64):
+    h = np.zeros((ny + 2, nx + 2), dtype)
+    return h, h.copy(), h.copy()
+
+
+def bumpy_bed(nx, ny, walls=1e6):
+    j, i = np.meshgrid(np.arange(ny + 2), np.arange(nx + 2), indexing="ij")
+    z = 2.0 * np.exp(-((i - nx / 3) ** 2 + (j - ny / 2) ** 2) / 40.0) + 1.5 * np.exp(
+        -((i - 2 * nx / 3) ** 2 + (j - ny / 3) ** 2) / 25.0) + 0.01 * i
+    z[0, :] = z[-1, :] = z[:, 0] = z[:, -1] = walls
+    return z
+
+
+def test_lake_at_rest_fully_wet():
+    nx, ny = 40, 30
+    z = bumpy_bed(nx, ny)
+    h, qx, qy = fields(nx, ny)
+    h[1:-1, 1:-1] = 5.0 - z[1:-1, 1:-1]
+    h0 = h.copy()
+    n2 = np.full_like(h, 0.035 ** 2)
+    run(h, qx, qy, z, n2, 200)
+    assert np.max(np.abs(h - h0)) < 1e-12
+    assert np.max(np.abs(qx)) < 1e-12 and np.max(np.abs(qy)) < 1e-12
+
+
+def test_lake_at_rest_with_shoreline():
+    nx, ny = 40, 30
+    z = bumpy_bed(nx, ny)
+    h, qx, qy = fields(nx, ny)
+    h[1:-1, 1:-1] = np.maximum(1.2 - z[1:-1, 1:-1], 0.0)  # the bumps stick out
+    assert (h[1:-1, 1:-1] == 0).any() and (h[1:-1, 1:-1] > 0).any()
+    h0 = h.copy()
+    n2 = np.full_like(h, 0.035 ** 2)
+    run(h, qx, qy, z, n2, 200)
+    assert np.max(np.abs(h - h0)) < 1e-12
+    assert np.max(np.abs(qx)) < 1e-12 and np.max(np.abs(qy)) < 1e-12
+
+
+def test_dry_without_rain_stays_dry():
+    nx, ny = 20, 16
+    z = bumpy_bed(nx, ny)
+    h, qx, qy = fields(nx, ny)
+    n2 = np.full_like(h, 0.035 ** 2)
+    t, v, dt = run(h, qx, qy, z, n2, 5, dt_fallback=2.5)
+    assert np.all(h == 0) and np.all(qx == 0) and np.all(qy == 0)
+    assert np.all(dt == 2.5) and np.allclose(t, 2.5 * np.arange(1, 6))
+
+
+def test_rain_mass_balance_closed_domain():
+    nx, ny = 48, 40
+    z = bumpy_bed(nx, ny)
+    h, qx, qy = fields(nx, ny)
+    n2 = np.full_like(h, 0.035 ** 2)
+    rain = 50e-3 / 3600.0
+    t, v, dt = run(h, qx, qy, z, n2, 300, rain=rain, dt_fallback=5.0)
+    area = nx * ny * DX * DX
+    expected = rain * area * t
+    # mass is created only by clipping negative depths: a relative residual well below 1e-6
+    assert np.max(np.abs(v - expected) / expected) < 1e-6
+    assert (qx != 0).any()  # the water did move
+
+
+def test_mirror_symmetry_x():
+    nx, ny = 41, 24
+    j, i = np.meshgrid(np.arange(ny + 2), np.arange(nx + 2), indexing="ij")
+    z = np.exp(-((i - (nx + 1) / 2) ** 2) / 60.0) * 3.0 + 0.2 * np.cos(j / 3.0)
+    z[0, :] = z[-1, :] = z[:, 0] = z[:, -1] = 1e6
+    h, qx, qy = fields(nx, ny)
+    h[1:-1, 1:-1] = np.maximum(2.5 - z[1:-1, 1:-1], 0) + 0.3 * np.exp(-((i - (nx + 1) / 2) ** 2 + (j - 8) ** 2)
+                                                                     / 20.0)[1:-1, 1:-1]
+    n2 = np.full_like(h, 0.03 ** 2)
+    run(h, qx, qy, z, n2, 150, rain=1e-5)
+    assert np.max(np.abs(h - h[:, ::-1])) < 1e-11 * np.max(h)
+    assert np.max(np.abs(qx + qx[:, ::-1])) < 1e-11 * max(np.max(np.abs(qx)), 1e-30)
+    assert np.max(np.abs(qx)) > 1e-6
+
+
+def test_x_and_y_functors_agree_through_transpose():
+    n = 36
+    j, i = np.meshgrid(np.arange(n + 2), np.arange(n + 2), indexing="ij")
+    z = 0.5 * np.exp(-((i - 12.0) ** 2 + (j - 20.0) ** 2) / 50.0) + 0.02 * i + 0.01 * j
+    z[0, :] = z[-1, :] = z[:, 0] = z[:, -1] = 1e6
+    h, qx, qy = fields(n, n)
+    h[1:-1, 1:-1] = 1.0 + 0.5 * np.exp(-((i - 20.0) ** 2 + (j - 14.0) ** 2) / 30.0)[1:-1, 1:-1]
+    n2 = np.full_like(h, 0.03 ** 2)
+    ht, qxt, qyt = h.T.copy(), qx.T.copy(), qy.T.copy()
+    zt, n2t = z.T.copy(), n2.T.copy()
+    run(h, qx, qy, z, n2, 120)
+    run(ht, qxt, qyt, zt, n2t, 120)
+    s = np.max(np.abs(qx))
+    assert s > 1e-3
+    assert np.max(np.abs(ht.T - h)) < 1e-10
+    assert np.max(np.abs(qxt.T + qy)) < 1e-10 * s
+    assert np.max(np.abs(qyt.T + qx)) < 1e-10 * s
+
+
+def test_dam_break_dry_bed_ritter():
+    # 1D dam break over a flat frictionless dry bed: x in cells, dam at the centre.
+    nx, ny, h0, g = 800, 3, 1.0, 9.81
+    dx = 1.0
+    z = np.zeros((ny + 2, nx + 2))
+    z[0, :] = z[-1, :] = z[:, 0] = z[:, -1] = 1e6
+    h, qx, qy = fields(nx, ny)
+    x = (np.arange(nx + 2) - 0.5) * dx - nx * dx / 2  # cell centres relative to the dam
+    h[1:-1, 1:-1] = np.where(x[1:-1] < 0, h0, 0.0)[None, :]
+    n2 = np.zeros_like(h)
+    T = 40.0
+    # print_interval = T clamps dt so that the run lands on t = T exactly
+    probe = [a.copy() for a in (h, qx, qy)]
+    t, _, _ = run(*probe, z, n2, 2000, courant=0.45, hextra=1e-6, print_interval=T, dx=dx)
+    steps = int(np.argmax(t >= T - 1e-12)) + 1
+    t, _, _ = run(h, qx, qy, z, n2, steps, courant=0.45, hextra=1e-6, print_interval=T, dx=dx)
+    assert abs(t[-1] - T) < 1e-9
+    c0 = np.sqrt(g * h0)
+    xc = x[1:-1]
+    exact = np.where(xc <= -c0 * T, h0, np.where(xc >= 2 * c0 * T, 0.0, (2 * c0 - xc / T) ** 2 / (9 * g)))
+    num = h[2, 1:-1]
+    l1 = np.sum(np.abs(num - exact)) / np.sum(exact)
+    assert l1 < 0.03, l1
+    # the rarefaction head has left at -c0 T, the depth at the dam is 4/9 h0
+    assert abs(num[np.argmin(np.abs(xc))] - 4.0 / 9.0 * h0) < 0.02
+    assert num[np.argmin(np.abs(xc + 1.3 * c0 * T))] > 0.999 * h0  # first-order smearing of the head
+
+
+def test_f32_chain_tracks_f64():
+    nx, ny = 40, 32
+    z = bumpy_bed(nx, ny)
+    out = []
+    for dt_ in (np.float64, np.float32):
+        h, qx, qy = fields(nx, ny, dt_)
+        h[1:-1, 1:-1] = np.maximum(1.5 - z[1:-1, 1:-1], 0.05)
+        n2 = np.full_like(h, 0.035 ** 2)
+        run(h, qx, qy, z.astype(dt_), n2, 100, rain=1e-5)
+        out.append(h.astype(np.float64))
+    rel = np.linalg.norm(out[0] - out[1]) / np.linalg.norm(out[0])
+    assert rel < 1e-4
