@@ -11,11 +11,14 @@
 //       pass and six rhs arrays never touch memory).
 //   K2 (wet/dry fix + CFL + volume): the momentum fix of triton_appfis.hpp:1095-1112 in
 //       place, then the CFL candidate of triton_appfis.hpp:567-586 from the fixed state
-//       (wave/block min, one atomicMin per block) and a compensated FP64 sum of h.  The
-//       last block of a scenario (atomic ticket) finalises on the device what the
-//       reference does on the host (triton_appfis.hpp:322-335, 381-385): simtime += dt,
-//       the print_interval bookkeeping, the next dt (fallback when every cell is dry,
-//       clamp to the next print boundary), and records time / volume / dt of the step.
+//       (wave/block min) and a compensated FP64 sum of h, written as one partial per
+//       tile -- no atomics (per-scenario atomics serialise hundreds of thousands of
+//       blocks on a handful of addresses).
+//   K3 (finalise): one block per scenario combines its tile partials in a fixed order
+//       and does on the device what the reference does on the host
+//       (triton_appfis.hpp:322-335, 381-385): simtime += dt, the print_interval
+//       bookkeeping, the next dt (fallback when every cell is dry, clamp to the next
+//       print boundary), and records time / volume / dt of the step.
 //
 // No host round trip per step: the reference's Execute2DReduce(MIN) + D2H + 3 host
 // syncs per step become device-side state, so a chunk of steps can be captured in a
@@ -46,13 +49,10 @@ template <> struct RealLim<float> {
 };
 
 struct ScenState {
-    double dt;        // dt of the step about to run
-    double simtime;   // simulated time (s)
-    unsigned long long dtmin;  // ordered bits of the running CFL minimum (R-typed value)
+    double dt;       // dt of the step about to run
+    double simtime;  // simulated time (s)
     int print_id;
-    int step;         // steps completed in this call
-    unsigned ticket;  // blocks of this scenario done in the current K2
-    int pad;
+    int step;        // steps completed in this call
 };
 
 template <typename R>
@@ -77,16 +77,6 @@ struct Params {
     double dx, cn, hextra, dt_fallback, print_interval, rain;
     int nsteps_rec;  // length of the record arrays
 };
-
-template <typename R> __device__ inline unsigned long long min_bits(R v);
-template <> __device__ inline unsigned long long min_bits<double>(double v)
-{
-    return (unsigned long long)__double_as_longlong(v);
-}
-template <> __device__ inline unsigned long long min_bits<float>(float v) { return (unsigned long long)__float_as_uint(v); }
-template <typename R> __device__ inline R from_bits(unsigned long long b);
-template <> __device__ inline double from_bits<double>(unsigned long long b) { return __longlong_as_double((long long)b); }
-template <> __device__ inline float from_bits<float>(unsigned long long b) { return __uint_as_float((unsigned)b); }
 
 // Roe flux with entropy fix, bed-slope source and wet/dry treatment -- the shared body
 // of the X functor (triton_appfis.hpp:608-824) and the Y functor (827-1043).
@@ -300,13 +290,14 @@ __device__ inline void two_sum(double& s, double& cmp, double x)
     s = t;
 }
 
-// MODE 0: initial CFL only (no fix, no time advance, no record); MODE 1: full step end
+// K2: wet/dry fix + per-tile CFL minimum and compensated volume partial (no atomics;
+// one tile of 64 x 4 cells per block, partials[3t..3t+2] = (min, sum, compensation)).
+// MODE 0: initial CFL only (no fix); MODE 1: end of a step.
 template <typename R, int MODE>
-__global__ __launch_bounds__(BT) void k_fix_cfl(Scen<R>* __restrict__ sc, int nscen, int par, Params P)
+__global__ __launch_bounds__(BT) void k_fix_cfl(const Scen<R>* __restrict__ sc, int nscen, int par, Params P)
 {
     __shared__ R smin[BT / 64];
     __shared__ double ssum[BT / 64][2];
-    __shared__ bool last;
     const int tile = blockIdx.x;
     const int s = find_scen(sc, nscen, tile);
     const Scen<R> S = sc[s];
@@ -321,7 +312,7 @@ __global__ __launch_bounds__(BT) void k_fix_cfl(Scen<R>* __restrict__ sc, int ns
     if (i <= S.nx && j <= S.ny) {
         const int W = S.nx + 2;
         const size_t c = (size_t)i + (size_t)W * (size_t)j;
-        R* h = S.h[par];
+        const R* h = S.h[par];
         R* qx = S.qx[par];
         R* qy = S.qy[par];
         const R* z = S.z;
@@ -345,7 +336,6 @@ __global__ __launch_bounds__(BT) void k_fix_cfl(Scen<R>* __restrict__ sc, int ns
         }
         vs = (double)hij;
     }
-    // block min of the CFL candidates, compensated block sum of h
     for (int o = 32; o > 0; o >>= 1) {
         R u = __shfl_xor(cfl, o, 64);
         cfl = u < cfl ? u : cfl;
@@ -368,81 +358,86 @@ __global__ __launch_bounds__(BT) void k_fix_cfl(Scen<R>* __restrict__ sc, int ns
             two_sum(bs, bc, ssum[w][0]);
             bc = __dadd_rn(bc, ssum[w][1]);
         }
-        if (m < RealLim<R>::max()) atomicMin(&S.st->dtmin, min_bits<R>(m));
-        S.partials[2 * t] = bs;
-        S.partials[2 * t + 1] = bc;
-        __threadfence();
-        unsigned prev = atomicAdd(&S.st->ticket, 1u);
-        last = (prev == (unsigned)S.ntiles - 1);
+        double* pp = S.partials + 3 * (size_t)t;
+        pp[0] = (double)m;
+        pp[1] = bs;
+        pp[2] = bc;
     }
-    __syncthreads();
-    if (!last) return;
-    __threadfence();
-    // last block of this scenario: volume, dt bookkeeping, records
+}
+
+// Finalise: one block per scenario combines its tile partials in a fixed order (the
+// reference's Execute2DReduce(MIN) plus the config's FP64 Kahan volume), then does the
+// host-side bookkeeping of triton_appfis.hpp:322-335, 381-385 on the device.
+template <typename R, int MODE>
+__global__ __launch_bounds__(BT) void k_finalize(const Scen<R>* __restrict__ sc, Params P)
+{
+    __shared__ double smin[BT / 64];
+    __shared__ double ssum[BT / 64][2];
+    const Scen<R> S = sc[blockIdx.x];
+    double m = (double)RealLim<R>::max();
     double as = 0.0, ac = 0.0;
     for (int p = threadIdx.x; p < S.ntiles; p += BT) {
-        two_sum(as, ac, __hip_atomic_load(S.partials + 2 * p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
-        ac = __dadd_rn(ac, __hip_atomic_load(S.partials + 2 * p + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+        const double* pp = S.partials + 3 * (size_t)p;
+        m = pp[0] < m ? pp[0] : m;
+        two_sum(as, ac, pp[1]);
+        ac = __dadd_rn(ac, pp[2]);
     }
     for (int o = 32; o > 0; o >>= 1) {
+        double u = __shfl_xor(m, o, 64);
+        m = u < m ? u : m;
         double us = __shfl_xor(as, o, 64), uc = __shfl_xor(ac, o, 64);
         two_sum(as, ac, us);
         ac = __dadd_rn(ac, uc);
     }
-    __syncthreads();
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
     if (lane == 0) {
+        smin[wid] = m;
         ssum[wid][0] = as;
         ssum[wid][1] = ac;
     }
     __syncthreads();
-    if (threadIdx.x == 0) {
-        double bs = ssum[0][0], bc = ssum[0][1];
-        for (int w = 1; w < BT / 64; w++) {
-            two_sum(bs, bc, ssum[w][0]);
-            bc = __dadd_rn(bc, ssum[w][1]);
-        }
-        ScenState* st = S.st;
-        const unsigned long long mb = __hip_atomic_load(&st->dtmin, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        const double PI = P.print_interval;
-        double simtime = st->simtime;
-        int pid = st->print_id;
-        if (MODE == 1) {
-            const double dtn = st->dt;
-            simtime += dtn;
-            if (PI > 0.0 && simtime >= PI * (pid + 1)) pid++;
-            const int stp = st->step;
-            if (stp < P.nsteps_rec) {
-                const double vol = __dmul_rn(__dmul_rn(__dadd_rn(bs, bc), P.dx), P.dx);
-                if (S.rec_time) S.rec_time[stp] = simtime;
-                if (S.rec_vol) S.rec_vol[stp] = vol;
-                if (S.rec_dt) S.rec_dt[stp] = dtn;
-            }
-            st->step = stp + 1;
-            st->simtime = simtime;
-            st->print_id = pid;
-        }
-        // next dt (triton_appfis.hpp:322-335)
-        const R m = from_bits<R>(mb);
-        double gdt = (double)m;
-        if (gdt >= (double)RealLim<R>::max() - 1.0) gdt = P.dt_fallback;
-        if (PI > 0.0 && simtime + gdt > PI * (pid + 1)) gdt = PI * (pid + 1) - simtime;
-        st->dt = (double)(R)gdt;
-        st->dtmin = min_bits<R>(RealLim<R>::max());
-        st->ticket = 0u;
+    if (threadIdx.x != 0) return;
+    double bs = ssum[0][0], bc = ssum[0][1];
+    for (int w = 1; w < BT / 64; w++) {
+        m = smin[w] < m ? smin[w] : m;
+        two_sum(bs, bc, ssum[w][0]);
+        bc = __dadd_rn(bc, ssum[w][1]);
     }
+    m = smin[0] < m ? smin[0] : m;
+    ScenState* st = S.st;
+    const double PI = P.print_interval;
+    double simtime = st->simtime;
+    int pid = st->print_id;
+    if (MODE == 1) {
+        const double dtn = st->dt;
+        simtime += dtn;
+        if (PI > 0.0 && simtime >= PI * (pid + 1)) pid++;
+        const int stp = st->step;
+        if (stp < P.nsteps_rec) {
+            const double vol = __dmul_rn(__dmul_rn(__dadd_rn(bs, bc), P.dx), P.dx);
+            if (S.rec_time) S.rec_time[stp] = simtime;
+            if (S.rec_vol) S.rec_vol[stp] = vol;
+            if (S.rec_dt) S.rec_dt[stp] = dtn;
+        }
+        st->step = stp + 1;
+        st->simtime = simtime;
+        st->print_id = pid;
+    }
+    // next dt (triton_appfis.hpp:322-335): the minimum is an R value held exactly in m
+    double gdt = (double)(R)m;
+    if (gdt >= (double)RealLim<R>::max() - 1.0) gdt = P.dt_fallback;
+    if (PI > 0.0 && simtime + gdt > PI * (pid + 1)) gdt = PI * (pid + 1) - simtime;
+    st->dt = (double)(R)gdt;
 }
 
-__global__ void k_init_state(ScenState* st, int n, double start_time, unsigned long long maxbits)
+__global__ void k_init_state(ScenState* st, int n, double start_time)
 {
     int s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= n) return;
     st[s].dt = 0.0;
     st[s].simtime = start_time;
-    st[s].dtmin = maxbits;
     st[s].print_id = 0;
     st[s].step = 0;
-    st[s].ticket = 0u;
-    st[s].pad = 0;
 }
 
 template <typename R>
@@ -474,7 +469,7 @@ int run_flood(appfis_ctx* ctx, int nscen, const int* nx, const int* ny, void* co
     const int rec = steps > 0 ? steps : 1;
     // workspace: second buffers (h, qx, qy) | states | partials | records | descriptor table
     const size_t st_bytes = ((size_t)nscen * sizeof(ScenState) + 255) / 256 * 256;
-    const size_t part_bytes = ((size_t)total_tiles * 16 + 255) / 256 * 256;
+    const size_t part_bytes = ((size_t)total_tiles * 24 + 255) / 256 * 256;
     const size_t rec_bytes = ((size_t)nscen * rec * 8 + 255) / 256 * 256;
     const size_t tab_bytes = ((size_t)nscen * sizeof(Scen<R>) + 255) / 256 * 256;
     int stt = ensure_work(ctx, 3 * field_bytes + st_bytes + part_bytes + 3 * rec_bytes + tab_bytes);
@@ -507,7 +502,7 @@ int run_flood(appfis_ctx* ctx, int nscen, const int* nx, const int* ny, void* co
     Scen<R>* tab = (Scen<R>*)(base + st_bytes + part_bytes + 3 * rec_bytes);
     for (int s = 0; s < nscen; s++) {
         hs[s].st = st + s;
-        hs[s].partials = partials + 2 * (size_t)hs[s].tile0;
+        hs[s].partials = partials + 3 * (size_t)hs[s].tile0;
         hs[s].rec_time = rt + (size_t)s * rec;
         hs[s].rec_vol = rv + (size_t)s * rec;
         hs[s].rec_dt = rd + (size_t)s * rec;
@@ -515,27 +510,20 @@ int run_flood(appfis_ctx* ctx, int nscen, const int* nx, const int* ny, void* co
     APPFIS_HIP_CHECK(hipMemcpyAsync(tab, hs.data(), (size_t)nscen * sizeof(Scen<R>), hipMemcpyHostToDevice, ctx->stream));
     APPFIS_HIP_CHECK(hipStreamSynchronize(ctx->stream));  // the host table is a local
     Params P{prm->dx, prm->courant, prm->hextra, prm->dt_fallback, prm->print_interval, prm->rain_rate, rec};
-    unsigned long long maxbits;
-    if (sizeof(R) == 8) {
-        double m = DBL_MAX;
-        memcpy(&maxbits, &m, 8);
-    } else {
-        float m = FLT_MAX;
-        unsigned u;
-        memcpy(&u, &m, 4);
-        maxbits = u;
-    }
-    hipLaunchKernelGGL(k_init_state, dim3(ceil_div(nscen, 64)), dim3(64), 0, ctx->stream, st, nscen, prm->start_time,
-                       maxbits);
+    hipLaunchKernelGGL(k_init_state, dim3(ceil_div(nscen, 64)), dim3(64), 0, ctx->stream, st, nscen, prm->start_time);
     APPFIS_LAUNCH_CHECK();
     const dim3 grid((unsigned)total_tiles);
     // dt of the first step from the initial state
     hipLaunchKernelGGL((k_fix_cfl<R, 0>), grid, dim3(BT), 0, ctx->stream, tab, nscen, 0, P);
     APPFIS_LAUNCH_CHECK();
+    hipLaunchKernelGGL((k_finalize<R, 0>), dim3(nscen), dim3(BT), 0, ctx->stream, tab, P);
+    APPFIS_LAUNCH_CHECK();
     auto step_pair = [&](int par) -> int {
         hipLaunchKernelGGL((k_flux_update<R>), grid, dim3(BT), 0, ctx->stream, tab, nscen, par, P);
         APPFIS_LAUNCH_CHECK();
         hipLaunchKernelGGL((k_fix_cfl<R, 1>), grid, dim3(BT), 0, ctx->stream, tab, nscen, par ^ 1, P);
+        APPFIS_LAUNCH_CHECK();
+        hipLaunchKernelGGL((k_finalize<R, 1>), dim3(nscen), dim3(BT), 0, ctx->stream, tab, P);
         APPFIS_LAUNCH_CHECK();
         return APPFIS_OK;
     };

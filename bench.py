@@ -1,17 +1,25 @@
 #!/usr/bin/env python3
 """APPFIS iterative-stencil benchmark on one MI355X.
 
-Default workload (the headline line): GameOfLife 2D, 8192 x 8192 uint8 cells, periodic,
-rule B3/S23, seeded Bernoulli(0.25), one step = one generation over the whole grid;
-`--steps 1000` is the config's 1000 generations.  Inputs are generated on the device
-before the timed region (resident in HBM).  The timed region is bracketed by a device
-synchronisation on both sides and measured with HIP events on the library's stream.
+Headline (the printed JSON line's `value`): GameOfLife 2D, 8192 x 8192 uint8 cells,
+periodic, rule B3/S23, seeded Bernoulli(0.25); one step = one generation over the whole
+grid, `--steps 1000` is the config's 1000 generations.  Inputs are generated on the
+device before the timed region (resident in HBM).  Each timed region is bracketed by a
+device synchronisation on both sides and measured with HIP events recorded on the
+library's stream (the stream every kernel of the call is launched on).
 
-Printed: ONE JSON line with GLUP/s (interior cells x generations / s / 1e9), the
-roofline of the dominant kernel and the CPU baseline (the oracle's OpenMP restatement of
-the reference CPU path, timed on a bounded sample on this host).
+The same line carries `suite`: the other configs measured the same way (Heat2D FP64,
+FDTD3d FP32/FP64/FP16/BF16, Himeno FP32/FP64/defect-correction, the flood ensemble in
+plain FP64 one scenario at a time and in the batched FP32 form), each with the
+roofline of its dominant kernel; `--headline-only` skips them.
 
-Other workloads (`--workload heat|fdtd|himeno|flood|all`) print their own JSON lines.
+Roofline: achieved = algorithmic bytes per launch of the dominant kernel / its average
+launch time (timed region / launches); peak = 8 TB/s HBM.  `traffic` is the HBM bytes
+per launch measured with rocprofv3 --pmc (FETCH_SIZE + WRITE_SIZE, calibrated on a copy
+kernel), read from profiles/traffic.json when present.
+
+cpu_baseline: the oracle (OpenMP C restatement of the reference CPU path) timed on this
+host on a bounded sample of the headline workload.
 """
 import argparse
 import json
@@ -25,17 +33,15 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+FDTD_COEFF = [0.5, 0.125, 0.0625, 0.125 / 3, 0.03125]
 
 
 def _traffic(kernel_key):
-    """HBM bytes per launch measured with rocprofv3 --pmc (FETCH_SIZE + WRITE_SIZE,
-    scaled by the copy-kernel calibration) and committed under profiles/."""
     path = os.path.join(ROOT, "profiles", "traffic.json")
     if not os.path.exists(path):
         return None
     try:
-        data = json.load(open(path))
-        v = data.get(kernel_key)
+        v = json.load(open(path)).get(kernel_key)
         return None if v is None else float(v["bytes_per_launch"])
     except Exception:
         return None
@@ -44,37 +50,37 @@ def _traffic(kernel_key):
 def roofline(alg_bytes_per_launch, launch_ms, kernel_key):
     achieved = alg_bytes_per_launch / (launch_ms * 1e-3) / 1e9
     return {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": _traffic(kernel_key),
-            "kernel": kernel_key, "alg_bytes_per_launch": alg_bytes_per_launch,
-            "avg_launch_us": round(launch_ms * 1e3, 3)}
+            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": _traffic(kernel_key), "kernel": kernel_key,
+            "alg_bytes_per_launch": alg_bytes_per_launch, "avg_launch_us": round(launch_ms * 1e3, 3)}
+
+
+def timed(ctx, fn):
+    ctx.sync()
+    t0 = time.perf_counter()
+    ctx.timer_start()
+    fn()
+    ms = ctx.timer_stop()
+    ctx.sync()
+    return ms, time.perf_counter() - t0
 
 
 # ------------------------------------------------------------------------------------
 # GameOfLife (headline)
 # ------------------------------------------------------------------------------------
-def bench_gol(ctx, hip, steps, warmup, variant, cpu=True, nx=8192, ny=8192):
+def bench_gol(ctx, hip, steps, warmup, variant, nx=8192, ny=8192):
     d0 = ctx.array((ny + 2, nx + 2), hip.U8)
     d1 = ctx.array((ny + 2, nx + 2), hip.U8)
     ctx.fill_bernoulli(d0, 1, 1002, 0.25)
     if warmup:
         ctx.gol(d0, d1, nx, ny, warmup, variant)
-        ctx.sync()
         ctx.fill_bernoulli(d0, 1, 1002, 0.25)
-    ctx.sync()
-    t0 = time.perf_counter()
-    ctx.timer_start()
-    ctx.gol(d0, d1, nx, ny, steps, variant)
-    ms = ctx.timer_stop()
-    ctx.sync()
-    wall = time.perf_counter() - t0
+    ms, wall = timed(ctx, lambda: ctx.gol(d0, d1, nx, ny, steps, variant))
     final = (d0, d1)[steps % 2]
     live = int(ctx.reduce_sum_wide(final, [1, nx + 1, 1, ny + 1]))
-    lups = nx * ny * steps
-    res = {"variant": "plain" if variant == hip.PLAIN else "chosen", "ms": ms, "wall_s": wall,
-           "glups": lups / (ms * 1e-3) / 1e9, "live_count": live}
+    res = {"form": "plain" if variant == hip.PLAIN else "chosen", "ms": round(ms, 4), "wall_s": round(wall, 4),
+           "glups": round(nx * ny * steps / (ms * 1e-3) / 1e9, 2), "live_count": live}
     if variant == hip.PLAIN:
-        launches = steps
-        res["roofline"] = roofline(2.0 * nx * ny, ms / launches, "gol_plain")
+        res["roofline"] = roofline(2.0 * nx * ny, ms / steps, "gol_plain")
     return res
 
 
@@ -87,13 +93,184 @@ def cpu_baseline_gol(nx=8192, ny=8192, target_s=12.0):
     t0 = time.perf_counter()
     oracle.gol_u8(a, b, nx, ny, 2)
     per = (time.perf_counter() - t0) / 2
-    gens = max(2, min(200, int(target_s / max(per, 1e-6))))
+    gens = max(2, min(1000, int(target_s / max(per, 1e-6))))
+    oracle.fill_bernoulli(a, 1, 1002, 0.25)
     t0 = time.perf_counter()
     oracle.gol_u8(a, b, nx, ny, gens)
     dt = time.perf_counter() - t0
     return {"value": round(nx * ny * gens / dt / 1e9, 4), "unit": "GLUP/s", "cores": oracle.omp_threads(),
-            "kind": "port", "sample": f"GameOfLife {nx}x{ny} uint8, {gens} generations (of 1000), "
-                                       f"oracle OpenMP restatement, {dt:.1f} s"}
+            "kind": "port",
+            "sample": f"GameOfLife {nx}x{ny} uint8, first {gens} of the 1000 generations, oracle OpenMP "
+                      f"restatement (schedule(dynamic,1) over rows), {dt:.1f} s on {os.cpu_count()} host CPUs"}
+
+
+# ------------------------------------------------------------------------------------
+# suite
+# ------------------------------------------------------------------------------------
+def bench_heat(ctx, hip, n=1024, iters=500):
+    out = {}
+    a = ctx.array((n + 2, n + 2), hip.F64)
+    b = ctx.array((n + 2, n + 2), hip.F64)
+    for name, variant in (("plain", hip.PLAIN), ("chosen", hip.CHOSEN)):
+        ctx.fill_uniform(a, 1, 1001)
+        ctx.heat2d(a, b, n, n, 0.1, 20, variant)  # warm-up
+        ctx.fill_uniform(a, 1, 1001)
+        ms, _ = timed(ctx, lambda: ctx.heat2d(a, b, n, n, 0.1, iters, variant))
+        chk = float(ctx.reduce_sum_wide(a, [1, n + 1, 1, n + 1]))
+        launches = iters if variant == hip.PLAIN else len(_plan(iters, 8))
+        per_launch_sweeps = iters / launches
+        r = {"ms": round(ms, 4), "glups": round(n * n * iters / (ms * 1e-3) / 1e9, 2), "checksum": chk,
+             "roofline": roofline(16.0 * n * n * per_launch_sweeps, ms / launches,
+                                  "heat2d_" + name)}
+        out[name] = r
+    out["workload"] = f"Heat2D {n}x{n} FP64 5-point, {iters} sweeps (16 B/LUP)"
+    return out
+
+
+def _plan(iters, ts):
+    n = (iters + ts - 1) // ts
+    if (n & 1) != (iters & 1):
+        n += 1
+    return [0] * min(n, iters)
+
+
+def bench_fdtd(ctx, hip, n=512, steps=100):
+    out = {"workload": f"FDTD3d r=4 {n}^3, {steps} steps"}
+    for name, dt, bpl in (("f32", hip.F32, 8), ("f64", hip.F64, 16), ("f16", hip.F16, 4), ("bf16", hip.BF16, 4)):
+        shape = (n + 8, n + 8, n + 8)
+        a = ctx.array(shape, dt)
+        b = ctx.array(shape, dt)
+        coeff = FDTD_COEFF if dt != hip.F16 else [c / 2.0625 for c in FDTD_COEFF]
+        ctx.fill_uniform(a, 4, 1004)
+        ctx.fdtd3d(a, b, n, n, n, coeff, 2)
+        ctx.fill_uniform(a, 4, 1004)
+        ms, _ = timed(ctx, lambda: ctx.fdtd3d(a, b, n, n, n, coeff, steps))
+        out[name] = {"ms": round(ms, 3), "glups": round(n ** 3 * steps / (ms * 1e-3) / 1e9, 2),
+                     "roofline": roofline(float(bpl) * n ** 3, ms / steps, "fdtd3d_" + name)}
+        a.free()
+        b.free()
+    return out
+
+
+def _himeno_dev(ctx, hip, nx, ny, nz, dt):
+    npdt = np.float64 if dt == hip.F64 else np.float32
+    k = np.arange(nz, dtype=np.float64).reshape(nz, 1, 1)
+    p = np.ascontiguousarray(np.broadcast_to((k / (nz - 1)) ** 2, (nz, ny, nx)).astype(npdt))
+    arrs = [ctx.from_numpy(p, dt), ctx.from_numpy(p, dt)]
+    for v in [1.0, 1.0, 1.0, 1.0 / 6.0, 0.0, 0.0, 0.0, 1.0, 1.0, 1.0, 1.0, 0.0]:
+        a = ctx.array((nz, ny, nx), dt)
+        ctx.fill_value(a, v)
+        arrs.append(a)
+    return arrs, p
+
+
+def bench_himeno(ctx, hip, nx=257, ny=257, nz=513, iters=200):
+    out = {"workload": f"Himeno 19-point {nx}x{ny}x{nz} (incl. shell), {iters} sweeps + gosa"}
+    lups = (nx - 2) * (ny - 2) * (nz - 2) * iters
+    gosas = {}
+    for name, dt, variant, bpl in (("f32", hip.F32, hip.PLAIN, 56), ("f64", hip.F64, hip.PLAIN, 112),
+                                   ("dc", hip.F64, hip.CHOSEN, None)):
+        arrs, p = _himeno_dev(ctx, hip, nx, ny, nz, dt)
+        ctx.himeno(arrs, nx, ny, nz, 0.8, 3, variant)  # warm-up
+        arrs[0].upload(p)
+        arrs[1].upload(p)
+
+        def run():
+            gosas[name] = ctx.himeno(arrs, nx, ny, nz, 0.8, iters, variant)
+
+        ms, _ = timed(ctx, run)
+        r = {"ms": round(ms, 3), "glups": round(lups / (ms * 1e-3) / 1e9, 2), "gosa_last": float(gosas[name][-1])}
+        if bpl:
+            r["roofline"] = roofline(float(bpl) * (nx - 2) * (ny - 2) * (nz - 2), ms / iters, "himeno_" + name)
+        out[name] = r
+        for a in arrs:
+            a.free()
+    ref = gosas["f64"]
+    for name in ("f32", "dc"):
+        out[name]["gosa_max_rel_err_vs_f64"] = float(np.max(np.abs(gosas[name] - ref) / ref))
+    out["dc"]["speedup_vs_f64"] = round(out["f64"]["ms"] / out["dc"]["ms"], 3)
+    return out
+
+
+def flood_sizes(nscen=24, seed=1005):
+    """N_s = 512 + floor(u_s * 3585), u_s = (r(seed, s) >> 11) * 2^-53 (splitmix64)."""
+    sizes = []
+    for s in range(nscen):
+        z = seed + (s + 1) * 0x9E3779B97F4A7C15
+        z &= (1 << 64) - 1
+        z = ((z ^ (z >> 30)) * 0xBF58476D1CE4E5B9) & ((1 << 64) - 1)
+        z = ((z ^ (z >> 27)) * 0x94D049BB133111EB) & ((1 << 64) - 1)
+        z ^= z >> 31
+        sizes.append(512 + int((z >> 11) * (4096 - 512 + 1) / 2 ** 53))
+    return sizes
+
+
+def bench_flood(ctx, hip, steps=2000, nscen=24):
+    sizes = flood_sizes(nscen)
+    cells = sum(n * n for n in sizes)
+    prm = hip.FloodParams(10.0, 0.5, 1e-3, 1.0, 0.0, 50e-3 / 3600.0, 0.0)
+    out = {"workload": f"Triton flood ensemble: {nscen} scenarios, N in [512,4096] (sum N^2 = {cells}), "
+                       f"{steps} steps, 10 m cells, rain 50 mm/h, n = 0.035, CFL 0.5",
+           "sizes": sizes}
+
+    def make(dt):
+        sc = []
+        for s, n in enumerate(sizes):
+            d = {"nx": n, "ny": n}
+            for key in ("h", "qx", "qy", "z", "n2"):
+                d[key] = ctx.array((n + 2, n + 2), dt)
+            ctx.flood_dem(d["z"], n, n, 10.0, 1005 + s)
+            ctx.fill_value(d["n2"], 0.035 ** 2)
+            sc.append(d)
+        return sc
+
+    def reset(sc):
+        for d in sc:
+            for key in ("h", "qx", "qy"):
+                ctx.fill_value(d[key], 0.0)
+
+    # plain FP64, one scenario at a time, direct launches
+    sc = make(hip.F64)
+    vol64 = []
+
+    def run_plain():
+        for d in sc:
+            t, v = ctx.flood_run(d["h"], d["qx"], d["qy"], d["z"], d["n2"], d["nx"], d["ny"], prm, steps, hip.PLAIN)
+            vol64.append(v[-1])
+
+    ctx.flood_run(sc[0]["h"], sc[0]["qx"], sc[0]["qy"], sc[0]["z"], sc[0]["n2"], sizes[0], sizes[0], prm, 4,
+                  hip.PLAIN)
+    reset(sc)
+    ms64, _ = timed(ctx, run_plain)
+    h64 = [sc[s]["h"].download() for s in (0, nscen - 1)]
+    for d in sc:
+        for k in ("h", "qx", "qy", "z", "n2"):
+            d[k].free()
+    # chosen FP32 bulk, all scenarios batched, graph-captured chunks
+    sc = make(hip.F32)
+    ctx.flood_ensemble_run(sc, prm, 4, hip.F32)
+    reset(sc)
+    res = {}
+
+    def run_chosen():
+        res["t"], res["v"] = ctx.flood_ensemble_run(sc, prm, steps, hip.F32)
+
+    ms32, _ = timed(ctx, run_chosen)
+    h32 = [sc[s]["h"].download().astype(np.float64) for s in (0, nscen - 1)]
+    rel = max(float(np.linalg.norm(a - b) / np.linalg.norm(a)) for a, b in zip(h64, h32))
+    vrel = float(np.max(np.abs(np.array(vol64) - res["v"][:, -1]) / np.array(vol64)))
+    lups = cells * steps
+    out["plain_f64"] = {"ms": round(ms64, 2), "glups": round(lups / (ms64 * 1e-3) / 1e9, 2),
+                        "roofline": roofline(64.0 * cells / nscen, ms64 / (2 * steps * nscen), "flood_f64_step")}
+    out["chosen_f32_ensemble"] = {"ms": round(ms32, 2), "glups": round(lups / (ms32 * 1e-3) / 1e9, 2),
+                                  "speedup_vs_plain_f64": round(ms64 / ms32, 3),
+                                  "h_rel_l2_vs_f64(first,last scenario)": rel,
+                                  "volume_rel_err_vs_f64": vrel,
+                                  "roofline": roofline(32.0 * cells, ms32 / (2 * steps), "flood_f32_ensemble_step")}
+    for d in sc:
+        for k in ("h", "qx", "qy", "z", "n2"):
+            d[k].free()
+    return out
 
 
 def main():
@@ -101,55 +278,61 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=1000)
     ap.add_argument("--warmup", type=int, default=50)
-    ap.add_argument("--workload", default="gol")
+    ap.add_argument("--headline-only", action="store_true")
+    ap.add_argument("--only", default="", help="comma list of suite entries (heat,fdtd,himeno,flood)")
     ap.add_argument("--no-cpu", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py measures one GPU: this build has no multi-GPU path (asked for --gpus {args.gpus}). "
-              "See DESIGN.md for how the stencils would shard.", file=sys.stderr)
+        print(f"bench.py measures one GPU: this build has no multi-GPU path (asked for --gpus {args.gpus}); "
+              "DESIGN.md describes how the stencils would shard.", file=sys.stderr)
         sys.exit(2)
 
     from appfis_amd import hip
 
     ctx = hip.Context(0)
-    device = ctx.name()
-    if args.workload == "gol":
-        plain = bench_gol(ctx, hip, args.steps, args.warmup, hip.PLAIN)
-        variants = {"plain": plain}
-        best = plain
-        try:
-            chosen = bench_gol(ctx, hip, args.steps, args.warmup, hip.CHOSEN)
-            if chosen["live_count"] != plain["live_count"]:
-                raise RuntimeError("chosen form disagrees with plain form")
-            variants["chosen"] = chosen
-            if chosen["glups"] > plain["glups"]:
-                best = chosen
-        except hip.AppfisError as e:
-            variants["chosen"] = {"error": str(e)}
-        cpu = None if args.no_cpu else cpu_baseline_gol()
-        line = {
-            "metric": "GLUP/s (GameOfLife 8192^2 uint8 B3/S23 periodic)",
-            "value": round(best["glups"], 2),
-            "unit": "GLUP/s",
-            "n_gpus": 1,
-            "steps": args.steps,
-            "warmup": args.warmup,
-            "ms_per_step": round(best["ms"] / args.steps, 5),
-            "higher_is_better": True,
-            "scaling": "weak",
-            "vs_baseline": None,
-            "dtype": "u8",
-            "data": "synthetic (splitmix64 Bernoulli p=0.25, seed 1002, generated on device)",
-            "config": {"workload": "GameOfLife 2D 8192x8192 uint8, periodic, B3/S23, 1 step = 1 generation",
-                       "nx": 8192, "ny": 8192, "form": best["variant"], "device": device},
-            "roofline": plain["roofline"],
-            "cpu_baseline": cpu,
-            "variants": {k: {kk: vv for kk, vv in v.items() if kk != "roofline"} for k, v in variants.items()},
-        }
-        print(json.dumps(line), flush=True)
-    else:
-        print(json.dumps({"error": f"unknown workload {args.workload}"}))
-        sys.exit(2)
+    device = ctx.name().strip()
+    plain = bench_gol(ctx, hip, args.steps, args.warmup, hip.PLAIN)
+    variants = {"plain": plain}
+    best = plain
+    try:
+        chosen = bench_gol(ctx, hip, args.steps, args.warmup, hip.CHOSEN)
+        if chosen["live_count"] != plain["live_count"]:
+            raise RuntimeError("chosen GoL form disagrees with the plain form")
+        variants["chosen"] = chosen
+        if chosen["glups"] > plain["glups"]:
+            best = chosen
+    except hip.AppfisError as e:
+        variants["chosen"] = {"error": str(e)}
+    suite = {}
+    if not args.headline_only:
+        only = set(x for x in args.only.split(",") if x) or {"heat", "fdtd", "himeno", "flood"}
+        for name, fn in (("heat", bench_heat), ("fdtd", bench_fdtd), ("himeno", bench_himeno),
+                         ("flood", bench_flood)):
+            if name in only:
+                print(f"[bench] {name}", file=sys.stderr, flush=True)
+                suite[name] = fn(ctx, hip)
+    cpu = None if args.no_cpu else cpu_baseline_gol()
+    line = {
+        "metric": "GLUP/s (GameOfLife 8192^2 uint8 B3/S23 periodic); suite: GLUP/s and % of 8 TB/s per app",
+        "value": best["glups"],
+        "unit": "GLUP/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(best["ms"] / args.steps, 5),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u8",
+        "data": "synthetic (splitmix64 Bernoulli p=0.25, seed 1002, generated on device)",
+        "config": {"workload": "GameOfLife 2D 8192x8192 uint8, periodic, B3/S23, 1 step = 1 generation",
+                   "nx": 8192, "ny": 8192, "form": best["form"], "device": device},
+        "roofline": plain["roofline"],
+        "cpu_baseline": cpu,
+        "variants": {k: {kk: vv for kk, vv in v.items() if kk != "roofline"} for k, v in variants.items()},
+        "suite": suite,
+    }
+    print(json.dumps(line), flush=True)
     ctx.close()
 
 
