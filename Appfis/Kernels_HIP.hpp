@@ -1,0 +1,106 @@
+/*
+ * Kernels_HIP.hpp -- the hand-written MI355X kernels of the config stencils, applied
+ * to APPFIS grids in place of a generic Execute loop.
+ *
+ * Each call runs `n` sweeps on the grid's device buffers (current -> next -> ...) and
+ * leaves the grid swapped exactly as n calls of Execute with GridsToUpdate(grid) would:
+ * after an odd count the former NEXT buffer is current.  The grids must be between
+ * InitializeExecution and FinalizeExecution.  Variants: APPFIS_PLAIN (one sweep per
+ * launch, reference operation order) or APPFIS_CHOSEN (temporal blocking / packed /
+ * mixed precision; see include/appfis_hip.h).
+ */
+#ifndef APPFIS_KERNELS_HIP_HPP
+#define APPFIS_KERNELS_HIP_HPP
+
+#include <cstdint>
+#include <vector>
+
+#include "Common.hpp"
+#include "Grid.hpp"
+
+namespace APPFIS
+{
+    namespace kernels
+    {
+        namespace detail_k
+        {
+            template <typename T>
+            void RequireDual(Grid<T>& g, const char* what)
+            {
+                if (!EXECUTION_FLAG || g.GetGridType() != DUAL || !g.HasDeviceMirror()) {
+                    std::cerr << "APPFIS::kernels::" << what << " needs a DUAL grid inside InitializeExecution"
+                              << std::endl;
+                    exit(EXIT_FAILURE);
+                }
+            }
+            template <typename T>
+            void SwapParity(Grid<T>& g, int n)
+            {
+                if (n & 1) g.Swap();
+            }
+        }  // namespace detail_k
+
+        // HeatDiffusion 2D 5-point (ghost 1): u' = u + c*(uN + uS + uE + uW - 4u)
+        template <typename T>
+        void Heat2D(Grid<T>& g, double c, int iters, int variant = APPFIS_CHOSEN, int tsteps = 0)
+        {
+            detail_k::RequireDual(g, "Heat2D");
+            Array<T>* h = g.GetSubGridCPU();
+            const int nx = h->GetDim(X) - 2, ny = h->GetDim(Y) - 2;
+            APPFIS::detail::Check(appfis_heat2d(GetContext(), APPFIS::detail::DtypeOf<T>(), g.GetSubGridsGPUData(),
+                                                g.GetSubGridsNextGPUData(), nx, ny, c, iters, variant, tsteps),
+                                  "appfis_heat2d");
+            detail_k::SwapParity(g, iters);
+        }
+
+        // GameOfLife B3/S23 on a periodic uint8 grid (ghost 1)
+        inline void GameOfLife(Grid<uint8_t>& g, int gens, int variant = APPFIS_PLAIN, int tsteps = 0)
+        {
+            detail_k::RequireDual(g, "GameOfLife");
+            Array<uint8_t>* h = g.GetSubGridCPU();
+            const int nx = h->GetDim(X) - 2, ny = h->GetDim(Y) - 2;
+            APPFIS::detail::Check(appfis_gol(GetContext(), g.GetSubGridsGPUData(), g.GetSubGridsNextGPUData(), nx, ny,
+                                             gens, variant, tsteps),
+                                  "appfis_gol");
+            detail_k::SwapParity(g, gens);
+        }
+
+        // FDTD3d radius-4 star (ghost 4): u' = c0 u + sum_r c_r (6 neighbours at r)
+        template <typename T>
+        void FDTD3d(Grid<T>& g, const double coeff5[5], int steps, int variant = APPFIS_PLAIN)
+        {
+            detail_k::RequireDual(g, "FDTD3d");
+            Array<T>* h = g.GetSubGridCPU();
+            const int nx = h->GetDim(X) - 8, ny = h->GetDim(Y) - 8, nz = h->GetDim(Z) - 8;
+            APPFIS::detail::Check(appfis_fdtd3d(GetContext(), APPFIS::detail::DtypeOf<T>(), g.GetSubGridsGPUData(),
+                                                g.GetSubGridsNextGPUData(), nx, ny, nz, coeff5, steps, variant),
+                                  "appfis_fdtd3d");
+            detail_k::SwapParity(g, steps);
+        }
+
+        // Himeno 19-point Jacobi with gosa per sweep (p DUAL, the 12 coefficient grids
+        // SINGLE, all with the boundary shell as ghost 1)
+        template <typename T>
+        std::vector<double> Himeno(Grid<T>& p, Grid<T>& a0, Grid<T>& a1, Grid<T>& a2, Grid<T>& a3, Grid<T>& b0,
+                                   Grid<T>& b1, Grid<T>& b2, Grid<T>& c0, Grid<T>& c1, Grid<T>& c2, Grid<T>& bnd,
+                                   Grid<T>& wrk1, double omega, int iters, int variant = APPFIS_PLAIN)
+        {
+            detail_k::RequireDual(p, "Himeno");
+            Array<T>* h = p.GetSubGridCPU();
+            void* arrays[14] = {p.GetSubGridsGPUData(), p.GetSubGridsNextGPUData(), a0.GetSubGridsGPUData(),
+                                a1.GetSubGridsGPUData(), a2.GetSubGridsGPUData(), a3.GetSubGridsGPUData(),
+                                b0.GetSubGridsGPUData(), b1.GetSubGridsGPUData(), b2.GetSubGridsGPUData(),
+                                c0.GetSubGridsGPUData(), c1.GetSubGridsGPUData(), c2.GetSubGridsGPUData(),
+                                bnd.GetSubGridsGPUData(), wrk1.GetSubGridsGPUData()};
+            std::vector<double> gosa((size_t)(iters > 0 ? iters : 1));
+            APPFIS::detail::Check(appfis_himeno(GetContext(), APPFIS::detail::DtypeOf<T>(), arrays, h->GetDim(X),
+                                                h->GetDim(Y), h->GetDim(Z), omega, iters, variant, 0, gosa.data()),
+                                  "appfis_himeno");
+            gosa.resize((size_t)(iters > 0 ? iters : 0));
+            detail_k::SwapParity(p, iters);
+            return gosa;
+        }
+    }  // namespace kernels
+}  // namespace APPFIS
+
+#endif

@@ -1,0 +1,68 @@
+// FDTD3d config workload (radius-4 star, c0 = 0.5, c_r = 0.125/r, ghost 4 at zero) on
+// the APPFIS API: user functor through the generic Execute3D path against the
+// hand-written kernel (APPFIS::kernels::FDTD3d), FP32.  One JSON line.
+//   fdtd3d [-dimx=512] [-dimy=512] [-dimz=512] [-iteration=100]
+#include <cstdio>
+#include <cstring>
+
+#include "Appfis/Appfis.hpp"
+
+namespace APPFIS
+{
+    PARALLEL void STENCIL(const int i, const int j, const int k, float c0, float c1, float c2, float c3, float c4,
+                          Array<float>* cur, Array<float>* next);
+}
+
+PARALLEL void APPFIS::STENCIL(const int i, const int j, const int k, float c0, float c1, float c2, float c3, float c4,
+                              Array<float>* cur, Array<float>* next)
+{
+    const float c[5] = {c0, c1, c2, c3, c4};
+    float acc = c[0] * cur->Get(i, j, k);
+    for (int r = 1; r <= 4; r++) {
+        const float sx = cur->Get(i - r, j, k) + cur->Get(i + r, j, k);
+        const float sy = cur->Get(i, j - r, k) + cur->Get(i, j + r, k);
+        const float sz = cur->Get(i, j, k - r) + cur->Get(i, j, k + r);
+        acc = acc + c[r] * ((sx + sy) + sz);
+    }
+    next->Set(i, j, k, acc);
+}
+
+int main(int argc, char* argv[])
+{
+    int nx = 512, ny = 512, nz = 512, steps = 100;
+    if (APPFIS::CheckArgument(argc, argv, "dimx")) nx = APPFIS::GetArgument(argc, argv, "dimx");
+    if (APPFIS::CheckArgument(argc, argv, "dimy")) ny = APPFIS::GetArgument(argc, argv, "dimy");
+    if (APPFIS::CheckArgument(argc, argv, "dimz")) nz = APPFIS::GetArgument(argc, argv, "dimz");
+    if (APPFIS::CheckArgument(argc, argv, "iteration")) steps = APPFIS::GetArgument(argc, argv, "iteration");
+    APPFIS::Attribute at;
+    at.ghostLayer = 4;
+    APPFIS::Initialize(argc, argv, APPFIS::DIM_3D, APPFIS::DIM_3D, at);
+    APPFIS::Grid<float> ga(APPFIS::DIM_3D, nx, ny, nz, APPFIS::DUAL), gb(APPFIS::DIM_3D, nx, ny, nz, APPFIS::DUAL);
+    APPFIS::Config<float> config(ga);
+    config.GridsToUpdate(ga);
+    APPFIS::InitializeExecution(ga, gb);
+    const int dims[3] = {nx + 8, ny + 8, nz + 8};
+    for (auto* g : {&ga, &gb})
+        APPFIS::detail::Check(appfis_fill_uniform(APPFIS::GetContext(), g->GetSubGridsGPUData(), APPFIS_F32, 3, dims,
+                                                  4, 1004, 0.0, 1.0), "fill");
+    const double cf[5] = {0.5, 0.125, 0.0625, 0.125 / 3, 0.03125};
+    APPFIS::Timer timer;
+    timer.Start("GENERIC");
+    for (int s = 0; s < steps; s++)
+        APPFIS::Execute3D(config, (float)cf[0], (float)cf[1], (float)cf[2], (float)cf[3], (float)cf[4],
+                          ga.GetSubGrid(), ga.GetSubGridNext());
+    timer.Stop("GENERIC");
+    timer.Start("KERNEL");
+    APPFIS::kernels::FDTD3d(gb, cf, steps);
+    timer.Stop("KERNEL");
+    APPFIS::FinalizeExecution(ga, gb);
+    const size_t n = (size_t)dims[0] * dims[1] * dims[2];
+    const bool same = memcmp(ga.GetSubGrid()->GetData(), gb.GetSubGrid()->GetData(), n * sizeof(float)) == 0;
+    const long long pts = (long long)nx * ny * nz;
+    printf("{\"app\": \"fdtd3d\", \"n\": [%d, %d, %d], \"steps\": %d, \"generic_glups\": %.3f, \"kernel_glups\": %.3f, "
+           "\"bit_identical\": %s}\n",
+           nx, ny, nz, steps, pts * (double)steps / timer.GetCustomTime("GENERIC") / 1e9,
+           pts * (double)steps / timer.GetCustomTime("KERNEL") / 1e9, same ? "true" : "false");
+    APPFIS::Finalize();
+    return same ? 0 : 1;
+}

@@ -1,0 +1,69 @@
+// GameOfLife config workload (uint8 cells, periodic, B3/S23, every cell written) on
+// the APPFIS API: user functor through the generic Execute2D path (periodic exchange
+// on the device after every generation) against the hand-written kernel
+// (APPFIS::kernels::GameOfLife).  Prints one JSON line (live counts, GLUP/s, identity).
+//   gol2d [-dimx=8192] [-dimy=8192] [-iteration=1000] [-chosen]
+#include <cstdio>
+#include <cstring>
+
+#include "Appfis/Appfis.hpp"
+
+namespace APPFIS
+{
+    PARALLEL void STENCIL(const int i, const int j, Array<uint8_t>* cur, Array<uint8_t>* next);
+}
+
+PARALLEL void APPFIS::STENCIL(const int i, const int j, Array<uint8_t>* cur, Array<uint8_t>* next)
+{
+    const int n = cur->Get(i - 1, j - 1) + cur->Get(i, j - 1) + cur->Get(i + 1, j - 1) + cur->Get(i - 1, j) +
+                  cur->Get(i + 1, j) + cur->Get(i - 1, j + 1) + cur->Get(i, j + 1) + cur->Get(i + 1, j + 1);
+    next->Set(i, j, (uint8_t)(n == 3 || (n == 2 && cur->Get(i, j))));
+}
+
+int main(int argc, char* argv[])
+{
+    int nx = 8192, ny = 8192, gens = 1000;
+    if (APPFIS::CheckArgument(argc, argv, "dimx")) nx = APPFIS::GetArgument(argc, argv, "dimx");
+    if (APPFIS::CheckArgument(argc, argv, "dimy")) ny = APPFIS::GetArgument(argc, argv, "dimy");
+    if (APPFIS::CheckArgument(argc, argv, "iteration")) gens = APPFIS::GetArgument(argc, argv, "iteration");
+    const int variant = APPFIS::CheckArgument(argc, argv, "chosen") ? APPFIS_CHOSEN : APPFIS_PLAIN;
+    APPFIS::Attribute at;
+    at.ghostLayer = 1;
+    at.periodic = true;
+    APPFIS::Initialize(argc, argv, APPFIS::DIM_2D, APPFIS::DIM_2D, at);
+    APPFIS::Grid<uint8_t> ga(APPFIS::DIM_2D, nx, ny, APPFIS::DUAL), gb(APPFIS::DIM_2D, nx, ny, APPFIS::DUAL);
+    APPFIS::Config<uint8_t> config(ga);
+    config.GridsToUpdate(ga);
+    APPFIS::InitializeExecution(ga, gb);
+    const int dims[2] = {nx + 2, ny + 2};
+    for (auto* g : {&ga, &gb})
+        APPFIS::detail::Check(appfis_fill_bernoulli_u8(APPFIS::GetContext(), g->GetSubGridsGPUData(), 2, dims, 1,
+                                                       1002, 0.25), "fill");
+    // periodic ring of the seeded buffer (the reference exchanges at InitializeExecution)
+    APPFIS::detail::Check(appfis_exchange(APPFIS::GetContext(), ga.GetSubGridsGPUData(), APPFIS_U8, 2, dims, 1, 1),
+                          "exchange");
+    APPFIS::Timer timer;
+    timer.Start("GENERIC");
+    for (int g = 0; g < gens; g++) APPFIS::Execute2D(config, ga.GetSubGrid(), ga.GetSubGridNext());
+    timer.Stop("GENERIC");
+    timer.Start("KERNEL");
+    APPFIS::kernels::GameOfLife(gb, gens, variant);
+    timer.Stop("KERNEL");
+    int64_t la = 0, lb = 0;
+    const int box[4] = {1, nx + 1, 1, ny + 1};
+    APPFIS::detail::Check(appfis_reduce_sum_wide(APPFIS::GetContext(), ga.GetSubGridsGPUData(), APPFIS_U8, 2, dims,
+                                                 box, &la), "live count");
+    APPFIS::detail::Check(appfis_reduce_sum_wide(APPFIS::GetContext(), gb.GetSubGridsGPUData(), APPFIS_U8, 2, dims,
+                                                 box, &lb), "live count");
+    APPFIS::FinalizeExecution(ga, gb);
+    const size_t n = (size_t)(nx + 2) * (ny + 2);
+    const bool same = memcmp(ga.GetSubGrid()->GetData(), gb.GetSubGrid()->GetData(), n) == 0;
+    printf("{\"app\": \"gol2d\", \"nx\": %d, \"ny\": %d, \"generations\": %d, \"live_count\": %lld, "
+           "\"live_count_kernel\": %lld, \"generic_glups\": %.3f, \"kernel_glups\": %.3f, \"kernel_form\": \"%s\", "
+           "\"bit_identical\": %s}\n",
+           nx, ny, gens, (long long)la, (long long)lb, timer.CalculateBLUPS("GENERIC", nx * ny, gens),
+           timer.CalculateBLUPS("KERNEL", nx * ny, gens), variant == APPFIS_PLAIN ? "plain" : "chosen",
+           same ? "true" : "false");
+    APPFIS::Finalize();
+    return same ? 0 : 1;
+}
