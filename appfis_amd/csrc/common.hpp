@@ -8,9 +8,18 @@
 #include <cstdint>
 #include <cstdio>
 #include <cstring>
+#include <functional>
 #include <string>
+#include <vector>
 
 #include "../../include/appfis_hip.h"
+
+// one autotuning decision: kernel id, dtype and shape -> index of the chosen variant
+struct appfis_tune_entry {
+    int key[6];
+    int choice;
+    std::string report;
+};
 
 struct appfis_ctx {
     int device = 0;
@@ -28,6 +37,8 @@ struct appfis_ctx {
     void* pinned = nullptr;
     size_t pinned_bytes = 0;
     int num_cu = 0;
+    std::vector<appfis_tune_entry> tuned;  // autotuner cache
+    std::string tune_report;               // text of the most recent decision
 };
 
 namespace appfis {
@@ -54,6 +65,13 @@ inline size_t dtype_size(int dtype)
     default: return 0;
     }
 }
+
+// Autotuner: look up / run / record the variant choice for a (kernel, dtype, shape) key.
+// run_variant(i) must enqueue the timed work of candidate i on ctx->stream (on scratch
+// buffers, never on the caller's data) and return a status; each candidate is run once
+// untimed, then `reps` times under hipEvents; the fastest mean wins.
+int tune_select(appfis_ctx* ctx, const int key[6], int ncand, const char* const* names,
+                const std::function<int(int)>& run_variant, int reps, int* choice);
 
 // Validates ndim in 1..3, dims > 0, returns padded extents in d[3] (missing axes = 1).
 int check_shape(int ndim, const int* dims, int* d);

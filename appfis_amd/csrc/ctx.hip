@@ -81,6 +81,50 @@ int ensure_pinned(appfis_ctx* ctx, size_t bytes)
     return APPFIS_OK;
 }
 
+int tune_select(appfis_ctx* ctx, const int key[6], int ncand, const char* const* names,
+                const std::function<int(int)>& run_variant, int reps, int* choice)
+{
+    for (const appfis_tune_entry& e : ctx->tuned)
+        if (memcmp(e.key, key, sizeof(e.key)) == 0) {
+            *choice = e.choice;
+            return APPFIS_OK;
+        }
+    std::string rep = "autotune kernel " + std::to_string(key[0]) + " dtype " + std::to_string(key[1]) + " shape " +
+                      std::to_string(key[2]) + "x" + std::to_string(key[3]) + "x" + std::to_string(key[4]) + ":";
+    int best = 0;
+    float best_ms = 0.f;
+    for (int i = 0; i < ncand; i++) {
+        int st = run_variant(i);  // warm-up (code load, caches)
+        if (st) return st;
+        APPFIS_HIP_CHECK(hipEventRecord(ctx->ev_start, ctx->stream));
+        for (int r = 0; r < reps; r++) {
+            st = run_variant(i);
+            if (st) return st;
+        }
+        APPFIS_HIP_CHECK(hipEventRecord(ctx->ev_stop, ctx->stream));
+        APPFIS_HIP_CHECK(hipEventSynchronize(ctx->ev_stop));
+        float ms = 0.f;
+        APPFIS_HIP_CHECK(hipEventElapsedTime(&ms, ctx->ev_start, ctx->ev_stop));
+        ms /= (float)reps;
+        char line[160];
+        snprintf(line, sizeof(line), " [%s %.4f ms]", names[i], ms);
+        rep += line;
+        if (i == 0 || ms < best_ms) {
+            best = i;
+            best_ms = ms;
+        }
+    }
+    rep += std::string(" -> ") + names[best];
+    appfis_tune_entry e;
+    memcpy(e.key, key, sizeof(e.key));
+    e.choice = best;
+    e.report = rep;
+    ctx->tuned.push_back(e);
+    ctx->tune_report = rep;
+    *choice = best;
+    return APPFIS_OK;
+}
+
 int check_shape(int ndim, const int* dims, int* d)
 {
     if (ndim < 1 || ndim > 3) return invalid("ndim must be 1, 2 or 3 (got %d)", ndim);
@@ -116,6 +160,15 @@ const char* appfis_status_string(int status)
 }
 
 const char* appfis_last_error(void) { return g_err; }
+
+int appfis_tune_report(appfis_ctx* ctx, char* buf, int len)
+{
+    APPFIS_REQUIRE(ctx && buf && len > 0, "bad argument");
+    std::string all;
+    for (const appfis_tune_entry& e : ctx->tuned) all += e.report + "\n";
+    snprintf(buf, (size_t)len, "%s", all.c_str());
+    return APPFIS_OK;
+}
 
 int appfis_device_count(int* count)
 {

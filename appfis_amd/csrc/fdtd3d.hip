@@ -5,11 +5,14 @@
 //   s_r = ((u[x-r] + u[x+r]) + (u[y-r] + u[y+r])) + (u[z-r] + u[z+r])
 //   u'  = (((c0*u + c1*s_1) + c2*s_2) + c3*s_3) + c4*s_4        (no FMA contraction)
 //
-// 2.5D k-marching: a block owns a TX x (TY*RY) column of the x-y plane and a chunk of
-// z.  Each thread keeps a 9-deep register queue along z for each of its RY rows, so
-// the z-neighbours cost no extra loads; the x-y plane at the current z is staged in LDS
-// with its 4-wide halo for the x and y neighbours.  Per output: one queue load, the
-// plane halo share (~0.6 loads, mostly L2 hits) and one store.
+// 2.5D k-march.  A block of 64 x TY threads owns a (64*VX) x (TY*RY) column of the x-y
+// plane and a chunk of ZC planes.  Each thread owns VX x RY points of the plane and keeps
+// a 9-deep register queue along z for each of them, so the z neighbours cost nothing;
+// the y neighbours inside the thread's own RY rows come from the same registers; only
+// the x neighbours and the y neighbours across thread rows are read from the plane
+// staged in LDS.  The LDS plane is double-buffered and the next plane's halo and queue
+// entries are prefetched into registers before the barrier, so each z step has one
+// barrier and its global-load latency overlaps the previous step's arithmetic.
 //
 // dtype F32 / F64: storage and arithmetic in that type.  F16 / BF16: storage only,
 // arithmetic in FP32, round-to-nearest-even on store.
@@ -41,114 +44,263 @@ template <> struct Cvt<__hip_bfloat16> {
     __device__ static __hip_bfloat16 st(float v) { return __float2bfloat16(v); }
 };
 
-constexpr int R = 4;
+constexpr int R = 4;     // stencil radius = ghost width
+constexpr int TXT = 64;  // threads along x (one wave per thread row)
 
-template <typename S, int TX, int TY, int RY, int ZC>
-__global__ __launch_bounds__(TX* TY) void k_fdtd3d(const S* __restrict__ u, S* __restrict__ v, int nx, int ny, int nz,
-                                                    typename Cvt<S>::C c0, typename Cvt<S>::C c1,
-                                                    typename Cvt<S>::C c2, typename Cvt<S>::C c3,
-                                                    typename Cvt<S>::C c4)
-{
-    using C = typename Cvt<S>::C;
-    constexpr int PY = TY * RY;           // tile rows
-    constexpr int LX = TX + 2 * R;        // LDS plane width
-    constexpr int LY = PY + 2 * R;        // LDS plane height
-    __shared__ C plane[LY][LX];
-
-    const int W = nx + 2 * R, H = ny + 2 * R;
-    const size_t sy = (size_t)W, sz = (size_t)W * (size_t)H;
-    const int tx = threadIdx.x, ty = threadIdx.y;
-    const int tid = ty * TX + tx;
-    const int gx = blockIdx.x * TX + tx + R;         // padded x of this thread
-    const int gy0 = blockIdx.y * PY + ty * RY + R;   // padded y of row 0 of this thread
-    const int z0 = blockIdx.z * ZC + R;              // first padded z of the chunk
-    const int z1 = min(z0 + ZC, nz + R);             // one past the last
-    const bool xin = gx < nx + R;   // interior column: stores
-    const bool xpad = gx < W;       // inside the padded array: loads (ghost values kept)
-    const C cc[5] = {c0, c1, c2, c3, c4};
-
-    // register queues: q[r][m] holds u at z + m - 4 for row r
-    C q[RY][2 * R + 1];
+// VX consecutive storage elements of a point group; VX = 2 with 16-bit storage moves
+// both through one 32-bit access (the pair offset is even: see the launch checks).
+template <typename S, int VX>
+struct Vec {
+    __device__ static void load(const S* __restrict__ base, int off, typename Cvt<S>::C (&out)[VX])
+    {
+        if constexpr (VX == 2 && sizeof(S) == 2) {
+            const uint32_t w = reinterpret_cast<const uint32_t*>(base)[off >> 1];
+            S lo, hi;
+            const uint16_t l16 = (uint16_t)(w & 0xFFFFu), h16 = (uint16_t)(w >> 16);
+            __builtin_memcpy(&lo, &l16, 2);
+            __builtin_memcpy(&hi, &h16, 2);
+            out[0] = Cvt<S>::ld(lo);
+            out[1] = Cvt<S>::ld(hi);
+        } else {
 #pragma unroll
-    for (int r = 0; r < RY; r++) {
-        const int gy = gy0 + r;
-        const bool in = xpad && gy < H;
-#pragma unroll
-        for (int m = 0; m < 2 * R; m++) {
-            const int z = z0 - R + m;
-            q[r][m + 1] = in ? Cvt<S>::ld(u[(size_t)gx + sy * gy + sz * z]) : (C)0;
+            for (int e = 0; e < VX; e++) out[e] = Cvt<S>::ld(base[off + e]);
         }
     }
+    __device__ static void store(S* __restrict__ base, int off, const typename Cvt<S>::C (&in)[VX])
+    {
+        if constexpr (VX == 2 && sizeof(S) == 2) {
+            const S lo = Cvt<S>::st(in[0]), hi = Cvt<S>::st(in[1]);
+            uint16_t l16, h16;
+            __builtin_memcpy(&l16, &lo, 2);
+            __builtin_memcpy(&h16, &hi, 2);
+            reinterpret_cast<uint32_t*>(base)[off >> 1] = (uint32_t)l16 | ((uint32_t)h16 << 16);
+        } else {
+#pragma unroll
+            for (int e = 0; e < VX; e++) base[off + e] = Cvt<S>::st(in[e]);
+        }
+    }
+};
 
-    for (int z = z0; z < z1; z++) {
-        // advance the queues: q[.][8] = u at z + 4
+// Tile coordinates of this block.  Consecutive block ids land on consecutive XCDs, so
+// the linear id is remapped to give every XCD a contiguous run of tiles (x fastest):
+// tiles that share halos then share an L2.
+__device__ inline void tile_of(int& bx, int& by, int& bz)
+{
+    const int gx = gridDim.x, gy = gridDim.y;
+    const int nb = gx * gy * gridDim.z;
+    int bid = blockIdx.x + gx * (blockIdx.y + gy * blockIdx.z);
+    if ((nb & 7) == 0) bid = (bid & 7) * (nb >> 3) + (bid >> 3);
+    bx = bid % gx;
+    const int r = bid / gx;
+    by = r % gy;
+    bz = r / gy;
+}
+
+template <typename S, int VX, int TY, int RY, int ZC, int D>
+__global__ __launch_bounds__(TXT* TY) void k_fdtd3d(const S* __restrict__ u, S* __restrict__ v, int nx, int ny, int nz,
+                                                     typename Cvt<S>::C c0, typename Cvt<S>::C c1,
+                                                     typename Cvt<S>::C c2, typename Cvt<S>::C c3,
+                                                     typename Cvt<S>::C c4)
+{
+    using C = typename Cvt<S>::C;
+    static_assert((D & 1) == 0, "prefetch depth must be even (LDS buffer parity per slot)");
+    constexpr int NT = TXT * TY;
+    constexpr int PX = TXT * VX;       // tile width
+    constexpr int PY = TY * RY;        // tile height
+    constexpr int LX = PX + 2 * R;     // LDS plane width
+    constexpr int LY = PY + 2 * R;     // LDS plane height
+    constexpr int NHALO = 2 * R * PX + 2 * R * PY;
+    constexpr int HPT = (NHALO + NT - 1) / NT;  // halo cells per thread
+    __shared__ C plane[2][LY][LX];
+
+    int tbx, tby, tbz;
+    tile_of(tbx, tby, tbz);
+    const int W = nx + 2 * R, H = ny + 2 * R;
+    const long long szl = (long long)W * H;
+    const int tx = threadIdx.x, ty = threadIdx.y;
+    const int tid = ty * TXT + tx;
+    const int bx = tbx * PX + R, by = tby * PY + R;  // padded origin of the tile
+    const int z0 = tbz * ZC + R;
+    const int z1 = min(z0 + ZC, nz + R);
+    const C cc[5] = {c0, c1, c2, c3, c4};
+
+    // own point group: x = gx .. gx+VX-1, rows y = gy0 + r; in-plane 32-bit offsets
+    const int gx = bx + tx * VX;
+    const bool xload = gx < W, xstore = gx < nx + R;
+    const int gy0 = by + ty * RY;
+    const int own0 = gx + W * gy0;
+
+    // halo slots of this thread: LDS position and in-plane offset (fixed for the march)
+    int hoff[HPT], hlds[HPT];
+    bool hval[HPT], huse[HPT];
+#pragma unroll
+    for (int s = 0; s < HPT; s++) {
+        const int p = tid + s * NT;
+        int lx = 0, ly = 0;
+        huse[s] = p < NHALO;
+        if (p < 2 * R * PX) {
+            const int hy = p / PX;
+            lx = p - hy * PX + R;
+            ly = hy < R ? hy : PY + hy;
+        } else if (huse[s]) {
+            const int p2 = p - 2 * R * PX;
+            const int hx = p2 / PY;
+            ly = p2 - hx * PY + R;
+            lx = hx < R ? hx : PX + hx;
+        }
+        const int xg = bx - R + lx, yg = by - R + ly;
+        hval[s] = huse[s] && xg < W && yg < H;
+        hoff[s] = hval[s] ? xg + W * yg : 0;
+        hlds[s] = ly * LX + lx;
+    }
+
+    // register queues q[r][e][m] = u at z + m - 4
+    C q[RY][VX][2 * R + 1];
+#pragma unroll
+    for (int r = 0; r < RY; r++) {
+        const bool in = xload && gy0 + r < H;
+#pragma unroll
+        for (int m = 0; m < 2 * R; m++) {
+            C t[VX];
+            if (in) Vec<S, VX>::load(u + (z0 - R + m) * szl, own0 + r * W, t);
+            else
+#pragma unroll
+                for (int e = 0; e < VX; e++) t[e] = (C)0;
+#pragma unroll
+            for (int e = 0; e < VX; e++) q[r][e][m + 1] = t[e];
+        }
+    }
+    // prefetch ring: slot d holds the queue entry (plane z+R) and halo (plane z) of step
+    // z0 + d (+ D, 2D, ... as the march proceeds)
+    C qp[D][RY][VX];
+    C hp[D][HPT];
+    auto prefetch = [&](int z, C(&qd)[RY][VX], C(&hd)[HPT]) {
+        const S* pq = u + (long long)(z + R) * szl;
+        const S* ph = u + (long long)z * szl;
 #pragma unroll
         for (int r = 0; r < RY; r++) {
+            if (xload && gy0 + r < H) Vec<S, VX>::load(pq, own0 + r * W, qd[r]);
+            else
 #pragma unroll
-            for (int m = 0; m < 2 * R; m++) q[r][m] = q[r][m + 1];
-            const int gy = gy0 + r;
-            const bool in = xpad && gy < H;
-            q[r][2 * R] = in ? Cvt<S>::ld(u[(size_t)gx + sy * gy + sz * (z + R)]) : (C)0;
+                for (int e = 0; e < VX; e++) qd[r][e] = (C)0;
         }
-        __syncthreads();  // previous plane fully consumed
-        // own cells
 #pragma unroll
-        for (int r = 0; r < RY; r++) plane[ty * RY + r + R][tx + R] = q[r][R];
-        // halo: y-halo rows (R above, R below) x TX columns, x-halo columns (R left,
-        // R right) x PY rows; clamped to the padded array
-        const size_t zoff = sz * (size_t)z;
-        const int bx = blockIdx.x * TX + R, by = blockIdx.y * PY + R;
-        for (int p = tid; p < 2 * R * TX + 2 * R * PY; p += TX * TY) {
-            int lx, ly;
-            if (p < 2 * R * TX) {
-                int hy = p / TX;
-                lx = p - hy * TX + R;
-                ly = hy < R ? hy : PY + hy;  // rows 0..R-1 and PY+R..PY+2R-1
-            } else {
-                int p2 = p - 2 * R * TX;
-                int hx = p2 / PY;
-                ly = p2 - hx * PY + R;
-                lx = hx < R ? hx : TX + hx;
+        for (int s = 0; s < HPT; s++) hd[s] = hval[s] ? Cvt<S>::ld(ph[hoff[s]]) : (C)0;
+    };
+#pragma unroll
+    for (int d = 0; d < D; d++)
+        if (z0 + d < z1) prefetch(z0 + d, qp[d], hp[d]);
+
+    C* pl = &plane[0][0][0];
+    for (int zb = z0; zb < z1; zb += D) {
+#pragma unroll
+        for (int d = 0; d < D; d++) {
+            const int z = zb + d;
+            if (z >= z1) break;
+            C* buf = pl + (d & 1) * (LY * LX);
+            // advance the queues to plane z
+#pragma unroll
+            for (int r = 0; r < RY; r++)
+#pragma unroll
+                for (int e = 0; e < VX; e++) {
+#pragma unroll
+                    for (int m = 0; m < 2 * R; m++) q[r][e][m] = q[r][e][m + 1];
+                    q[r][e][2 * R] = qp[d][r][e];
+                }
+            // stage plane z: own centres + prefetched halo
+#pragma unroll
+            for (int r = 0; r < RY; r++)
+#pragma unroll
+                for (int e = 0; e < VX; e++) buf[(ty * RY + r + R) * LX + tx * VX + e + R] = q[r][e][R];
+#pragma unroll
+            for (int s = 0; s < HPT; s++)
+                if (huse[s]) buf[hlds[s]] = hp[d][s];
+            // refill this slot with step z + D
+            if (z + D < z1) prefetch(z + D, qp[d], hp[d]);
+            __syncthreads();
+            S* vz = v + (long long)z * szl;
+#pragma unroll
+            for (int r = 0; r < RY; r++) {
+                const int ly = ty * RY + r + R;
+                C out[VX];
+#pragma unroll
+                for (int e = 0; e < VX; e++) {
+                    const int lx = tx * VX + e + R;
+                    C acc = cc[0] * q[r][e][R];
+#pragma unroll
+                    for (int k = 1; k <= R; k++) {
+                        const C sx = buf[ly * LX + lx - k] + buf[ly * LX + lx + k];
+                        const C ym = (r - k >= 0) ? q[r - k][e][R] : buf[(ly - k) * LX + lx];
+                        const C yp = (r + k < RY) ? q[r + k][e][R] : buf[(ly + k) * LX + lx];
+                        const C sY = ym + yp;
+                        const C sZ = q[r][e][R - k] + q[r][e][R + k];
+                        const C sr = (sx + sY) + sZ;
+                        acc = acc + cc[k] * sr;
+                    }
+                    out[e] = acc;
+                }
+                if (xstore && gy0 + r < ny + R) Vec<S, VX>::store(vz, own0 + r * W, out);
             }
-            const int xg = bx - R + lx, yg = by - R + ly;
-            C val = (C)0;
-            if (xg < W && yg < H) val = Cvt<S>::ld(u[(size_t)xg + sy * yg + zoff]);
-            plane[ly][lx] = val;
-        }
-        __syncthreads();
-#pragma unroll
-        for (int r = 0; r < RY; r++) {
-            const int gy = gy0 + r;
-            const int ly = ty * RY + r + R, lx = tx + R;
-            C acc = cc[0] * q[r][R];
-#pragma unroll
-            for (int k = 1; k <= R; k++) {
-                C sx = plane[ly][lx - k] + plane[ly][lx + k];
-                C sY = plane[ly - k][lx] + plane[ly + k][lx];
-                C sZ = q[r][R - k] + q[r][R + k];
-                C sr = (sx + sY) + sZ;
-                acc = acc + cc[k] * sr;
-            }
-            if (xin && gy < ny + R) v[(size_t)gx + sy * gy + zoff] = Cvt<S>::st(acc);
         }
     }
 }
 
-constexpr int TX = 64, TY = 4, RY = 4, ZC = 64;
-
-template <typename S>
+template <typename S, int VX, int TY, int RY, int ZC, int D>
 int run(appfis_ctx* ctx, S* b0, S* b1, int nx, int ny, int nz, const double* cf, int steps)
 {
     using C = typename Cvt<S>::C;
-    dim3 block(TX, TY), grid(ceil_div(nx, TX), ceil_div(ny, TY * RY), ceil_div(nz, ZC));
+    dim3 block(TXT, TY), grid(ceil_div(nx, TXT * VX), ceil_div(ny, TY * RY), ceil_div(nz, ZC));
     S* buf[2] = {b0, b1};
     for (int s = 0; s < steps; s++) {
-        hipLaunchKernelGGL((k_fdtd3d<S, TX, TY, RY, ZC>), grid, block, 0, ctx->stream, buf[s & 1], buf[(s + 1) & 1], nx,
-                           ny, nz, (C)cf[0], (C)cf[1], (C)cf[2], (C)cf[3], (C)cf[4]);
+        hipLaunchKernelGGL((k_fdtd3d<S, VX, TY, RY, ZC, D>), grid, block, 0, ctx->stream, buf[s & 1],
+                           buf[(s + 1) & 1], nx, ny, nz, (C)cf[0], (C)cf[1], (C)cf[2], (C)cf[3], (C)cf[4]);
         APPFIS_LAUNCH_CHECK();
     }
     return APPFIS_OK;
 }
+
+typedef int (*fdtd_fn)(appfis_ctx*, void*, void*, int, int, int, const double*, int);
+struct FdtdVariant {
+    const char* name;
+    fdtd_fn fn;
+    bool pairs;  // needs the 32-bit pair path (even nx, 4-byte aligned buffers)
+};
+
+template <typename S, int VX, int TY, int RY, int ZC, int D>
+int run_v(appfis_ctx* ctx, void* b0, void* b1, int nx, int ny, int nz, const double* cf, int steps)
+{
+    return run<S, VX, TY, RY, ZC, D>(ctx, (S*)b0, (S*)b1, nx, ny, nz, cf, steps);
+}
+
+// Candidate tilings (VX x-points per thread, TY thread rows, RY rows per thread, ZC
+// planes per block, D-deep prefetch).  Index 0 is the PLAIN default.
+#define FDTD_VARIANTS(S)                                                          \
+    {"vx1 ty4 ry4 zc64 d2", run_v<S, 1, 4, 4, 64, 2>, false},                      \
+    {"vx1 ty4 ry8 zc64 d2", run_v<S, 1, 4, 8, 64, 2>, false},                      \
+    {"vx1 ty4 ry4 zc32 d2", run_v<S, 1, 4, 4, 32, 2>, false},                      \
+    {"vx1 ty4 ry4 zc128 d2", run_v<S, 1, 4, 4, 128, 2>, false},                    \
+    {"vx1 ty8 ry4 zc64 d2", run_v<S, 1, 8, 4, 64, 2>, false},                      \
+    {"vx1 ty4 ry2 zc64 d2", run_v<S, 1, 4, 2, 64, 2>, false},                      \
+    {"vx1 ty4 ry4 zc64 d4", run_v<S, 1, 4, 4, 64, 4>, false}
+
+static const FdtdVariant kF32[] = {FDTD_VARIANTS(float)};
+static const FdtdVariant kF64[] = {
+    {"vx1 ty4 ry2 zc64 d2", run_v<double, 1, 4, 2, 64, 2>, false},
+    {"vx1 ty4 ry4 zc64 d2", run_v<double, 1, 4, 4, 64, 2>, false},
+    {"vx1 ty4 ry2 zc32 d2", run_v<double, 1, 4, 2, 32, 2>, false},
+    {"vx1 ty4 ry2 zc128 d2", run_v<double, 1, 4, 2, 128, 2>, false},
+    {"vx1 ty8 ry2 zc64 d2", run_v<double, 1, 8, 2, 64, 2>, false},
+};
+static const FdtdVariant kF16[] = {
+    FDTD_VARIANTS(__half),
+    {"vx2 ty4 ry4 zc64 d2", run_v<__half, 2, 4, 4, 64, 2>, true},
+    {"vx2 ty4 ry2 zc64 d2", run_v<__half, 2, 4, 2, 64, 2>, true},
+};
+static const FdtdVariant kBF16[] = {
+    FDTD_VARIANTS(__hip_bfloat16),
+    {"vx2 ty4 ry4 zc64 d2", run_v<__hip_bfloat16, 2, 4, 4, 64, 2>, true},
+    {"vx2 ty4 ry2 zc64 d2", run_v<__hip_bfloat16, 2, 4, 2, 64, 2>, true},
+};
 
 }  // namespace
 
@@ -161,12 +313,46 @@ extern "C" int appfis_fdtd3d(appfis_ctx* ctx, int dtype, void* buf0, void* buf1,
     APPFIS_REQUIRE((double)(nx + 8) * (ny + 8) * (nz + 8) < 2147483647.0, "grid exceeds the int index range");
     APPFIS_REQUIRE(steps >= 0, "steps must be >= 0");
     APPFIS_REQUIRE(variant == APPFIS_PLAIN || variant == APPFIS_CHOSEN, "fdtd3d: unknown variant %d", variant);
+    const FdtdVariant* tab;
+    int ntab;
     switch (dtype) {
-    case APPFIS_F32: return run<float>(ctx, (float*)buf0, (float*)buf1, nx, ny, nz, coeff5, steps);
-    case APPFIS_F64: return run<double>(ctx, (double*)buf0, (double*)buf1, nx, ny, nz, coeff5, steps);
-    case APPFIS_F16: return run<__half>(ctx, (__half*)buf0, (__half*)buf1, nx, ny, nz, coeff5, steps);
-    case APPFIS_BF16:
-        return run<__hip_bfloat16>(ctx, (__hip_bfloat16*)buf0, (__hip_bfloat16*)buf1, nx, ny, nz, coeff5, steps);
+    case APPFIS_F32: tab = kF32; ntab = (int)(sizeof(kF32) / sizeof(kF32[0])); break;
+    case APPFIS_F64: tab = kF64; ntab = (int)(sizeof(kF64) / sizeof(kF64[0])); break;
+    case APPFIS_F16: tab = kF16; ntab = (int)(sizeof(kF16) / sizeof(kF16[0])); break;
+    case APPFIS_BF16: tab = kBF16; ntab = (int)(sizeof(kBF16) / sizeof(kBF16[0])); break;
     default: return invalid("fdtd3d: dtype must be F32, F64, F16 or BF16");
     }
+    if (steps == 0) return APPFIS_OK;
+    if (variant == APPFIS_PLAIN) return tab[0].fn(ctx, buf0, buf1, nx, ny, nz, coeff5, steps);
+    // CHOSEN: the fastest tiling for this shape, timed once on scratch buffers.  Every
+    // candidate evaluates the same expression in the same order: results are identical.
+    const bool pairs = ((nx & 1) == 0) && (((uintptr_t)buf0 | (uintptr_t)buf1) & 3) == 0;
+    std::vector<int> cand;
+    std::vector<const char*> names;
+    for (int i = 0; i < ntab; i++)
+        if (!tab[i].pairs || pairs) {
+            cand.push_back(i);
+            names.push_back(tab[i].name);
+        }
+    const int key[6] = {3, dtype, nx, ny, nz, pairs ? 1 : 0};
+    int pick = 0;
+    const size_t bytes = (size_t)(nx + 8) * (ny + 8) * (nz + 8) * dtype_size(dtype);
+    int st = APPFIS_OK;
+    bool cached = false;
+    for (const appfis_tune_entry& e : ctx->tuned)
+        if (memcmp(e.key, key, sizeof(key)) == 0) cached = true;
+    void* s0 = nullptr;
+    void* s1 = nullptr;
+    if (!cached) {
+        const size_t half = (bytes + 255) / 256 * 256;
+        st = ensure_work(ctx, 2 * half);
+        if (st) return st;
+        s0 = ctx->work;
+        s1 = (char*)ctx->work + half;
+        APPFIS_HIP_CHECK(hipMemsetAsync(ctx->work, 0, 2 * half, ctx->stream));
+    }
+    st = tune_select(ctx, key, (int)cand.size(), names.data(),
+                     [&](int i) { return tab[cand[i]].fn(ctx, s0, s1, nx, ny, nz, coeff5, 2); }, 3, &pick);
+    if (st) return st;
+    return tab[cand[pick]].fn(ctx, buf0, buf1, nx, ny, nz, coeff5, steps);
 }

@@ -91,6 +91,7 @@ def lib():
         "appfis_ctx_stream": ([vp], vp),
         "appfis_sync": ([vp], i),
         "appfis_device_name": ([vp, C.c_char_p, i], i),
+        "appfis_tune_report": ([vp, C.c_char_p, i], i),
         "appfis_malloc": ([vp, sz, C.POINTER(vp)], i),
         "appfis_free": ([vp, vp], i),
         "appfis_memcpy_h2d": ([vp, vp, vp, sz], i),
@@ -214,6 +215,11 @@ class Context:
 
     def sync(self):
         check(lib().appfis_sync(self.handle), "sync")
+
+    def tune_report(self) -> str:
+        buf = C.create_string_buffer(65536)
+        check(lib().appfis_tune_report(self.handle, buf, 65536), "tune_report")
+        return buf.value.decode()
 
     def array(self, shape, dtype: int) -> DeviceArray:
         return DeviceArray(self, shape, dtype)

@@ -135,20 +135,22 @@ def _plan(iters, ts):
 
 
 def bench_fdtd(ctx, hip, n=512, steps=100):
-    out = {"workload": f"FDTD3d r=4 {n}^3, {steps} steps"}
+    out = {"workload": f"FDTD3d r=4 {n}^3, {steps} steps (CHOSEN = autotuned tiling, same values as PLAIN)"}
     for name, dt, bpl in (("f32", hip.F32, 8), ("f64", hip.F64, 16), ("f16", hip.F16, 4), ("bf16", hip.BF16, 4)):
         shape = (n + 8, n + 8, n + 8)
         a = ctx.array(shape, dt)
         b = ctx.array(shape, dt)
         coeff = FDTD_COEFF if dt != hip.F16 else [c / 2.0625 for c in FDTD_COEFF]
-        ctx.fill_uniform(a, 4, 1004)
-        ctx.fdtd3d(a, b, n, n, n, coeff, 2)
-        ctx.fill_uniform(a, 4, 1004)
-        ms, _ = timed(ctx, lambda: ctx.fdtd3d(a, b, n, n, n, coeff, steps))
-        out[name] = {"ms": round(ms, 3), "glups": round(n ** 3 * steps / (ms * 1e-3) / 1e9, 2),
-                     "roofline": roofline(float(bpl) * n ** 3, ms / steps, "fdtd3d_" + name)}
+        for form, variant in (("plain", hip.PLAIN), ("chosen", hip.CHOSEN)):
+            ctx.fill_uniform(a, 4, 1004)
+            ctx.fdtd3d(a, b, n, n, n, coeff, 2, variant)  # warm-up (and the one-time autotuning)
+            ctx.fill_uniform(a, 4, 1004)
+            ms, _ = timed(ctx, lambda: ctx.fdtd3d(a, b, n, n, n, coeff, steps, variant))
+            out[f"{name}_{form}"] = {"ms": round(ms, 3), "glups": round(n ** 3 * steps / (ms * 1e-3) / 1e9, 2),
+                                     "roofline": roofline(float(bpl) * n ** 3, ms / steps, f"fdtd3d_{name}_{form}")}
         a.free()
         b.free()
+    out["autotune"] = [l for l in ctx.tune_report().splitlines() if "kernel 3 " in l]
     return out
 
 

@@ -70,6 +70,9 @@ int appfis_ctx_device(appfis_ctx* ctx, int* device);
 void* appfis_ctx_stream(appfis_ctx* ctx); /* the hipStream_t, for header-side launches */
 int appfis_sync(appfis_ctx* ctx);
 int appfis_device_name(appfis_ctx* ctx, char* buf, int len);
+/* Text of every autotuning decision taken on this context (candidate timings and the
+ * chosen variant per kernel / dtype / shape); the CHOSEN variants tune on first use. */
+int appfis_tune_report(appfis_ctx* ctx, char* buf, int len);
 
 /* ---- memory (replaces Grid::CreateSubGridsGPU / CopySubGridsGPUToCPU,
  *      reference Appfis/Grid.hpp:403-479) --------------------------------------- */
@@ -158,7 +161,9 @@ int appfis_gol(appfis_ctx* ctx, uint8_t* buf0, uint8_t* buf1, int nx, int ny, in
 /* ---- FDTD3d radius-4 star (config form of FDTD3d/src/fdtd3d_appfis.hpp:129-143):
  * u' = c[0]*u + sum_{r=1..4} c[r]*(6 neighbours at distance r); ghost 4 held at 0.
  * dtype F32 / F64 (arithmetic in that type) or F16 / BF16 (storage only, FP32
- * arithmetic).  Field in bufs[steps % 2] at exit. */
+ * arithmetic).  Field in bufs[steps % 2] at exit.  PLAIN runs the default tiling;
+ * CHOSEN runs the tiling the autotuner picked for this dtype and shape (timed once on
+ * scratch buffers, cached in the context) -- both compute identical values. */
 int appfis_fdtd3d(appfis_ctx* ctx, int dtype, void* buf0, void* buf1, int nx, int ny, int nz,
                   const double* coeff5, int steps, int variant);
 

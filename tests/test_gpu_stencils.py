@@ -100,13 +100,14 @@ def _fdtd_pair(ctx, n, dt, seed=1004):
     return a, ctx.from_numpy(a, dt), ctx.array(shape, dt)
 
 
+@pytest.mark.parametrize("variant", [hip.PLAIN, hip.CHOSEN])
 @pytest.mark.parametrize("dt", [hip.F32, hip.F64, hip.F16, hip.BF16])
-@pytest.mark.parametrize("n,steps", [((37, 29, 23), 4), ((64, 64, 64), 3), ((70, 17, 130), 2)])
-def test_fdtd3d_bit_exact(ctx, dt, n, steps):
+@pytest.mark.parametrize("n,steps", [((37, 29, 23), 4), ((64, 64, 64), 3), ((70, 17, 130), 2), ((130, 66, 70), 3)])
+def test_fdtd3d_bit_exact(ctx, dt, n, steps, variant):
     nx, ny, nz = n
     coeff = FDTD_COEFF if dt not in (hip.F16,) else [c / 2.0625 for c in FDTD_COEFF]
     a, d0, d1 = _fdtd_pair(ctx, n, dt)
-    ctx.fdtd3d(d0, d1, nx, ny, nz, coeff, steps)
+    ctx.fdtd3d(d0, d1, nx, ny, nz, coeff, steps, variant)
     ref = oracle.fdtd3d(a.copy(), np.zeros_like(a), nx, ny, nz, coeff, steps, bf16=(dt == hip.BF16))
     got = (d0, d1)[steps % 2].download()
     assert np.array_equal(got.view(np.uint8), ref.view(np.uint8))
