@@ -78,6 +78,16 @@ int check_shape(int ndim, const int* dims, int* d);
 
 inline unsigned ceil_div(size_t a, size_t b) { return (unsigned)((a + b - 1) / b); }
 
+// Workgroup b of a launch is placed on XCD b % 8 (round-robin over gfx950's 8 XCDs, each
+// with its own L2).  xcd_tile maps b to a tile index such that every XCD takes one
+// contiguous run of the nb tiles: tiles that read each other's halo rows then share an
+// L2 instead of fetching the same lines from HBM once per XCD.  Bijective on [0, nb).
+__device__ __forceinline__ unsigned xcd_tile(unsigned b, unsigned nb)
+{
+    const unsigned x = b & 7u, q = nb >> 3, r = nb & 7u;
+    return x * q + (x < r ? x : r) + (b >> 3);
+}
+
 // splitmix64 (Steele, Lea, Flood 2014): r(seed, n) = mix(seed + (n+1)*golden)
 __host__ __device__ inline uint64_t mix64(uint64_t z)
 {

@@ -93,18 +93,33 @@ int tune_select(appfis_ctx* ctx, const int key[6], int ncand, const char* const*
                       std::to_string(key[2]) + "x" + std::to_string(key[3]) + "x" + std::to_string(key[4]) + ":";
     int best = 0;
     float best_ms = 0.f;
+    // events of its own: the caller's timer (ctx->ev_start/ev_stop) may be running
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    APPFIS_HIP_CHECK(hipEventCreate(&e0));
+    if (hipEventCreate(&e1) != hipSuccess) {
+        (void)hipEventDestroy(e0);
+        return hip_fail(hipErrorOutOfMemory, "hipEventCreate", __FILE__, __LINE__);
+    }
+    struct Guard {
+        hipEvent_t a, b;
+        ~Guard()
+        {
+            (void)hipEventDestroy(a);
+            (void)hipEventDestroy(b);
+        }
+    } guard{e0, e1};
     for (int i = 0; i < ncand; i++) {
         int st = run_variant(i);  // warm-up (code load, caches)
         if (st) return st;
-        APPFIS_HIP_CHECK(hipEventRecord(ctx->ev_start, ctx->stream));
+        APPFIS_HIP_CHECK(hipEventRecord(e0, ctx->stream));
         for (int r = 0; r < reps; r++) {
             st = run_variant(i);
             if (st) return st;
         }
-        APPFIS_HIP_CHECK(hipEventRecord(ctx->ev_stop, ctx->stream));
-        APPFIS_HIP_CHECK(hipEventSynchronize(ctx->ev_stop));
+        APPFIS_HIP_CHECK(hipEventRecord(e1, ctx->stream));
+        APPFIS_HIP_CHECK(hipEventSynchronize(e1));
         float ms = 0.f;
-        APPFIS_HIP_CHECK(hipEventElapsedTime(&ms, ctx->ev_start, ctx->ev_stop));
+        APPFIS_HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
         ms /= (float)reps;
         char line[160];
         snprintf(line, sizeof(line), " [%s %.4f ms]", names[i], ms);
@@ -287,10 +302,29 @@ int appfis_memcpy_d2h(appfis_ctx* ctx, void* dst, const void* src, size_t bytes)
     return APPFIS_OK;
 }
 
+// 16-byte copy, one element per thread: each byte is read once and written once
+__global__ __launch_bounds__(256) void k_copy16(const uint4* __restrict__ src, uint4* __restrict__ dst, size_t n)
+{
+    const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
+    if (i < n) dst[i] = src[i];
+}
+
 int appfis_memcpy_d2d(appfis_ctx* ctx, void* dst, const void* src, size_t bytes)
 {
     APPFIS_REQUIRE(ctx && dst && src, "NULL argument");
-    APPFIS_HIP_CHECK(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, ctx->stream));
+    size_t head = 0;
+    if ((((uintptr_t)dst | (uintptr_t)src) & 15) == 0) {
+        const size_t n = bytes / 16;
+        head = n * 16;
+        if (n) {
+            hipLaunchKernelGGL(k_copy16, dim3(ceil_div(n, 256)), dim3(256), 0, ctx->stream, (const uint4*)src,
+                               (uint4*)dst, n);
+            APPFIS_LAUNCH_CHECK();
+        }
+    }
+    if (bytes > head)
+        APPFIS_HIP_CHECK(hipMemcpyAsync((char*)dst + head, (const char*)src + head, bytes - head,
+                                        hipMemcpyDeviceToDevice, ctx->stream));
     return APPFIS_OK;
 }
 

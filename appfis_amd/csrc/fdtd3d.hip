@@ -81,15 +81,13 @@ struct Vec {
     }
 };
 
-// Tile coordinates of this block.  Consecutive block ids land on consecutive XCDs, so
-// the linear id is remapped to give every XCD a contiguous run of tiles (x fastest):
-// tiles that share halos then share an L2.
+// Tile coordinates of this block: every XCD takes a contiguous run of tiles (x fastest),
+// so tiles that share halos share an L2.
 __device__ inline void tile_of(int& bx, int& by, int& bz)
 {
     const int gx = gridDim.x, gy = gridDim.y;
-    const int nb = gx * gy * gridDim.z;
-    int bid = blockIdx.x + gx * (blockIdx.y + gy * blockIdx.z);
-    if ((nb & 7) == 0) bid = (bid & 7) * (nb >> 3) + (bid >> 3);
+    const unsigned nb = gx * gy * gridDim.z;
+    const int bid = (int)xcd_tile(blockIdx.x + gx * (blockIdx.y + gy * blockIdx.z), nb);
     bx = bid % gx;
     const int r = bid / gx;
     by = r % gy;

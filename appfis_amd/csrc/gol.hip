@@ -15,7 +15,8 @@
 //   wrapped value new(xw, yw), recomputed from the current buffer whose ring is valid,
 //   so the written buffer leaves with its ring already exchanged: no second launch.
 //
-// CHOSEN: implemented in gol_bits.hip (bit-packed, several generations per launch).
+// CHOSEN: gol_bits.hip (bit-packed copy of the field, 32 cells per word) when nx % 32 == 0,
+//   this byte kernel otherwise.
 #include "common.hpp"
 
 using namespace appfis;
@@ -73,7 +74,8 @@ __global__ __launch_bounds__(256) void k_gol_plain(const uint8_t* __restrict__ u
     // below-row window starts at p + W: offset DB.
     constexpr int DA = (16 - DB) & 15;
     // nbytes < 2^31 (checked on the host): 32-bit offsets throughout
-    const int c = (int)(blockIdx.x * 256 + threadIdx.x);
+    // blocks two apart read each other's rows: keep them on one XCD (shared L2)
+    const int c = (int)(xcd_tile(blockIdx.x, gridDim.x) * 256 + threadIdx.x);
     const int lane = threadIdx.x & 63;
     const int p0 = c * 16;
     const bool valid = c < nchunks;
@@ -183,8 +185,10 @@ extern "C" int appfis_gol(appfis_ctx* ctx, uint8_t* buf0, uint8_t* buf1, int nx,
     APPFIS_REQUIRE(gens >= 0, "gens must be >= 0");
     APPFIS_REQUIRE(((uintptr_t)buf0 & 15) == 0 && ((uintptr_t)buf1 & 15) == 0, "GoL buffers must be 16-byte aligned");
     if (gens == 0) return APPFIS_OK;
-    if (variant == APPFIS_CHOSEN) return gol_bits_run(ctx, buf0, buf1, nx, ny, gens, tsteps);
-    APPFIS_REQUIRE(variant == APPFIS_PLAIN, "gol: unknown variant %d", variant);
+    APPFIS_REQUIRE(variant == APPFIS_PLAIN || variant == APPFIS_CHOSEN, "gol: unknown variant %d", variant);
+    // the bit-packed form needs the periodic wrap on word boundaries; other widths run
+    // the byte kernel, which gives the same bytes
+    if (variant == APPFIS_CHOSEN && nx % 32 == 0) return gol_bits_run(ctx, buf0, buf1, nx, ny, gens, tsteps);
     const int W = nx + 2, H = ny + 2;
     const int nbytes = W * H;
     const int nchunks = (nbytes + 15) / 16;

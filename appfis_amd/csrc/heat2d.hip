@@ -19,21 +19,41 @@ using namespace appfis;
 
 namespace {
 
-template <typename T>
+// One sweep; a thread owns RY consecutive rows of one column, so the column's RY+2
+// values are loaded once and its loads are all in flight together.  Block 64 x 4
+// threads = a 64 x 4*RY tile; tiles are spread over the XCDs in contiguous runs.
+template <typename T, int RY>
 __global__ __launch_bounds__(256) void k_heat2d_plain(const T* __restrict__ u, T* __restrict__ v, int nx, int ny,
                                                       T c)
 {
     const int W = nx + 2;
-    const int i = blockIdx.x * 64 + threadIdx.x + 1;
-    const int j = blockIdx.y * 4 + threadIdx.y + 1;
-    if (i > nx || j > ny) return;
-    const size_t o = (size_t)i + (size_t)W * (size_t)j;
-    const T uc = u[o];
-    T s = u[o - W] + u[o + W];
-    s = s + u[o + 1];
-    s = s + u[o - 1];
-    s = s - (T)4 * uc;
-    v[o] = uc + c * s;
+    const unsigned gx = gridDim.x;
+    const unsigned t = xcd_tile(blockIdx.x + gx * blockIdx.y, gx * gridDim.y);
+    const int i = (int)(t % gx) * 64 + threadIdx.x + 1;
+    const int j0 = ((int)(t / gx) * 4 + threadIdx.y) * RY + 1;
+    if (i > nx || j0 > ny) return;
+    T col[RY + 2], e[RY], w[RY];
+#pragma unroll
+    for (int r = 0; r < RY + 2; r++) {
+        const int j = j0 - 1 + r;
+        col[r] = j <= ny + 1 ? u[i + (size_t)W * j] : (T)0;
+    }
+#pragma unroll
+    for (int r = 0; r < RY; r++) {
+        const size_t o = i + (size_t)W * (j0 + r);
+        e[r] = j0 + r <= ny ? u[o + 1] : (T)0;
+        w[r] = j0 + r <= ny ? u[o - 1] : (T)0;
+    }
+#pragma unroll
+    for (int r = 0; r < RY; r++) {
+        if (j0 + r > ny) break;
+        const T uc = col[r + 1];
+        T s = col[r] + col[r + 2];
+        s = s + e[r];
+        s = s + w[r];
+        s = s - (T)4 * uc;
+        v[i + (size_t)W * (j0 + r)] = uc + c * s;
+    }
 }
 
 // Temporal blocking: TX x TY output tile, TS sweeps per launch, 256 threads.
@@ -47,8 +67,11 @@ __global__ __launch_bounds__(256) void k_heat2d_tb(const T* __restrict__ u, T* _
     __shared__ T tile[RY][RX];
     const int W = nx + 2, H = ny + 2;
     // window origin in padded coordinates (may be negative / past the array)
-    const int gx0 = blockIdx.x * TX + 1 - TS;
-    const int gy0 = blockIdx.y * TY + 1 - TS;
+    const unsigned nbx = gridDim.x;
+    const unsigned t = xcd_tile(blockIdx.x + nbx * blockIdx.y, nbx * gridDim.y);
+    const int tbx = (int)(t % nbx), tby = (int)(t / nbx);
+    const int gx0 = tbx * TX + 1 - TS;
+    const int gy0 = tby * TY + 1 - TS;
     const int tid = threadIdx.x;
     for (int p = tid; p < NCELL; p += 256) {
         int ly = p / RX, lx = p - ly * RX;
@@ -93,18 +116,21 @@ __global__ __launch_bounds__(256) void k_heat2d_tb(const T* __restrict__ u, T* _
     // write the TX x TY centre (interior cells only)
     for (int p = tid; p < TX * TY; p += 256) {
         int ly = p / TX, lx = p - ly * TX;
-        int gx = blockIdx.x * TX + 1 + lx, gy = blockIdx.y * TY + 1 + ly;
+        int gx = tbx * TX + 1 + lx, gy = tby * TY + 1 + ly;
         if (gx <= nx && gy <= ny) v[(size_t)gx + (size_t)W * (size_t)gy] = tile[ly + TS][lx + TS];
     }
 }
 
+constexpr int PLAIN_RY = 4;
+
 template <typename T>
 int run_plain(appfis_ctx* ctx, T* b0, T* b1, int nx, int ny, T c, int iters)
 {
-    dim3 block(64, 4), grid(ceil_div(nx, 64), ceil_div(ny, 4));
+    dim3 block(64, 4), grid(ceil_div(nx, 64), ceil_div(ny, 4 * PLAIN_RY));
     T* buf[2] = {b0, b1};
     for (int it = 0; it < iters; it++) {
-        hipLaunchKernelGGL(k_heat2d_plain<T>, grid, block, 0, ctx->stream, buf[it & 1], buf[(it + 1) & 1], nx, ny, c);
+        hipLaunchKernelGGL((k_heat2d_plain<T, PLAIN_RY>), grid, block, 0, ctx->stream, buf[it & 1],
+                           buf[(it + 1) & 1], nx, ny, c);
         APPFIS_LAUNCH_CHECK();
     }
     return APPFIS_OK;
@@ -125,12 +151,12 @@ inline int plan_launches(int iters, int ts, int* counts, int max_launches)
 }
 
 template <typename T, int TX, int TY, int TS>
-int run_tb(appfis_ctx* ctx, T* b0, T* b1, int nx, int ny, T c, int iters)
+int run_tb(appfis_ctx* ctx, T* b0, T* b1, int nx, int ny, T c, int iters, int depth)
 {
     T* buf[2] = {b0, b1};
     dim3 grid(ceil_div(nx, TX), ceil_div(ny, TY));
     std::vector<int> counts((size_t)iters);
-    int n = plan_launches(iters, TS, counts.data(), iters);
+    int n = plan_launches(iters, depth < TS ? depth : TS, counts.data(), iters);
     if (n <= 0) return invalid("heat2d: cannot split %d sweeps", iters);
     for (int l = 0; l < n; l++) {
         hipLaunchKernelGGL((k_heat2d_tb<T, TX, TY, TS>), grid, dim3(256), 0, ctx->stream, buf[l & 1],
@@ -139,6 +165,30 @@ int run_tb(appfis_ctx* ctx, T* b0, T* b1, int nx, int ny, T c, int iters)
     }
     return APPFIS_OK;
 }
+
+template <typename T, int TX, int TY, int TS>
+int run_tb_v(appfis_ctx* ctx, void* b0, void* b1, int nx, int ny, double c, int iters, int depth)
+{
+    return run_tb<T, TX, TY, TS>(ctx, (T*)b0, (T*)b1, nx, ny, (T)c, iters, depth);
+}
+
+struct HeatVariant {
+    const char* name;
+    int (*fn)(appfis_ctx*, void*, void*, int, int, double, int, int);
+    int ts;  // sweeps per launch at most
+};
+
+// temporal-blocking candidates (output tile TX x TY, TS sweeps per launch)
+static const HeatVariant kHeatF64[] = {
+    {"tb 32x32 ts8", run_tb_v<double, 32, 32, 8>, 8},   {"tb 32x32 ts4", run_tb_v<double, 32, 32, 4>, 4},
+    {"tb 64x32 ts8", run_tb_v<double, 64, 32, 8>, 8},   {"tb 64x64 ts8", run_tb_v<double, 64, 64, 8>, 8},
+    {"tb 64x32 ts12", run_tb_v<double, 64, 32, 12>, 12}, {"tb 64x32 ts16", run_tb_v<double, 64, 32, 16>, 16},
+};
+static const HeatVariant kHeatF32[] = {
+    {"tb 64x32 ts8", run_tb_v<float, 64, 32, 8>, 8},    {"tb 32x32 ts4", run_tb_v<float, 32, 32, 4>, 4},
+    {"tb 64x64 ts8", run_tb_v<float, 64, 64, 8>, 8},    {"tb 64x64 ts16", run_tb_v<float, 64, 64, 16>, 16},
+    {"tb 128x64 ts16", run_tb_v<float, 128, 64, 16>, 16},
+};
 
 }  // namespace
 
@@ -157,16 +207,39 @@ extern "C" int appfis_heat2d(appfis_ctx* ctx, int dtype, void* buf0, void* buf1,
         return invalid("heat2d: dtype must be F64 or F32");
     }
     if (variant == APPFIS_CHOSEN) {
-        int ts = tsteps <= 0 ? 8 : tsteps;
-        if (dtype == APPFIS_F64) {
-            if (ts <= 4) return run_tb<double, 32, 32, 4>(ctx, (double*)buf0, (double*)buf1, nx, ny, c, iters);
-            return run_tb<double, 32, 32, 8>(ctx, (double*)buf0, (double*)buf1, nx, ny, c, iters);
+        if (dtype != APPFIS_F64 && dtype != APPFIS_F32) return invalid("heat2d: dtype must be F64 or F32");
+        const HeatVariant* tab = dtype == APPFIS_F64 ? kHeatF64 : kHeatF32;
+        const int ntab = dtype == APPFIS_F64 ? (int)(sizeof(kHeatF64) / sizeof(kHeatF64[0]))
+                                             : (int)(sizeof(kHeatF32) / sizeof(kHeatF32[0]));
+        if (tsteps > 0) {
+            // caller-fixed depth: the first candidate that holds that many sweeps per launch
+            for (int i = 0; i < ntab; i++)
+                if (tab[i].ts >= tsteps) return tab[i].fn(ctx, buf0, buf1, nx, ny, c, iters, tsteps);
+            return invalid("heat2d: tsteps %d exceeds the deepest temporal blocking (16)", tsteps);
         }
-        if (dtype == APPFIS_F32) {
-            if (ts <= 4) return run_tb<float, 32, 32, 4>(ctx, (float*)buf0, (float*)buf1, nx, ny, (float)c, iters);
-            return run_tb<float, 64, 32, 8>(ctx, (float*)buf0, (float*)buf1, nx, ny, (float)c, iters);
+        // autotuned tile shape and depth, timed once per shape on scratch buffers; every
+        // candidate computes the same values in the same order
+        const int key[6] = {1, dtype, nx, ny, 0, 0};
+        bool cached = false;
+        for (const appfis_tune_entry& e : ctx->tuned)
+            if (memcmp(e.key, key, sizeof(key)) == 0) cached = true;
+        void* s0 = nullptr;
+        void* s1 = nullptr;
+        if (!cached) {
+            const size_t half = ((size_t)(nx + 2) * (ny + 2) * dtype_size(dtype) + 255) / 256 * 256;
+            int st = ensure_work(ctx, 2 * half);
+            if (st) return st;
+            s0 = ctx->work;
+            s1 = (char*)ctx->work + half;
+            APPFIS_HIP_CHECK(hipMemsetAsync(ctx->work, 0, 2 * half, ctx->stream));
         }
-        return invalid("heat2d: dtype must be F64 or F32");
+        std::vector<const char*> names;
+        for (int i = 0; i < ntab; i++) names.push_back(tab[i].name);
+        int pick = 0;
+        int st = tune_select(ctx, key, ntab, names.data(),
+                             [&](int i) { return tab[i].fn(ctx, s0, s1, nx, ny, c, 48, tab[i].ts); }, 3, &pick);
+        if (st) return st;
+        return tab[pick].fn(ctx, buf0, buf1, nx, ny, c, iters, tab[pick].ts);
     }
     return invalid("heat2d: unknown variant %d", variant);
 }

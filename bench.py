@@ -68,19 +68,33 @@ def timed(ctx, fn):
 # GameOfLife (headline)
 # ------------------------------------------------------------------------------------
 def bench_gol(ctx, hip, steps, warmup, variant, nx=8192, ny=8192):
+    """One call of `steps` generations, timed.  The per-launch time of the generation
+    kernel is the difference quotient against a call of steps // 10 generations, which
+    removes the per-call work (the PLAIN form's initial ring exchange; the CHOSEN form's
+    pack, two unpacks and exchanges)."""
     d0 = ctx.array((ny + 2, nx + 2), hip.U8)
     d1 = ctx.array((ny + 2, nx + 2), hip.U8)
     ctx.fill_bernoulli(d0, 1, 1002, 0.25)
     if warmup:
         ctx.gol(d0, d1, nx, ny, warmup, variant)
-        ctx.fill_bernoulli(d0, 1, 1002, 0.25)
+    short = max(1, steps // 10)
+    ctx.fill_bernoulli(d0, 1, 1002, 0.25)
+    ms_short, _ = timed(ctx, lambda: ctx.gol(d0, d1, nx, ny, short, variant))
+    ctx.fill_bernoulli(d0, 1, 1002, 0.25)
     ms, wall = timed(ctx, lambda: ctx.gol(d0, d1, nx, ny, steps, variant))
     final = (d0, d1)[steps % 2]
     live = int(ctx.reduce_sum_wide(final, [1, nx + 1, 1, ny + 1]))
+    launch_ms = (ms - ms_short) / (steps - short) if steps > short else ms / steps
     res = {"form": "plain" if variant == hip.PLAIN else "chosen", "ms": round(ms, 4), "wall_s": round(wall, 4),
            "glups": round(nx * ny * steps / (ms * 1e-3) / 1e9, 2), "live_count": live}
     if variant == hip.PLAIN:
-        res["roofline"] = roofline(2.0 * nx * ny, ms / steps, "gol_plain")
+        # uint8 field read + written once per generation
+        res["roofline"] = roofline(2.0 * nx * ny, launch_ms, "gol_plain")
+    else:
+        # bit-packed field (1 bit per cell) read + written once per generation
+        res["roofline"] = roofline(2.0 * nx * ny / 8, launch_ms, "gol_bits_step")
+    d0.free()
+    d1.free()
     return res
 
 
@@ -104,6 +118,21 @@ def cpu_baseline_gol(nx=8192, ny=8192, target_s=12.0):
                       f"restatement (schedule(dynamic,1) over rows), {dt:.1f} s on {os.cpu_count()} host CPUs"}
 
 
+COPY_BYTES = 1 << 30
+
+
+def calibrate_copy(ctx, hip):
+    """A device-to-device copy of COPY_BYTES, run 4 times: it reads and writes exactly
+    COPY_BYTES per dispatch, which scales the FETCH_SIZE / WRITE_SIZE counters."""
+    a = ctx.array((COPY_BYTES,), hip.U8)
+    b = ctx.array((COPY_BYTES,), hip.U8)
+    for _ in range(4):
+        hip.check(hip.lib().appfis_memcpy_d2d(ctx.handle, b.ptr, a.ptr, COPY_BYTES), "d2d")
+    ctx.sync()
+    a.free()
+    b.free()
+
+
 # ------------------------------------------------------------------------------------
 # suite
 # ------------------------------------------------------------------------------------
@@ -117,14 +146,29 @@ def bench_heat(ctx, hip, n=1024, iters=500):
         ctx.fill_uniform(a, 1, 1001)
         ms, _ = timed(ctx, lambda: ctx.heat2d(a, b, n, n, 0.1, iters, variant))
         chk = float(ctx.reduce_sum_wide(a, [1, n + 1, 1, n + 1]))
-        launches = iters if variant == hip.PLAIN else len(_plan(iters, 8))
+        if variant == hip.PLAIN:
+            launches = iters
+        else:
+            tname = _tuned(ctx, 1, hip.F64)  # e.g. "tb 64x32 ts8"
+            launches = len(_plan(iters, int(tname.rsplit("ts", 1)[1])))
         per_launch_sweeps = iters / launches
         r = {"ms": round(ms, 4), "glups": round(n * n * iters / (ms * 1e-3) / 1e9, 2), "checksum": chk,
              "roofline": roofline(16.0 * n * n * per_launch_sweeps, ms / launches,
                                   "heat2d_" + name)}
+        if variant == hip.CHOSEN:
+            r["tuned"] = tname
         out[name] = r
     out["workload"] = f"Heat2D {n}x{n} FP64 5-point, {iters} sweeps (16 B/LUP)"
     return out
+
+
+def _tuned(ctx, kernel, dtype):
+    """Name of the variant the autotuner picked last for (kernel, dtype)."""
+    pick = None
+    for line in ctx.tune_report().splitlines():
+        if line.startswith(f"autotune kernel {kernel} dtype {dtype} "):
+            pick = line.rsplit("-> ", 1)[1].strip()
+    return pick
 
 
 def _plan(iters, ts):
@@ -283,6 +327,10 @@ def main():
     ap.add_argument("--headline-only", action="store_true")
     ap.add_argument("--only", default="", help="comma list of suite entries (heat,fdtd,himeno,flood)")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--quick", action="store_true",
+                    help="suite entries at their config sizes but few sweeps (for profiler runs)")
+    ap.add_argument("--calibrate", action="store_true",
+                    help="first run a 1 GiB device-to-device copy 4 times (HBM counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: this build has no multi-GPU path (asked for --gpus {args.gpus}); "
@@ -293,6 +341,8 @@ def main():
 
     ctx = hip.Context(0)
     device = ctx.name().strip()
+    if args.calibrate:
+        calibrate_copy(ctx, hip)
     plain = bench_gol(ctx, hip, args.steps, args.warmup, hip.PLAIN)
     variants = {"plain": plain}
     best = plain
@@ -308,11 +358,13 @@ def main():
     suite = {}
     if not args.headline_only:
         only = set(x for x in args.only.split(",") if x) or {"heat", "fdtd", "himeno", "flood"}
+        quick = {"heat": {"iters": 40}, "fdtd": {"steps": 4}, "himeno": {"iters": 10},
+                 "flood": {"steps": 40}} if args.quick else {}
         for name, fn in (("heat", bench_heat), ("fdtd", bench_fdtd), ("himeno", bench_himeno),
                          ("flood", bench_flood)):
             if name in only:
                 print(f"[bench] {name}", file=sys.stderr, flush=True)
-                suite[name] = fn(ctx, hip)
+                suite[name] = fn(ctx, hip, **quick.get(name, {}))
     cpu = None if args.no_cpu else cpu_baseline_gol()
     line = {
         "metric": "GLUP/s (GameOfLife 8192^2 uint8 B3/S23 periodic); suite: GLUP/s and % of 8 TB/s per app",
@@ -329,9 +381,9 @@ def main():
         "data": "synthetic (splitmix64 Bernoulli p=0.25, seed 1002, generated on device)",
         "config": {"workload": "GameOfLife 2D 8192x8192 uint8, periodic, B3/S23, 1 step = 1 generation",
                    "nx": 8192, "ny": 8192, "form": best["form"], "device": device},
-        "roofline": plain["roofline"],
+        "roofline": best["roofline"],
         "cpu_baseline": cpu,
-        "variants": {k: {kk: vv for kk, vv in v.items() if kk != "roofline"} for k, v in variants.items()},
+        "variants": variants,
         "suite": suite,
     }
     print(json.dumps(line), flush=True)

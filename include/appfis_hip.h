@@ -154,7 +154,9 @@ int appfis_heat2d(appfis_ctx* ctx, int dtype, void* buf0, void* buf1, int nx, in
  *      game_appfis.hpp:121-135 plus the periodic exchange of Communication.hpp:613-678).
  * bufs as for heat2d (ghost 1, padded (nx+2)*(ny+2)).  The ghost ring of the
  * written buffer is refreshed every generation.  PLAIN: one generation per launch;
- * CHOSEN: bit-packed field, `tsteps` generations per launch. */
+ * CHOSEN (nx % 32 == 0): the generations run on a bit-packed copy of the field (32
+ * cells per word), both buffers are left exactly as PLAIN leaves them; other widths
+ * run the PLAIN kernel.  `tsteps` is reserved (pass 0).  Cells must hold 0 or 1. */
 int appfis_gol(appfis_ctx* ctx, uint8_t* buf0, uint8_t* buf1, int nx, int ny, int gens,
                int variant, int tsteps);
 

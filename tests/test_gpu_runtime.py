@@ -24,6 +24,20 @@ def test_abi_version_and_roundtrip(ctx):
     assert np.array_equal(d.download(), a)
 
 
+@pytest.mark.parametrize("nbytes,src_off,dst_off", [(4096, 0, 0), (1000003, 0, 0), (77, 16, 32), (5000, 3, 0),
+                                                     (16, 0, 16), (100000, 16, 5)])
+def test_memcpy_d2d(ctx, nbytes, src_off, dst_off):
+    # 16-byte aligned copies run the library's copy kernel (+ a tail copy), others the runtime's
+    rs = np.random.RandomState(nbytes)
+    a = rs.randint(0, 256, size=nbytes + 64).astype(np.uint8)
+    src = ctx.from_numpy(a)
+    dst = ctx.array((nbytes + 64,), hip.U8)
+    hip.check(hip.lib().appfis_memcpy_d2d(ctx.handle, dst.ptr + dst_off, src.ptr + src_off, nbytes), "d2d")
+    got = dst.download()
+    assert np.array_equal(got[dst_off:dst_off + nbytes], a[src_off:src_off + nbytes])
+    assert not got[:dst_off].any() and not got[dst_off + nbytes:].any()
+
+
 @pytest.mark.parametrize("dt", [hip.F64, hip.F32, hip.F16])
 @pytest.mark.parametrize("shape,ghost", [((37, 29), 1), ((9, 13, 17), 4), ((100,), 2)])
 def test_fill_uniform_matches_oracle(ctx, dt, shape, ghost):

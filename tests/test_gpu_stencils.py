@@ -23,7 +23,7 @@ def _heat_pair(ctx, nx, ny, dt, seed=1001):
 
 
 @pytest.mark.parametrize("dt", [hip.F64, hip.F32])
-@pytest.mark.parametrize("nx,ny,iters", [(37, 29, 11), (64, 64, 16), (130, 67, 9), (1, 5, 3)])
+@pytest.mark.parametrize("nx,ny,iters", [(37, 29, 11), (64, 64, 16), (130, 67, 9), (1, 5, 3), (300, 250, 37)])
 @pytest.mark.parametrize("variant", [hip.PLAIN, hip.CHOSEN])
 def test_heat2d_bit_exact(ctx, dt, nx, ny, iters, variant):
     a, d0, d1 = _heat_pair(ctx, nx, ny, dt)
@@ -33,7 +33,7 @@ def test_heat2d_bit_exact(ctx, dt, nx, ny, iters, variant):
     assert np.array_equal(got, ref)
 
 
-@pytest.mark.parametrize("tsteps", [1, 3, 4, 8])
+@pytest.mark.parametrize("tsteps", [1, 3, 4, 8, 11, 16])
 def test_heat2d_chosen_depths(ctx, tsteps):
     nx, ny, iters = 200, 150, 23
     a, d0, d1 = _heat_pair(ctx, nx, ny, hip.F64)
@@ -90,6 +90,50 @@ def test_gol_full_config_prefix(ctx):
     ctx.gol(d0, d1, nx, ny, 12, hip.PLAIN)
     ref = oracle.gol_u8(a.copy(), np.zeros_like(a), nx, ny, 12)
     assert np.array_equal(d0.download(), ref)
+
+
+@pytest.mark.parametrize("nx,ny,gens", [(32, 2, 3), (32, 9, 1), (64, 64, 10), (96, 7, 13), (160, 33, 21),
+                                        (256, 100, 2), (2048, 17, 5), (94, 33, 7), (50, 40, 6)])
+def test_gol_chosen_both_buffers(ctx, nx, ny, gens):
+    # bit-packed form (nx % 32 == 0; other widths take the byte kernel): buf[gens % 2]
+    # holds generation gens and the other buffer generation gens-1, rings exchanged
+    a, d0, d1 = _gol_pair(ctx, nx, ny, seed=nx * 7 + ny)
+    ctx.gol(d0, d1, nx, ny, gens, hip.CHOSEN)
+    ref = oracle.gol_u8(a.copy(), np.zeros_like(a), nx, ny, gens)
+    assert np.array_equal((d0, d1)[gens % 2].download(), ref)
+    prev = oracle.gol_u8(a.copy(), np.zeros_like(a), nx, ny, gens - 1)
+    assert np.array_equal((d0, d1)[(gens + 1) % 2].download(), prev)
+
+
+def test_gol_chosen_glider_period(ctx):
+    # a glider on an N x N torus returns to its start after 4N generations
+    n = 64
+    a = np.zeros((n + 2, n + 2), dtype=np.uint8)
+    for x, y in [(2, 1), (3, 2), (1, 3), (2, 3), (3, 3)]:
+        a[y + 1, x + 1] = 1
+    d0, d1 = ctx.from_numpy(a), ctx.array(a.shape, hip.U8)
+    ctx.gol(d0, d1, n, n, 4 * n, hip.CHOSEN)
+    got = d0.download()
+    assert np.array_equal(got[1:-1, 1:-1], a[1:-1, 1:-1])
+    ctx.gol(d0, d1, n, n, 2 * n, hip.CHOSEN)
+    assert not np.array_equal(d0.download()[1:-1, 1:-1], a[1:-1, 1:-1])
+
+
+def test_gol_chosen_full_config_prefix(ctx):
+    # 8192^2 config: 12 generations of the bit-packed form against the oracle, and
+    # 100 generations against the byte kernel
+    nx = ny = 8192
+    a, d0, d1 = _gol_pair(ctx, nx, ny)
+    ctx.gol(d0, d1, nx, ny, 12, hip.CHOSEN)
+    ref = oracle.gol_u8(a.copy(), np.zeros_like(a), nx, ny, 12)
+    assert np.array_equal(d0.download(), ref)
+    del ref
+    d0.upload(a)
+    ctx.gol(d0, d1, nx, ny, 100, hip.CHOSEN)
+    c0, c1 = d0.download(), d1.download()
+    d0.upload(a)
+    ctx.gol(d0, d1, nx, ny, 100, hip.PLAIN)
+    assert np.array_equal(d0.download(), c0) and np.array_equal(d1.download(), c1)
 
 
 def _fdtd_pair(ctx, n, dt, seed=1004):
