@@ -107,10 +107,169 @@ __global__ __launch_bounds__(256) void k_step(const uint32_t* __restrict__ P, ui
 
 constexpr int KROWS = 8;
 
+// Temporal blocking: G generations per launch.  A block of NWAVES waves spans whole
+// packed rows: NW = 64*WX words, lane l of wave (wx, wy) owns word 64*wx + l, and wave
+// row wy holds K consecutive rows in registers, so the block's window is WY*K rows of
+// the full periodic row width.  Its centre OUT = WY*K - 2G rows are exact after G
+// generations (each generation invalidates one more window row at each end); blocks
+// are spaced OUT rows apart.  Per generation:
+//   A  each wave publishes its lane-0 and lane-63 words (all K rows) to LDS; barrier
+//   B  horizontal sums of its K rows: the neighbour words come from lanes l-1 / l+1 by
+//      DPP wave shifts, and across wave edges (lanes 0 / 63) from the LDS words of the
+//      waves left / right (periodic in wx); the first and last rows' sums go to LDS;
+//      barrier
+//   C  vertical combine: the sums of the rows above / below the wave's band come from
+//      the waves above / below (zero outside the window), then the rule.
+template <int K, int G, int NWAVES>
+__global__ __launch_bounds__(64 * NWAVES) void k_step_tb(const uint32_t* __restrict__ P, uint32_t* __restrict__ Q,
+                                                          int NW, int ny, int WX, int OUT, int gens)
+{
+    __shared__ uint32_t edge_lo[NWAVES][K], edge_hi[NWAVES][K];  // lane-0 / lane-63 words
+    __shared__ uint32_t sum_top[2][NWAVES][64], sum_bot[2][NWAVES][64];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int wx = wv % WX, wy = wv / WX;
+    const int WY = NWAVES / WX;
+    const int w = wx * 64 + lane;
+    const int left = wy * WX + (wx == 0 ? WX - 1 : wx - 1);
+    const int right = wy * WX + (wx == WX - 1 ? 0 : wx + 1);
+    const int y0 = (int)blockIdx.x * OUT - G + wy * K;  // global row of register row 0
+    uint32_t R[K];
+#pragma unroll
+    for (int k = 0; k < K; k++) {
+        int y = y0 + k;
+        while (y < 0) y += ny;  // window rows may reach 2G below 0 or past ny: periodic
+        while (y >= ny) y -= ny;
+        R[k] = P[(size_t)y * NW + w];
+    }
+    for (int g = 0; g < gens; g++) {
+        if (lane == 0)
+#pragma unroll
+            for (int k = 0; k < K; k++) edge_lo[wv][k] = R[k];
+        if (lane == 63)
+#pragma unroll
+            for (int k = 0; k < K; k++) edge_hi[wv][k] = R[k];
+        __syncthreads();
+        // the neighbouring waves' edge words, read by every lane (one address per wave:
+        // LDS broadcast) so that no read waits behind a lane mask
+        uint32_t eh[K], el[K];
+#pragma unroll
+        for (int k = 0; k < K; k++) {
+            eh[k] = edge_hi[left][k];
+            el[k] = edge_lo[right][k];
+        }
+        uint32_t h0[K], h1[K];
+#pragma unroll
+        for (int k = 0; k < K; k++) {
+            // lane l receives lane l-1 (wave_shr:1) / lane l+1 (wave_shl:1); lanes 0 / 63
+            // have no source lane and keep the `old` operand: the neighbouring wave's word
+            const uint32_t l = (uint32_t)__builtin_amdgcn_update_dpp((int)eh[k], (int)R[k], 0x138, 0xF, 0xF, false);
+            const uint32_t r = (uint32_t)__builtin_amdgcn_update_dpp((int)el[k], (int)R[k], 0x130, 0xF, 0xF, false);
+            const uint32_t c = R[k];
+            const uint32_t Wc = __builtin_amdgcn_alignbit(c, l, 31);
+            const uint32_t Ec = __builtin_amdgcn_alignbit(r, c, 1);
+            h0[k] = Wc ^ c ^ Ec;
+            h1[k] = maj(Wc, c, Ec);
+        }
+        sum_top[0][wv][lane] = h0[0];
+        sum_top[1][wv][lane] = h1[0];
+        sum_bot[0][wv][lane] = h0[K - 1];
+        sum_bot[1][wv][lane] = h1[K - 1];
+        __syncthreads();
+        uint32_t a0 = 0u, a1 = 0u, z0 = 0u, z1 = 0u;
+        if (wy > 0) {
+            a0 = sum_bot[0][wv - WX][lane];
+            a1 = sum_bot[1][wv - WX][lane];
+        }
+        if (wy < WY - 1) {
+            z0 = sum_top[0][wv + WX][lane];
+            z1 = sum_top[1][wv + WX][lane];
+        }
+#pragma unroll
+        for (int k = 0; k < K; k++) {
+            const uint32_t u0 = k == 0 ? a0 : h0[k - 1], u1 = k == 0 ? a1 : h1[k - 1];
+            const uint32_t d0 = k == K - 1 ? z0 : h0[k + 1], d1 = k == K - 1 ? z1 : h1[k + 1];
+            const uint32_t s0 = u0 ^ h0[k] ^ d0;
+            const uint32_t c0 = maj(u0, h0[k], d0);
+            const uint32_t v1 = u1 ^ h1[k] ^ d1;
+            const uint32_t c1 = maj(u1, h1[k], d1);
+            const uint32_t s1 = v1 ^ c0;
+            const uint32_t s2 = c1 ^ (v1 & c0);
+            R[k] = (s0 & s1 & ~s2) | (R[k] & s2 & ~(s0 | s1));
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < K; k++) {
+        const int r = wy * K + k - G;  // output row within the block's OUT rows
+        const int y = (int)blockIdx.x * OUT + r;
+        if (r >= 0 && r < OUT && y < ny) Q[(size_t)y * NW + w] = R[k];
+    }
+}
+
+struct TbVariant {
+    const char* name;
+    void (*kernel)(const uint32_t*, uint32_t*, int, int, int, int, int);
+    int k, g, nwaves;
+};
+
+static const TbVariant kTb[] = {
+    {"tb k16 g8 w16", k_step_tb<16, 8, 16>, 16, 8, 16}, {"tb k10 g4 w16", k_step_tb<10, 4, 16>, 10, 4, 16},
+    {"tb k20 g4 w8", k_step_tb<20, 4, 8>, 20, 4, 8},    {"tb k12 g4 w8", k_step_tb<12, 4, 8>, 12, 4, 8},
+    {"tb k24 g8 w16", k_step_tb<24, 8, 16>, 24, 8, 16}, {"tb k8 g4 w16", k_step_tb<8, 4, 16>, 8, 4, 16},
+    {"tb k12 g4 w16", k_step_tb<12, 4, 16>, 12, 4, 16}, {"tb k32 g8 w8", k_step_tb<32, 8, 8>, 32, 8, 8},
+    {"tb k16 g4 w8", k_step_tb<16, 4, 8>, 16, 4, 8},    {"tb k11 g5 w16", k_step_tb<11, 5, 16>, 11, 5, 16},
+    {"tb k13 g6 w16", k_step_tb<13, 6, 16>, 13, 6, 16},
+};
+
 }  // namespace golbits
 
+namespace {
+
+// `gens` generations from P[*cur] with the one-generation kernel
+int run_step(appfis_ctx* ctx, uint32_t* const P[2], int* cur, int NW, int ny, int gens)
+{
+    const int nbands = (ny + golbits::KROWS - 1) / golbits::KROWS;
+    const unsigned nitems = (unsigned)NW * (unsigned)nbands;
+    for (int g = 0; g < gens; g++) {
+        hipLaunchKernelGGL(golbits::k_step<golbits::KROWS>, dim3(ceil_div(nitems, 256)), dim3(256), 0, ctx->stream,
+                           P[*cur], P[*cur ^ 1], NW, ny, nitems);
+        APPFIS_LAUNCH_CHECK();
+        *cur ^= 1;
+    }
+    return APPFIS_OK;
+}
+
+bool tb_fits(const golbits::TbVariant& v, int NW)
+{
+    if (NW % 64) return false;
+    const int WX = NW / 64;
+    return WX <= v.nwaves && v.nwaves % WX == 0 && (v.nwaves / WX) * v.k > 2 * v.g;
+}
+
+// `gens` generations from P[*cur] with temporal-blocking variant v (G per launch)
+int run_tb(appfis_ctx* ctx, const golbits::TbVariant& v, uint32_t* const P[2], int* cur, int NW, int ny, int gens)
+{
+    const int WX = NW / 64, WY = v.nwaves / WX;
+    const int OUT = WY * v.k - 2 * v.g;
+    const unsigned blocks = ceil_div((size_t)ny, (size_t)OUT);
+    for (int done = 0; done < gens;) {
+        const int n = gens - done < v.g ? gens - done : v.g;
+        hipLaunchKernelGGL(v.kernel, dim3(blocks), dim3(64 * v.nwaves), 0, ctx->stream, P[*cur], P[*cur ^ 1], NW, ny,
+                           WX, OUT, n);
+        APPFIS_LAUNCH_CHECK();
+        *cur ^= 1;
+        done += n;
+    }
+    return APPFIS_OK;
+}
+
+}  // namespace
+
 // Runs `gens` >= 1 generations of the nx x ny periodic field (nx % 32 == 0) held in
-// buf0; see the file comment for what the two buffers hold on return.
+// buf0; see the file comment for what the two buffers hold on return.  When the packed
+// row width is a multiple of 64 words (and at most 1024), generations 1 .. gens-1 run
+// temporally blocked
+// (tile height and depth autotuned per shape), and the last one with the
+// one-generation kernel, so that generation gens-1 is still at hand to unpack.
 int gol_bits_run(appfis_ctx* ctx, uint8_t* b0, uint8_t* b1, int nx, int ny, int gens, int tsteps)
 {
     (void)tsteps;
@@ -119,27 +278,52 @@ int gol_bits_run(appfis_ctx* ctx, uint8_t* b0, uint8_t* b1, int nx, int ny, int 
     const size_t half = (words * 4 + 255) / 256 * 256;
     int st = ensure_work(ctx, 2 * half);
     if (st) return st;
-    uint32_t* P[2] = {(uint32_t*)ctx->work, (uint32_t*)((char*)ctx->work + half)};
+    uint32_t* const P[2] = {(uint32_t*)ctx->work, (uint32_t*)((char*)ctx->work + half)};
+    // autotune the blocked variant on the two packed buffers before they hold the field
+    // (timing does not depend on the cell values)
+    std::vector<int> cand;
+    std::vector<const char*> names;
+    const int ntb = (int)(sizeof(golbits::kTb) / sizeof(golbits::kTb[0]));
+    for (int i = 0; i < ntb; i++)
+        if (tb_fits(golbits::kTb[i], NW)) {
+            cand.push_back(i);
+            names.push_back(golbits::kTb[i].name);
+        }
+    int pick = -1;
+    if (!cand.empty() && gens > 1) {
+        const int key[6] = {4, APPFIS_U8, nx, ny, 0, 0};
+        int c = 0;
+        st = tune_select(ctx, key, (int)cand.size(), names.data(),
+                         [&](int i) {
+                             int tmp = 0;
+                             return run_tb(ctx, golbits::kTb[cand[i]], P, &tmp, NW, ny, 48);
+                         },
+                         2, &c);
+        if (st) return st;
+        pick = cand[c];
+    }
     const unsigned pblocks = ceil_div(words, 256);
     hipLaunchKernelGGL(golbits::k_pack, dim3(pblocks), dim3(256), 0, ctx->stream, b0, P[0], nx, ny, NW);
     APPFIS_LAUNCH_CHECK();
-    const int nbands = (ny + golbits::KROWS - 1) / golbits::KROWS;
-    const unsigned nitems = (unsigned)NW * (unsigned)nbands;
-    const unsigned sblocks = ceil_div(nitems, 256);
-    for (int g = 0; g < gens; g++) {
-        hipLaunchKernelGGL(golbits::k_step<golbits::KROWS>, dim3(sblocks), dim3(256), 0, ctx->stream, P[g & 1],
-                           P[(g + 1) & 1], NW, ny, nitems);
-        APPFIS_LAUNCH_CHECK();
+    int cur = 0;
+    if (pick >= 0) {
+        st = run_tb(ctx, golbits::kTb[pick], P, &cur, NW, ny, gens - 1);
+        if (st) return st;
+        st = run_step(ctx, P, &cur, NW, ny, 1);
+    } else {
+        st = run_step(ctx, P, &cur, NW, ny, gens);
     }
-    // generation gens -> buf[gens % 2], generation gens-1 -> the other buffer, rings exchanged
+    if (st) return st;
+    // generation gens is in P[cur], generation gens-1 in P[cur ^ 1]: unpack them into
+    // buf[gens % 2] and the other buffer, rings exchanged
     uint8_t* buf[2] = {b0, b1};
     const int dims[2] = {nx + 2, ny + 2};
     for (int back = 0; back < 2; back++) {
         const int g = gens - back;
         uint8_t* dst = buf[g & 1];
         if (g > 0) {
-            hipLaunchKernelGGL(golbits::k_unpack, dim3(pblocks), dim3(256), 0, ctx->stream, P[g & 1], dst, nx, ny,
-                               NW);
+            hipLaunchKernelGGL(golbits::k_unpack, dim3(pblocks), dim3(256), 0, ctx->stream, P[cur ^ back], dst, nx,
+                               ny, NW);
             APPFIS_LAUNCH_CHECK();
         }
         st = appfis_exchange(ctx, dst, APPFIS_U8, 2, dims, 1, 1);

@@ -91,8 +91,14 @@ def bench_gol(ctx, hip, steps, warmup, variant, nx=8192, ny=8192):
         # uint8 field read + written once per generation
         res["roofline"] = roofline(2.0 * nx * ny, launch_ms, "gol_plain")
     else:
-        # bit-packed field (1 bit per cell) read + written once per generation
-        res["roofline"] = roofline(2.0 * nx * ny / 8, launch_ms, "gol_bits_step")
+        # bit-packed field (1 bit per cell) read + written once per launch; the blocked
+        # kernel advances G generations per launch (its autotuned name says "gG")
+        tname = _tuned(ctx, 4, hip.U8)
+        g = int(tname.split(" g", 1)[1].split()[0]) if tname else 1
+        res["tuned"] = tname
+        res["autotune"] = [l for l in ctx.tune_report().splitlines() if l.startswith("autotune kernel 4 ")]
+        res["roofline"] = roofline(2.0 * nx * ny / 8, launch_ms * g,
+                                   "gol_bits_step_tb" if tname else "gol_bits_step")
     d0.free()
     d1.free()
     return res

@@ -93,10 +93,13 @@ def test_gol_full_config_prefix(ctx):
 
 
 @pytest.mark.parametrize("nx,ny,gens", [(32, 2, 3), (32, 9, 1), (64, 64, 10), (96, 7, 13), (160, 33, 21),
-                                        (256, 100, 2), (2048, 17, 5), (94, 33, 7), (50, 40, 6)])
+                                        (256, 100, 2), (2048, 17, 5), (94, 33, 7), (50, 40, 6),
+                                        (2048, 3, 40), (2048, 130, 37), (4096, 50, 33), (16384, 20, 19),
+                                        (8192, 9, 2), (2048, 64, 1)])
 def test_gol_chosen_both_buffers(ctx, nx, ny, gens):
-    # bit-packed form (nx % 32 == 0; other widths take the byte kernel): buf[gens % 2]
-    # holds generation gens and the other buffer generation gens-1, rings exchanged
+    # bit-packed form (nx % 32 == 0; other widths take the byte kernel; nx % 2048 == 0
+    # also runs the temporally blocked kernel): buf[gens % 2] holds generation gens and
+    # the other buffer generation gens-1, rings exchanged
     a, d0, d1 = _gol_pair(ctx, nx, ny, seed=nx * 7 + ny)
     ctx.gol(d0, d1, nx, ny, gens, hip.CHOSEN)
     ref = oracle.gol_u8(a.copy(), np.zeros_like(a), nx, ny, gens)
