@@ -209,7 +209,7 @@ __global__ __launch_bounds__(BT) void k_flux_update(const Scen<R>* __restrict__ 
 {
     const int tile = blockIdx.x;
     const int s = find_scen(sc, nscen, tile);
-    const Scen<R> S = sc[s];
+    const Scen<R>& S = sc[s];  // a reference: a copy indexed by `par` would live in scratch
     const int t = tile - S.tile0;
     const int tyi = t / S.tiles_x, txi = t - tyi * S.tiles_x;
     const int i = txi * BX + (threadIdx.x & (BX - 1)) + 1;
@@ -300,7 +300,7 @@ __global__ __launch_bounds__(BT) void k_fix_cfl(const Scen<R>* __restrict__ sc, 
     __shared__ double ssum[BT / 64][2];
     const int tile = blockIdx.x;
     const int s = find_scen(sc, nscen, tile);
-    const Scen<R> S = sc[s];
+    const Scen<R>& S = sc[s];
     const int t = tile - S.tile0;
     const int tyi = t / S.tiles_x, txi = t - tyi * S.tiles_x;
     const int i = txi * BX + (threadIdx.x & (BX - 1)) + 1;
@@ -373,7 +373,7 @@ __global__ __launch_bounds__(BT) void k_finalize(const Scen<R>* __restrict__ sc,
 {
     __shared__ double smin[BT / 64];
     __shared__ double ssum[BT / 64][2];
-    const Scen<R> S = sc[blockIdx.x];
+    const Scen<R>& S = sc[blockIdx.x];
     double m = (double)RealLim<R>::max();
     double as = 0.0, ac = 0.0;
     for (int p = threadIdx.x; p < S.ntiles; p += BT) {

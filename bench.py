@@ -36,22 +36,29 @@ HBM_PEAK_GBS = 8000.0
 FDTD_COEFF = [0.5, 0.125, 0.0625, 0.125 / 3, 0.03125]
 
 
-def _traffic(kernel_key):
+def _traffic(kernels):
+    """HBM bytes per launch from profiles/traffic.json (rocprofv3 FETCH_SIZE + WRITE_SIZE,
+    calibrated on a copy kernel).  `kernels` maps kernel names (template form, as the
+    summary writes them) to weights: the weighted sum covers one roofline unit."""
     path = os.path.join(ROOT, "profiles", "traffic.json")
     if not os.path.exists(path):
         return None
     try:
-        v = json.load(open(path)).get(kernel_key)
-        return None if v is None else float(v["bytes_per_launch"])
-    except Exception:
+        tab = json.load(open(path))
+        return round(sum(w * float(tab[k]["bytes_per_launch"]) for k, w in kernels.items()), 1)
+    except (KeyError, ValueError, TypeError):
         return None
 
 
-def roofline(alg_bytes_per_launch, launch_ms, kernel_key):
+def roofline(alg_bytes_per_launch, launch_ms, kernels):
+    """`kernels`: the dominant kernel's name, or {name: launches per roofline unit}."""
+    if isinstance(kernels, str):
+        kernels = {kernels: 1.0}
     achieved = alg_bytes_per_launch / (launch_ms * 1e-3) / 1e9
     return {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": _traffic(kernel_key), "kernel": kernel_key,
-            "alg_bytes_per_launch": alg_bytes_per_launch, "avg_launch_us": round(launch_ms * 1e3, 3)}
+            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": _traffic(kernels),
+            "kernel": " + ".join(kernels), "alg_bytes_per_launch": alg_bytes_per_launch,
+            "avg_launch_us": round(launch_ms * 1e3, 3)}
 
 
 def timed(ctx, fn):
@@ -89,7 +96,7 @@ def bench_gol(ctx, hip, steps, warmup, variant, nx=8192, ny=8192):
            "glups": round(nx * ny * steps / (ms * 1e-3) / 1e9, 2), "live_count": live}
     if variant == hip.PLAIN:
         # uint8 field read + written once per generation
-        res["roofline"] = roofline(2.0 * nx * ny, launch_ms, "gol_plain")
+        res["roofline"] = roofline(2.0 * nx * ny, launch_ms, f"k_gol_plain<{(nx + 2) & 15}>")
     else:
         # bit-packed field (1 bit per cell) read + written once per launch; the blocked
         # kernel advances G generations per launch (its autotuned name says "gG")
@@ -97,8 +104,12 @@ def bench_gol(ctx, hip, steps, warmup, variant, nx=8192, ny=8192):
         g = int(tname.split(" g", 1)[1].split()[0]) if tname else 1
         res["tuned"] = tname
         res["autotune"] = [l for l in ctx.tune_report().splitlines() if l.startswith("autotune kernel 4 ")]
-        res["roofline"] = roofline(2.0 * nx * ny / 8, launch_ms * g,
-                                   "gol_bits_step_tb" if tname else "gol_bits_step")
+        if tname:
+            k, gg, w = (int(t[1:]) for t in tname.split()[1:4])
+            kname = f"k_step_tb<{k}, {gg}, {w}>"
+        else:
+            kname = "k_step<8>"
+        res["roofline"] = roofline(2.0 * nx * ny / 8, launch_ms * g, kname)
     d0.free()
     d1.free()
     return res
@@ -158,9 +169,13 @@ def bench_heat(ctx, hip, n=1024, iters=500):
             tname = _tuned(ctx, 1, hip.F64)  # e.g. "tb 64x32 ts8"
             launches = len(_plan(iters, int(tname.rsplit("ts", 1)[1])))
         per_launch_sweeps = iters / launches
+        if variant == hip.PLAIN:
+            kname = "k_heat2d_plain<double, 4>"
+        else:
+            tx, ty = tname.split()[1].split("x")
+            kname = f"k_heat2d_tb<double, {tx}, {ty}, {tname.rsplit('ts', 1)[1]}>"
         r = {"ms": round(ms, 4), "glups": round(n * n * iters / (ms * 1e-3) / 1e9, 2), "checksum": chk,
-             "roofline": roofline(16.0 * n * n * per_launch_sweeps, ms / launches,
-                                  "heat2d_" + name)}
+             "roofline": roofline(16.0 * n * n * per_launch_sweeps, ms / launches, kname)}
         if variant == hip.CHOSEN:
             r["tuned"] = tname
         out[name] = r
@@ -184,6 +199,11 @@ def _plan(iters, ts):
     return [0] * min(n, iters)
 
 
+FDTD_PLAIN = {"f32": "vx1 ty4 ry4 zc64 d2", "f64": "vx1 ty4 ry2 zc64 d2", "f16": "vx1 ty4 ry4 zc64 d2",
+              "bf16": "vx1 ty4 ry4 zc64 d2"}
+FDTD_CTYPE = {"f32": "float", "f64": "double", "f16": "__half", "bf16": "__hip_bfloat16"}
+
+
 def bench_fdtd(ctx, hip, n=512, steps=100):
     out = {"workload": f"FDTD3d r=4 {n}^3, {steps} steps (CHOSEN = autotuned tiling, same values as PLAIN)"}
     for name, dt, bpl in (("f32", hip.F32, 8), ("f64", hip.F64, 16), ("f16", hip.F16, 4), ("bf16", hip.BF16, 4)):
@@ -196,8 +216,12 @@ def bench_fdtd(ctx, hip, n=512, steps=100):
             ctx.fdtd3d(a, b, n, n, n, coeff, 2, variant)  # warm-up (and the one-time autotuning)
             ctx.fill_uniform(a, 4, 1004)
             ms, _ = timed(ctx, lambda: ctx.fdtd3d(a, b, n, n, n, coeff, steps, variant))
+            tname = FDTD_PLAIN[name] if variant == hip.PLAIN else _tuned(ctx, 3, dt)
+            vx, ty, ry, zc, dd = (int(t[2:]) if t[:2] in ("vx", "ty", "ry", "zc") else int(t[1:])
+                                  for t in tname.split())
+            kname = f"k_fdtd3d<{FDTD_CTYPE[name]}, {vx}, {ty}, {ry}, {zc}, {dd}>"
             out[f"{name}_{form}"] = {"ms": round(ms, 3), "glups": round(n ** 3 * steps / (ms * 1e-3) / 1e9, 2),
-                                     "roofline": roofline(float(bpl) * n ** 3, ms / steps, f"fdtd3d_{name}_{form}")}
+                                     "tiling": tname, "roofline": roofline(float(bpl) * n ** 3, ms / steps, kname)}
         a.free()
         b.free()
     out["autotune"] = [l for l in ctx.tune_report().splitlines() if "kernel 3 " in l]
@@ -232,8 +256,16 @@ def bench_himeno(ctx, hip, nx=257, ny=257, nz=513, iters=200):
 
         ms, _ = timed(ctx, run)
         r = {"ms": round(ms, 3), "glups": round(lups / (ms * 1e-3) / 1e9, 2), "gosa_last": float(gosas[name][-1])}
+        pts = (nx - 2) * (ny - 2) * (nz - 2)
         if bpl:
-            r["roofline"] = roofline(float(bpl) * (nx - 2) * (ny - 2) * (nz - 2), ms / iters, "himeno_" + name)
+            r["roofline"] = roofline(float(bpl) * pts, ms / iters, f"k_himeno<{'float' if dt == hip.F32 else 'double'}, 0>")
+        else:
+            # defect correction, per sweep on average over one outer cycle of K = 8: seven
+            # FP32 inner sweeps (11 coefficient copies, r, q read, q written: 56 B/LUP) and
+            # one FP64 rebase (11 coefficients, wrk1, B read, q read 4 B, B / r / q written
+            # 8 + 4 + 4 B: 124 B/LUP)
+            r["roofline"] = roofline((7 * 56.0 + 124.0) / 8 * pts, ms / iters,
+                                     {"k_himeno<float, 1>": 7 / 8, "k_himeno_rebase": 1 / 8})
         out[name] = r
         for a in arrs:
             a.free()
@@ -312,17 +344,112 @@ def bench_flood(ctx, hip, steps=2000, nscen=24):
     rel = max(float(np.linalg.norm(a - b) / np.linalg.norm(a)) for a, b in zip(h64, h32))
     vrel = float(np.max(np.abs(np.array(vol64) - res["v"][:, -1]) / np.array(vol64)))
     lups = cells * steps
+    # roofline over one whole step (flux/update, wet-dry/CFL/volume, finalise kernels):
+    # 64 B per cell FP64 (read h qx qy z n2, write h qx qy), 32 B FP32
     out["plain_f64"] = {"ms": round(ms64, 2), "glups": round(lups / (ms64 * 1e-3) / 1e9, 2),
-                        "roofline": roofline(64.0 * cells / nscen, ms64 / (2 * steps * nscen), "flood_f64_step")}
+                        "roofline": roofline(64.0 * cells / nscen, ms64 / (steps * nscen),
+                                             {"k_flux_update<double>": 1, "k_fix_cfl<double, 1>": 1,
+                                              "k_finalize<double, 1>": 1})}
     out["chosen_f32_ensemble"] = {"ms": round(ms32, 2), "glups": round(lups / (ms32 * 1e-3) / 1e9, 2),
                                   "speedup_vs_plain_f64": round(ms64 / ms32, 3),
                                   "h_rel_l2_vs_f64(first,last scenario)": rel,
                                   "volume_rel_err_vs_f64": vrel,
-                                  "roofline": roofline(32.0 * cells, ms32 / (2 * steps), "flood_f32_ensemble_step")}
+                                  "roofline": roofline(32.0 * cells, ms32 / steps,
+                                                       {"k_flux_update<float>": 1, "k_fix_cfl<float, 1>": 1,
+                                                        "k_finalize<float, 1>": 1})}
     for d in sc:
         for k in ("h", "qx", "qy", "z", "n2"):
             d[k].free()
     return out
+
+
+# ------------------------------------------------------------------------------------
+# profile summaries (rocprofv3 rocpd databases -> profiles/*.md, profiles/traffic.json)
+# ------------------------------------------------------------------------------------
+PROFILE_GROUPS = [
+    ("gol_8192x8192", "GameOfLife 8192^2 uint8 (byte kernel, bit-packed kernels, ring exchange)",
+     ("k_gol_plain", "k_pack", "k_unpack", "k_step", "k_wrap_axis<unsigned char>")),
+    ("heat2d_1024x1024_f64", "Heat2D 1024^2 FP64", ("k_heat2d",)),
+    ("fdtd3d_512cubed", "FDTD3d 512^3 r=4 (FP32/FP64/FP16/BF16, every autotuner candidate)", ("k_fdtd3d",)),
+    ("himeno_257x257x513", "Himeno 257x257x513 (FP32, FP64, defect correction)",
+     ("k_himeno", "k_add_correction", "k_f64_to_f32")),
+    ("flood_ensemble_24", "Triton flood: 24 scenarios N in [512,4096] (plain FP64 one at a time, FP32 ensemble)",
+     ("k_flux_update", "k_fix_cfl", "k_finalize", "k_init_state", "k_dem")),
+]
+
+
+def _short(name):
+    """Kernel name in template form: 'k_fdtd3d<float, 1, 8, 4, 64, 2>'."""
+    n = name.replace("(anonymous namespace)::", "")
+    if n.startswith("void "):
+        n = n[5:]
+    depth = 0
+    for i, ch in enumerate(n):
+        if ch == "<":
+            depth += 1
+        elif ch == ">":
+            depth -= 1
+        elif ch == "(" and depth == 0:
+            n = n[:i]
+            break
+    for ns in ("golbits::", "gol::"):
+        n = n.replace(ns, "")
+    return n.strip()
+
+
+def summarize(kt_db, fetch_db, write_db, cmds, out_dir):
+    import sqlite3
+
+    durs = {}
+    for name, dur in sqlite3.connect(kt_db).execute("select name, duration from kernels"):
+        durs.setdefault(_short(name), []).append(dur / 1000.0)
+
+    def pmc(db, counter):
+        acc = {}
+        if not db:
+            return acc
+        q = "select kernel_name, value from counters_collection where counter_name = ?"
+        for name, val in sqlite3.connect(db).execute(q, (counter,)):
+            acc.setdefault(_short(name), []).append(float(val))
+        return {k: (sum(v) / len(v), len(v)) for k, v in acc.items()}
+
+    fetch, write = pmc(fetch_db, "FETCH_SIZE"), pmc(write_db, "WRITE_SIZE")
+    traffic = {}
+    if "k_copy16" in fetch and "k_copy16" in write:
+        cf = COPY_BYTES / (fetch["k_copy16"][0] * 1024.0)
+        cw = COPY_BYTES / (write["k_copy16"][0] * 1024.0)
+        traffic["_calibration"] = {
+            "copy_bytes": COPY_BYTES, "copy_fetch_kib": fetch["k_copy16"][0], "copy_write_kib": write["k_copy16"][0],
+            "fetch_scale": round(cf, 4), "write_scale": round(cw, 4),
+            "note": "bytes = FETCH_SIZE KiB * 1024 * fetch_scale + WRITE_SIZE KiB * 1024 * write_scale; "
+                    "the counters see L2 misses, so MALL hits count as traffic"}
+        for k in sorted(set(fetch) & set(write)):
+            traffic[k] = {"fetch_kib": round(fetch[k][0], 1), "write_kib": round(write[k][0], 1),
+                          "dispatches": fetch[k][1],
+                          "bytes_per_launch": round(fetch[k][0] * 1024 * cf + write[k][0] * 1024 * cw, 1)}
+        with open(os.path.join(out_dir, "traffic.json"), "w") as f:
+            json.dump(traffic, f, indent=1, sort_keys=True)
+    for fname, title, prefixes in PROFILE_GROUPS:
+        names = sorted(k for k in durs if k.startswith(prefixes))
+        if not names:
+            continue
+        lines = [f"# {title}", "", "Made by (from the repository root, on one MI355X):", ""]
+        lines += [f"    {c}" for c in cmds]
+        lines += ["", "Per-dispatch kernel durations (kernel trace) and, where measured, HBM traffic per",
+                  "dispatch (FETCH_SIZE x fetch_scale + WRITE_SIZE x write_scale, KiB as read; scales "
+                  "from a 1 GiB copy kernel, see traffic.json `_calibration`).", "",
+                  "| kernel | calls | avg us | min us | max us | total ms | FETCH KiB | WRITE KiB | traffic MB |",
+                  "|---|---:|---:|---:|---:|---:|---:|---:|---:|"]
+        for k in names:
+            d = durs[k]
+            t = traffic.get(k)
+            tf = (f"{t['fetch_kib']:.0f} | {t['write_kib']:.0f} | {t['bytes_per_launch'] / 1e6:.2f}" if t
+                  else "- | - | -")
+            lines.append(f"| `{k}` | {len(d)} | {sum(d) / len(d):.2f} | {min(d):.2f} | {max(d):.2f} | "
+                         f"{sum(d) / 1e3:.3f} | {tf} |")
+        with open(os.path.join(out_dir, fname + ".md"), "w") as f:
+            f.write("\n".join(lines) + "\n")
+        print(f"wrote {fname}.md ({len(names)} kernels)")
 
 
 def main():
@@ -337,7 +464,15 @@ def main():
                     help="suite entries at their config sizes but few sweeps (for profiler runs)")
     ap.add_argument("--calibrate", action="store_true",
                     help="first run a 1 GiB device-to-device copy 4 times (HBM counter calibration)")
+    ap.add_argument("--summarize", metavar="KT_DB", help="write profiles/ summaries from a rocprofv3 "
+                    "--kernel-trace database (no GPU work)")
+    ap.add_argument("--fetch-db", help="rocprofv3 --pmc FETCH_SIZE database for --summarize")
+    ap.add_argument("--write-db", help="rocprofv3 --pmc WRITE_SIZE database for --summarize")
+    ap.add_argument("--prof-cmd", action="append", default=[], help="command text recorded in the summaries")
     args = ap.parse_args()
+    if args.summarize:
+        summarize(args.summarize, args.fetch_db, args.write_db, args.prof_cmd, os.path.join(ROOT, "profiles"))
+        return
     if args.gpus != 1:
         print(f"bench.py measures one GPU: this build has no multi-GPU path (asked for --gpus {args.gpus}); "
               "DESIGN.md describes how the stencils would shard.", file=sys.stderr)
