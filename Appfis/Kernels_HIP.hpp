@@ -54,7 +54,7 @@ namespace APPFIS
         }
 
         // GameOfLife B3/S23 on a periodic uint8 grid (ghost 1)
-        inline void GameOfLife(Grid<uint8_t>& g, int gens, int variant = APPFIS_PLAIN, int tsteps = 0)
+        inline void GameOfLife(Grid<uint8_t>& g, int gens, int variant = APPFIS_CHOSEN, int tsteps = 0)
         {
             detail_k::RequireDual(g, "GameOfLife");
             Array<uint8_t>* h = g.GetSubGridCPU();
@@ -67,7 +67,7 @@ namespace APPFIS
 
         // FDTD3d radius-4 star (ghost 4): u' = c0 u + sum_r c_r (6 neighbours at r)
         template <typename T>
-        void FDTD3d(Grid<T>& g, const double coeff5[5], int steps, int variant = APPFIS_PLAIN)
+        void FDTD3d(Grid<T>& g, const double coeff5[5], int steps, int variant = APPFIS_CHOSEN)
         {
             detail_k::RequireDual(g, "FDTD3d");
             Array<T>* h = g.GetSubGridCPU();

@@ -163,3 +163,35 @@ def test_f32_chain_tracks_f64():
         out.append(h.astype(np.float64))
     rel = np.linalg.norm(out[0] - out[1]) / np.linalg.norm(out[0])
     assert rel < 1e-4
+
+
+def test_dem_plane_and_wall_closed_form():
+    # no hills: the plane z = slope * x at cell centres x = (i - 1/2) dx, wall in the ring
+    nx, ny, dx = 40, 30, 10.0
+    z = oracle.flood_dem(nx, ny, dx, 5, nhills=0, slope=0.002, wall=123.0)
+    x = (np.arange(1, nx + 1) - 0.5) * dx
+    assert np.array_equal(z[1:-1, 1:-1], np.broadcast_to(0.002 * x, (ny, nx)))
+    ring = np.ones_like(z, dtype=bool)
+    ring[1:-1, 1:-1] = False
+    assert np.all(z[ring] == 123.0)
+
+
+def test_dem_single_hill_shape():
+    # one hill, no slope: a Gaussian with amplitude in [5, 50] m and sigma in [200, 2000] m;
+    # a Gaussian's integral over its peak is 2 pi sigma^2 (hill well inside a wide domain)
+    nx = ny = 2500
+    dx = 10.0
+    for seed in range(40):
+        z = oracle.flood_dem(nx, ny, dx, seed, nhills=1, slope=0.0)[1:-1, 1:-1]
+        peak = z.max()
+        jy, ix = np.unravel_index(np.argmax(z), z.shape)
+        margin = min(ix, jy, nx - 1 - ix, ny - 1 - jy) * dx
+        if margin < 8000:  # only hills whose tails stay in the domain
+            continue
+        assert 5.0 <= peak <= 50.0 + 1e-9
+        sigma = np.sqrt(z.sum() * dx * dx / (2 * np.pi * peak))
+        assert 200.0 * 0.99 <= sigma <= 2000.0 * 1.01
+        # radial symmetry about the peak
+        assert abs(z[jy, ix + 30] - z[jy, ix - 30]) < 1e-2 * peak and abs(z[jy + 30, ix] - z[jy, ix + 30]) < 1e-2 * peak
+        return
+    pytest.skip("no seed put a hill far enough inside")
