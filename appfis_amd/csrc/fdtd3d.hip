@@ -94,14 +94,13 @@ __device__ inline void tile_of(int& bx, int& by, int& bz)
     bz = r / gy;
 }
 
-template <typename S, int VX, int TY, int RY, int ZC, int D>
+template <typename S, int VX, int TY, int RY, int ZC, int D, int U>
 __global__ __launch_bounds__(TXT* TY) void k_fdtd3d(const S* __restrict__ u, S* __restrict__ v, int nx, int ny, int nz,
                                                      typename Cvt<S>::C c0, typename Cvt<S>::C c1,
                                                      typename Cvt<S>::C c2, typename Cvt<S>::C c3,
                                                      typename Cvt<S>::C c4)
 {
     using C = typename Cvt<S>::C;
-    static_assert((D & 1) == 0, "prefetch depth must be even (LDS buffer parity per slot)");
     constexpr int NT = TXT * TY;
     constexpr int PX = TXT * VX;       // tile width
     constexpr int PY = TY * RY;        // tile height
@@ -109,7 +108,10 @@ __global__ __launch_bounds__(TXT* TY) void k_fdtd3d(const S* __restrict__ u, S* 
     constexpr int LY = PY + 2 * R;     // LDS plane height
     constexpr int NHALO = 2 * R * PX + 2 * R * PY;
     constexpr int HPT = (NHALO + NT - 1) / NT;  // halo cells per thread
-    __shared__ C plane[2][LY][LX];
+    // two plane buffers, each followed by 64 dummy cells: a thread without a halo cell
+    // in a slot writes the dummy of its lane (distinct lanes, so no write conflict)
+    constexpr int PLANE = LY * LX + 64;
+    __shared__ C plane[2][PLANE];
 
     int tbx, tby, tbz;
     tile_of(tbx, tby, tbz);
@@ -148,71 +150,110 @@ __global__ __launch_bounds__(TXT* TY) void k_fdtd3d(const S* __restrict__ u, S* 
         }
         const int xg = bx - R + lx, yg = by - R + ly;
         hval[s] = huse[s] && xg < W && yg < H;
-        hoff[s] = hval[s] ? xg + W * yg : 0;
-        hlds[s] = ly * LX + lx;
+        hoff[s] = hval[s] ? xg + W * yg : 0;  // an in-bounds address either way
+        hlds[s] = huse[s] ? ly * LX + lx : LY * LX + tx;
     }
-
-    // register queues q[r][e][m] = u at z + m - 4
-    C q[RY][VX][2 * R + 1];
+    // own rows: in-bounds (clamped) offsets and a validity flag, so that the loads need
+    // no branch (an invalid row loads a valid cell and is replaced by 0)
+    int qoff[RY];
+    bool qval[RY];
 #pragma unroll
     for (int r = 0; r < RY; r++) {
-        const bool in = xload && gy0 + r < H;
+        qval[r] = xload && gy0 + r < H;
+        qoff[r] = qval[r] ? own0 + r * W : 0;
+    }
+
+    // register queues.  U (the unroll of the march) a multiple of 9: plane p sits in slot
+    // (p - zb + 4) mod 9 while the march is in planes zb .. zb+U-1, every slot index is a
+    // compile-time constant and the queue rotates without register moves.  Otherwise
+    // slot m holds plane z + m - 4 and the queue shifts by one slot per plane (moves, but
+    // fewer live registers).
+    constexpr int QN = 2 * R + 1;
+    constexpr int UNROLL = U;
+    constexpr bool BRANCHFREE = sizeof(S) == 2;
+    constexpr bool ROTATE = U % QN == 0;
+    static_assert(U % D == 0 && U % 2 == 0, "the unroll must be a multiple of the prefetch depth and of 2");
+    C q[RY][VX][QN];
+#pragma unroll
+    for (int r = 0; r < RY; r++) {
 #pragma unroll
         for (int m = 0; m < 2 * R; m++) {
             C t[VX];
-            if (in) Vec<S, VX>::load(u + (z0 - R + m) * szl, own0 + r * W, t);
-            else
+            Vec<S, VX>::load(u + (z0 - R + m) * szl, qoff[r], t);
 #pragma unroll
-                for (int e = 0; e < VX; e++) t[e] = (C)0;
-#pragma unroll
-            for (int e = 0; e < VX; e++) q[r][e][m + 1] = t[e];
+            for (int e = 0; e < VX; e++) q[r][e][ROTATE ? m : m + 1] = qval[r] ? t[e] : (C)0;  // shift: before its first shift
         }
     }
-    // prefetch ring: slot d holds the queue entry (plane z+R) and halo (plane z) of step
-    // z0 + d (+ D, 2D, ... as the march proceeds)
+    // prefetch ring: slot d holds the queue entry (plane z+R) and halo (plane z) of the
+    // steps z = z0 + d (mod D)
     C qp[D][RY][VX];
     C hp[D][HPT];
     auto prefetch = [&](int z, C(&qd)[RY][VX], C(&hd)[HPT]) {
         const S* pq = u + (long long)(z + R) * szl;
         const S* ph = u + (long long)z * szl;
+        // 16-bit storage: unconditional loads of in-bounds addresses and selects measure
+        // faster; 32/64-bit: predicated loads (exec-masked branches) measure faster
 #pragma unroll
         for (int r = 0; r < RY; r++) {
-            if (xload && gy0 + r < H) Vec<S, VX>::load(pq, own0 + r * W, qd[r]);
-            else
+            if constexpr (BRANCHFREE) {
+                C t[VX];
+                Vec<S, VX>::load(pq, qoff[r], t);
 #pragma unroll
-                for (int e = 0; e < VX; e++) qd[r][e] = (C)0;
+                for (int e = 0; e < VX; e++) qd[r][e] = qval[r] ? t[e] : (C)0;
+            } else {
+                if (qval[r]) Vec<S, VX>::load(pq, qoff[r], qd[r]);
+                else
+#pragma unroll
+                    for (int e = 0; e < VX; e++) qd[r][e] = (C)0;
+            }
         }
 #pragma unroll
-        for (int s = 0; s < HPT; s++) hd[s] = hval[s] ? Cvt<S>::ld(ph[hoff[s]]) : (C)0;
+        for (int s = 0; s < HPT; s++) {
+            if constexpr (BRANCHFREE) {
+                const C t = Cvt<S>::ld(ph[hoff[s]]);
+                hd[s] = hval[s] ? t : (C)0;
+            } else {
+                hd[s] = hval[s] ? Cvt<S>::ld(ph[hoff[s]]) : (C)0;
+            }
+        }
     };
 #pragma unroll
     for (int d = 0; d < D; d++)
         if (z0 + d < z1) prefetch(z0 + d, qp[d], hp[d]);
 
-    C* pl = &plane[0][0][0];
-    for (int zb = z0; zb < z1; zb += D) {
+    C* pl = &plane[0][0];
+    for (int zb = z0; zb < z1; zb += UNROLL) {
 #pragma unroll
-        for (int d = 0; d < D; d++) {
-            const int z = zb + d;
+        for (int st = 0; st < UNROLL; st++) {
+            const int z = zb + st;
             if (z >= z1) break;
-            C* buf = pl + (d & 1) * (LY * LX);
-            // advance the queues to plane z
+            const int d = st % D;
+            C* buf = pl + (st & 1) * PLANE;
+            // slots of planes z-4 .. z+4 at this step, m = 0 .. 8
+#define QS(m) (ROTATE ? (st + (m)) % QN : (m))
+            if constexpr (!ROTATE) {
+#pragma unroll
+                for (int r = 0; r < RY; r++)
+#pragma unroll
+                    for (int e = 0; e < VX; e++)
+#pragma unroll
+                        for (int m = 0; m < 2 * R; m++) q[r][e][m] = q[r][e][m + 1];
+            }
+            // incoming plane z + R
 #pragma unroll
             for (int r = 0; r < RY; r++)
 #pragma unroll
-                for (int e = 0; e < VX; e++) {
-#pragma unroll
-                    for (int m = 0; m < 2 * R; m++) q[r][e][m] = q[r][e][m + 1];
-                    q[r][e][2 * R] = qp[d][r][e];
-                }
+                for (int e = 0; e < VX; e++) q[r][e][QS(2 * R)] = qp[d][r][e];
             // stage plane z: own centres + prefetched halo
 #pragma unroll
             for (int r = 0; r < RY; r++)
 #pragma unroll
-                for (int e = 0; e < VX; e++) buf[(ty * RY + r + R) * LX + tx * VX + e + R] = q[r][e][R];
+                for (int e = 0; e < VX; e++) buf[(ty * RY + r + R) * LX + tx * VX + e + R] = q[r][e][QS(R)];
 #pragma unroll
-            for (int s = 0; s < HPT; s++)
-                if (huse[s]) buf[hlds[s]] = hp[d][s];
+            for (int s = 0; s < HPT; s++) {
+                if constexpr (BRANCHFREE) buf[hlds[s]] = hp[d][s];  // unused slots hit a dummy cell
+                else if (huse[s]) buf[hlds[s]] = hp[d][s];
+            }
             // refill this slot with step z + D
             if (z + D < z1) prefetch(z + D, qp[d], hp[d]);
             __syncthreads();
@@ -224,14 +265,14 @@ __global__ __launch_bounds__(TXT* TY) void k_fdtd3d(const S* __restrict__ u, S* 
 #pragma unroll
                 for (int e = 0; e < VX; e++) {
                     const int lx = tx * VX + e + R;
-                    C acc = cc[0] * q[r][e][R];
+                    C acc = cc[0] * q[r][e][QS(R)];
 #pragma unroll
                     for (int k = 1; k <= R; k++) {
                         const C sx = buf[ly * LX + lx - k] + buf[ly * LX + lx + k];
-                        const C ym = (r - k >= 0) ? q[r - k][e][R] : buf[(ly - k) * LX + lx];
-                        const C yp = (r + k < RY) ? q[r + k][e][R] : buf[(ly + k) * LX + lx];
+                        const C ym = (r - k >= 0) ? q[r - k][e][QS(R)] : buf[(ly - k) * LX + lx];
+                        const C yp = (r + k < RY) ? q[r + k][e][QS(R)] : buf[(ly + k) * LX + lx];
                         const C sY = ym + yp;
-                        const C sZ = q[r][e][R - k] + q[r][e][R + k];
+                        const C sZ = q[r][e][QS(R - k)] + q[r][e][QS(R + k)];
                         const C sr = (sx + sY) + sZ;
                         acc = acc + cc[k] * sr;
                     }
@@ -239,18 +280,19 @@ __global__ __launch_bounds__(TXT* TY) void k_fdtd3d(const S* __restrict__ u, S* 
                 }
                 if (xstore && gy0 + r < ny + R) Vec<S, VX>::store(vz, own0 + r * W, out);
             }
+#undef QS
         }
     }
 }
 
-template <typename S, int VX, int TY, int RY, int ZC, int D>
+template <typename S, int VX, int TY, int RY, int ZC, int D, int U>
 int run(appfis_ctx* ctx, S* b0, S* b1, int nx, int ny, int nz, const double* cf, int steps)
 {
     using C = typename Cvt<S>::C;
     dim3 block(TXT, TY), grid(ceil_div(nx, TXT * VX), ceil_div(ny, TY * RY), ceil_div(nz, ZC));
     S* buf[2] = {b0, b1};
     for (int s = 0; s < steps; s++) {
-        hipLaunchKernelGGL((k_fdtd3d<S, VX, TY, RY, ZC, D>), grid, block, 0, ctx->stream, buf[s & 1],
+        hipLaunchKernelGGL((k_fdtd3d<S, VX, TY, RY, ZC, D, U>), grid, block, 0, ctx->stream, buf[s & 1],
                            buf[(s + 1) & 1], nx, ny, nz, (C)cf[0], (C)cf[1], (C)cf[2], (C)cf[3], (C)cf[4]);
         APPFIS_LAUNCH_CHECK();
     }
@@ -264,43 +306,71 @@ struct FdtdVariant {
     bool pairs;  // needs the 32-bit pair path (even nx, 4-byte aligned buffers)
 };
 
-template <typename S, int VX, int TY, int RY, int ZC, int D>
+template <typename S, int VX, int TY, int RY, int ZC, int D, int U>
 int run_v(appfis_ctx* ctx, void* b0, void* b1, int nx, int ny, int nz, const double* cf, int steps)
 {
-    return run<S, VX, TY, RY, ZC, D>(ctx, (S*)b0, (S*)b1, nx, ny, nz, cf, steps);
+    return run<S, VX, TY, RY, ZC, D, U>(ctx, (S*)b0, (S*)b1, nx, ny, nz, cf, steps);
 }
 
 // Candidate tilings (VX x-points per thread, TY thread rows, RY rows per thread, ZC
-// planes per block, D-deep prefetch).  Index 0 is the PLAIN default.
-#define FDTD_VARIANTS(S)                                                          \
-    {"vx1 ty4 ry4 zc64 d2", run_v<S, 1, 4, 4, 64, 2>, false},                      \
-    {"vx1 ty4 ry8 zc64 d2", run_v<S, 1, 4, 8, 64, 2>, false},                      \
-    {"vx1 ty4 ry4 zc32 d2", run_v<S, 1, 4, 4, 32, 2>, false},                      \
-    {"vx1 ty4 ry4 zc128 d2", run_v<S, 1, 4, 4, 128, 2>, false},                    \
-    {"vx1 ty8 ry4 zc64 d2", run_v<S, 1, 8, 4, 64, 2>, false},                      \
-    {"vx1 ty4 ry2 zc64 d2", run_v<S, 1, 4, 2, 64, 2>, false},                      \
-    {"vx1 ty4 ry4 zc64 d4", run_v<S, 1, 4, 4, 64, 4>, false}
-
-static const FdtdVariant kF32[] = {FDTD_VARIANTS(float)};
+// planes per block, D-deep prefetch, U-step unroll: 18 rotates the queue).  Index 0 of
+// each table is the PLAIN default: the fastest measured tiling that fits any shape.
+static const FdtdVariant kF32[] = {
+    {"vx1 ty8 ry4 zc256 d2 u2", run_v<float, 1, 8, 4, 256, 2, 2>, false},
+    {"vx1 ty4 ry4 zc64 d2 u2", run_v<float, 1, 4, 4, 64, 2, 2>, false},
+    {"vx1 ty8 ry4 zc64 d2 u2", run_v<float, 1, 8, 4, 64, 2, 2>, false},
+    {"vx1 ty4 ry4 zc64 d2 u18", run_v<float, 1, 4, 4, 64, 2, 18>, false},
+    {"vx1 ty4 ry2 zc64 d2 u18", run_v<float, 1, 4, 2, 64, 2, 18>, false},
+    {"vx1 ty8 ry2 zc64 d2 u18", run_v<float, 1, 8, 2, 64, 2, 18>, false},
+    {"vx1 ty4 ry4 zc32 d2 u2", run_v<float, 1, 4, 4, 32, 2, 2>, false},
+    {"vx1 ty8 ry4 zc128 d2 u2", run_v<float, 1, 8, 4, 128, 2, 2>, false},
+    {"vx1 ty4 ry4 zc64 d3 u6", run_v<float, 1, 4, 4, 64, 3, 6>, false},
+    {"vx1 ty16 ry2 zc64 d2 u2", run_v<float, 1, 16, 2, 64, 2, 2>, false},
+    {"vx1 ty8 ry4 zc128 d3 u6", run_v<float, 1, 8, 4, 128, 3, 6>, false},
+    {"vx1 ty16 ry2 zc256 d2 u2", run_v<float, 1, 16, 2, 256, 2, 2>, false},
+};
 static const FdtdVariant kF64[] = {
-    {"vx1 ty4 ry2 zc64 d2", run_v<double, 1, 4, 2, 64, 2>, false},
-    {"vx1 ty4 ry4 zc64 d2", run_v<double, 1, 4, 4, 64, 2>, false},
-    {"vx1 ty4 ry2 zc32 d2", run_v<double, 1, 4, 2, 32, 2>, false},
-    {"vx1 ty4 ry2 zc128 d2", run_v<double, 1, 4, 2, 128, 2>, false},
-    {"vx1 ty8 ry2 zc64 d2", run_v<double, 1, 8, 2, 64, 2>, false},
+    {"vx1 ty8 ry2 zc64 d2 u2", run_v<double, 1, 8, 2, 64, 2, 2>, false},
+    {"vx1 ty4 ry2 zc64 d2 u2", run_v<double, 1, 4, 2, 64, 2, 2>, false},
+    {"vx1 ty4 ry4 zc64 d2 u2", run_v<double, 1, 4, 4, 64, 2, 2>, false},
+    {"vx1 ty4 ry2 zc128 d2 u2", run_v<double, 1, 4, 2, 128, 2, 2>, false},
+    {"vx1 ty4 ry2 zc64 d2 u18", run_v<double, 1, 4, 2, 64, 2, 18>, false},
+    {"vx1 ty8 ry2 zc64 d2 u18", run_v<double, 1, 8, 2, 64, 2, 18>, false},
+    {"vx1 ty8 ry2 zc256 d2 u2", run_v<double, 1, 8, 2, 256, 2, 2>, false},
+    {"vx1 ty4 ry4 zc256 d2 u2", run_v<double, 1, 4, 4, 256, 2, 2>, false},
+    {"vx1 ty8 ry2 zc128 d2 u18", run_v<double, 1, 8, 2, 128, 2, 18>, false},
 };
-static const FdtdVariant kF16[] = {
-    FDTD_VARIANTS(__half),
-    {"vx2 ty4 ry4 zc64 d2", run_v<__half, 2, 4, 4, 64, 2>, true},
-    {"vx2 ty4 ry2 zc64 d2", run_v<__half, 2, 4, 2, 64, 2>, true},
-};
-static const FdtdVariant kBF16[] = {
-    FDTD_VARIANTS(__hip_bfloat16),
-    {"vx2 ty4 ry4 zc64 d2", run_v<__hip_bfloat16, 2, 4, 4, 64, 2>, true},
-    {"vx2 ty4 ry2 zc64 d2", run_v<__hip_bfloat16, 2, 4, 2, 64, 2>, true},
-};
+#define FDTD16_VARIANTS(S)                                                 \
+    {"vx1 ty4 ry2 zc64 d2 u18", run_v<S, 1, 4, 2, 64, 2, 18>, false},      \
+    {"vx1 ty4 ry4 zc64 d2 u18", run_v<S, 1, 4, 4, 64, 2, 18>, false},      \
+    {"vx1 ty4 ry4 zc64 d2 u2", run_v<S, 1, 4, 4, 64, 2, 2>, false},        \
+    {"vx1 ty8 ry4 zc64 d2 u2", run_v<S, 1, 8, 4, 64, 2, 2>, false},        \
+    {"vx2 ty4 ry4 zc64 d2 u18", run_v<S, 2, 4, 4, 64, 2, 18>, true},       \
+    {"vx2 ty4 ry2 zc64 d2 u18", run_v<S, 2, 4, 2, 64, 2, 18>, true},       \
+    {"vx2 ty8 ry2 zc64 d2 u18", run_v<S, 2, 8, 2, 64, 2, 18>, true},       \
+    {"vx2 ty4 ry4 zc64 d2 u2", run_v<S, 2, 4, 4, 64, 2, 2>, true},        \
+    {"vx2 ty8 ry2 zc128 d2 u18", run_v<S, 2, 8, 2, 128, 2, 18>, true}
+static const FdtdVariant kF16[] = {FDTD16_VARIANTS(__half)};
+static const FdtdVariant kBF16[] = {FDTD16_VARIANTS(__hip_bfloat16)};
 
 }  // namespace
+
+extern "C" int appfis_fdtd3d_candidates(int dtype, char* names, int len)
+{
+    const FdtdVariant* tab;
+    int ntab;
+    switch (dtype) {
+    case APPFIS_F32: tab = kF32; ntab = (int)(sizeof(kF32) / sizeof(kF32[0])); break;
+    case APPFIS_F64: tab = kF64; ntab = (int)(sizeof(kF64) / sizeof(kF64[0])); break;
+    case APPFIS_F16: tab = kF16; ntab = (int)(sizeof(kF16) / sizeof(kF16[0])); break;
+    case APPFIS_BF16: tab = kBF16; ntab = (int)(sizeof(kBF16) / sizeof(kBF16[0])); break;
+    default: return -invalid("fdtd3d: dtype must be F32, F64, F16 or BF16");
+    }
+    std::string all;
+    for (int i = 0; i < ntab; i++) all += std::string(tab[i].name) + "\n";
+    if (names && len > 0) snprintf(names, (size_t)len, "%s", all.c_str());
+    return ntab;
+}
 
 extern "C" int appfis_fdtd3d(appfis_ctx* ctx, int dtype, void* buf0, void* buf1, int nx, int ny, int nz,
                              const double* coeff5, int steps, int variant)
@@ -310,7 +380,7 @@ extern "C" int appfis_fdtd3d(appfis_ctx* ctx, int dtype, void* buf0, void* buf1,
     APPFIS_REQUIRE(nx > 0 && ny > 0 && nz > 0, "interior must be non-empty");
     APPFIS_REQUIRE((double)(nx + 8) * (ny + 8) * (nz + 8) < 2147483647.0, "grid exceeds the int index range");
     APPFIS_REQUIRE(steps >= 0, "steps must be >= 0");
-    APPFIS_REQUIRE(variant == APPFIS_PLAIN || variant == APPFIS_CHOSEN, "fdtd3d: unknown variant %d", variant);
+    APPFIS_REQUIRE(variant >= 0, "fdtd3d: unknown variant %d", variant);
     const FdtdVariant* tab;
     int ntab;
     switch (dtype) {
@@ -320,11 +390,17 @@ extern "C" int appfis_fdtd3d(appfis_ctx* ctx, int dtype, void* buf0, void* buf1,
     case APPFIS_BF16: tab = kBF16; ntab = (int)(sizeof(kBF16) / sizeof(kBF16[0])); break;
     default: return invalid("fdtd3d: dtype must be F32, F64, F16 or BF16");
     }
+    const bool pairs = ((nx & 1) == 0) && (((uintptr_t)buf0 | (uintptr_t)buf1) & 3) == 0;
+    if (variant >= APPFIS_CANDIDATE0) {
+        const int i = variant - APPFIS_CANDIDATE0;
+        APPFIS_REQUIRE(i < ntab, "fdtd3d: candidate %d out of range (%d candidates)", i, ntab);
+        APPFIS_REQUIRE(!tab[i].pairs || pairs, "fdtd3d: candidate %d needs even nx and 4-byte aligned buffers", i);
+        return steps == 0 ? APPFIS_OK : tab[i].fn(ctx, buf0, buf1, nx, ny, nz, coeff5, steps);
+    }
     if (steps == 0) return APPFIS_OK;
     if (variant == APPFIS_PLAIN) return tab[0].fn(ctx, buf0, buf1, nx, ny, nz, coeff5, steps);
     // CHOSEN: the fastest tiling for this shape, timed once on scratch buffers.  Every
     // candidate evaluates the same expression in the same order: results are identical.
-    const bool pairs = ((nx & 1) == 0) && (((uintptr_t)buf0 | (uintptr_t)buf1) & 3) == 0;
     std::vector<int> cand;
     std::vector<const char*> names;
     for (int i = 0; i < ntab; i++)

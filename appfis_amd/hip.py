@@ -19,6 +19,7 @@ LIB_PATH = os.path.join(_HERE, "libappfis_hip.so")
 U8, I32, F32, F64, F16, BF16, I64 = 0, 1, 2, 3, 4, 5, 6
 MAX, MIN, SUM = 0, 1, 2
 PLAIN, CHOSEN = 0, 1
+CANDIDATE0 = 2  # CANDIDATE0 + i: candidate i of an autotuned kernel's table
 # faces (APPFIS::DIRECTION)
 LEFT, RIGHT, UP, DOWN, FRONT, BACK = 0, 1, 2, 3, 4, 5
 
@@ -116,6 +117,7 @@ def lib():
         "appfis_heat2d": ([vp, i, vp, vp, i, i, d, i, i, i], i),
         "appfis_gol": ([vp, vp, vp, i, i, i, i, i], i),
         "appfis_fdtd3d": ([vp, i, vp, vp, i, i, i, dp, i, i], i),
+        "appfis_fdtd3d_candidates": ([i, C.c_char_p, i], i),
         "appfis_himeno": ([vp, i, C.POINTER(vp), i, i, i, d, i, i, i, dp], i),
         "appfis_flood_run": ([vp, i, vp, vp, vp, vp, vp, i, i, C.POINTER(FloodParams), i, i, dp, dp], i),
         "appfis_flood_ensemble_run": ([vp, i, i, ip, ip, C.POINTER(vp), C.POINTER(vp), C.POINTER(vp),
@@ -128,6 +130,15 @@ def lib():
         fn.restype = res
     _lib = L
     return L
+
+
+def fdtd3d_candidates(dtype: int):
+    """Names of the FDTD3d candidate tilings (variant CANDIDATE0 + index)."""
+    buf = C.create_string_buffer(4096)
+    n = lib().appfis_fdtd3d_candidates(dtype, buf, 4096)
+    if n < 0:
+        raise AppfisError(lib().appfis_last_error().decode(errors="replace"))
+    return buf.value.decode().splitlines()[:n]
 
 
 def check(status: int, what: str = "") -> None:

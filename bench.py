@@ -199,8 +199,8 @@ def _plan(iters, ts):
     return [0] * min(n, iters)
 
 
-FDTD_PLAIN = {"f32": "vx1 ty4 ry4 zc64 d2", "f64": "vx1 ty4 ry2 zc64 d2", "f16": "vx1 ty4 ry4 zc64 d2",
-              "bf16": "vx1 ty4 ry4 zc64 d2"}
+FDTD_PLAIN = {"f32": "vx1 ty8 ry4 zc256 d2 u2", "f64": "vx1 ty8 ry2 zc64 d2 u2", "f16": "vx1 ty4 ry2 zc64 d2 u18",
+              "bf16": "vx1 ty4 ry2 zc64 d2 u18"}
 FDTD_CTYPE = {"f32": "float", "f64": "double", "f16": "__half", "bf16": "__hip_bfloat16"}
 
 
@@ -217,9 +217,9 @@ def bench_fdtd(ctx, hip, n=512, steps=100):
             ctx.fill_uniform(a, 4, 1004)
             ms, _ = timed(ctx, lambda: ctx.fdtd3d(a, b, n, n, n, coeff, steps, variant))
             tname = FDTD_PLAIN[name] if variant == hip.PLAIN else _tuned(ctx, 3, dt)
-            vx, ty, ry, zc, dd = (int(t[2:]) if t[:2] in ("vx", "ty", "ry", "zc") else int(t[1:])
-                                  for t in tname.split())
-            kname = f"k_fdtd3d<{FDTD_CTYPE[name]}, {vx}, {ty}, {ry}, {zc}, {dd}>"
+            vx, ty, ry, zc, dd, uu = (int(t[2:]) if t[:2] in ("vx", "ty", "ry", "zc") else int(t[1:])
+                                      for t in tname.split())
+            kname = f"k_fdtd3d<{FDTD_CTYPE[name]}, {vx}, {ty}, {ry}, {zc}, {dd}, {uu}>"
             out[f"{name}_{form}"] = {"ms": round(ms, 3), "glups": round(n ** 3 * steps / (ms * 1e-3) / 1e9, 2),
                                      "tiling": tname, "roofline": roofline(float(bpl) * n ** 3, ms / steps, kname)}
         a.free()

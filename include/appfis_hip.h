@@ -55,6 +55,10 @@ extern "C" {
 /* kernel variants of the hand-written config kernels */
 #define APPFIS_PLAIN 0   /* one sweep per launch, reference dtype and operation order */
 #define APPFIS_CHOSEN 1  /* temporal blocking / packed / mixed-precision form */
+/* APPFIS_CANDIDATE0 + i: candidate i of an autotuned kernel's table, run as is (for
+ * tests and tuning studies; appfis_*_candidates lists the table).  Every candidate
+ * computes the same values as PLAIN. */
+#define APPFIS_CANDIDATE0 2
 
 typedef struct appfis_ctx appfis_ctx;
 
@@ -165,9 +169,14 @@ int appfis_gol(appfis_ctx* ctx, uint8_t* buf0, uint8_t* buf1, int nx, int ny, in
  * dtype F32 / F64 (arithmetic in that type) or F16 / BF16 (storage only, FP32
  * arithmetic).  Field in bufs[steps % 2] at exit.  PLAIN runs the default tiling;
  * CHOSEN runs the tiling the autotuner picked for this dtype and shape (timed once on
- * scratch buffers, cached in the context) -- both compute identical values. */
+ * scratch buffers, cached in the context); APPFIS_CANDIDATE0 + i runs candidate i of
+ * appfis_fdtd3d_candidates -- all compute identical values. */
 int appfis_fdtd3d(appfis_ctx* ctx, int dtype, void* buf0, void* buf1, int nx, int ny, int nz,
                   const double* coeff5, int steps, int variant);
+
+/* Candidate tilings of appfis_fdtd3d for `dtype`, one name per line, into names[len]
+ * (may be NULL); returns their number, < 0 for a bad dtype.  Needs no context. */
+int appfis_fdtd3d_candidates(int dtype, char* names, int len);
 
 /* ---- Himeno 19-point Jacobi (Himeno/src/himeno_appfis.hpp:178-198) with the
  * residual gosa = sum ss^2 over the interior after every sweep.

@@ -160,6 +160,21 @@ def test_fdtd3d_bit_exact(ctx, dt, n, steps, variant):
     assert np.array_equal(got.view(np.uint8), ref.view(np.uint8))
 
 
+@pytest.mark.parametrize("dt", [hip.F32, hip.F64, hip.F16, hip.BF16])
+def test_fdtd3d_every_candidate_bit_exact(ctx, dt):
+    # every tiling the autotuner may pick, forced one at a time
+    n = (70, 33, 40)
+    coeff = FDTD_COEFF if dt != hip.F16 else [c / 2.0625 for c in FDTD_COEFF]
+    a, d0, d1 = _fdtd_pair(ctx, n, dt)
+    ref = oracle.fdtd3d(a.copy(), np.zeros_like(a), *n, coeff, 3, bf16=(dt == hip.BF16))
+    names = hip.fdtd3d_candidates(dt)
+    assert len(names) >= 2
+    for i, name in enumerate(names):
+        d0.upload(a)
+        ctx.fdtd3d(d0, d1, *n, coeff, 3, hip.CANDIDATE0 + i)
+        assert np.array_equal(d1.download().view(np.uint8), ref.view(np.uint8)), name
+
+
 def test_fdtd3d_full_config(ctx):
     # 512^3 FP32: 2 sweeps bit-exact against the oracle, then the full 100 sweeps checked
     # against the FP64 kernel on normalised fields (u / 2.0625^n)
