@@ -173,7 +173,7 @@ typedef void (*gol_kernel_t)(const uint8_t*, uint8_t*, int, int, int, int);
 
 }  // namespace gol
 
-int gol_bits_run(appfis_ctx* ctx, uint8_t* b0, uint8_t* b1, int nx, int ny, int gens, int tsteps);
+int gol_bits_run(appfis_ctx* ctx, uint8_t* b0, uint8_t* b1, int nx, int ny, int gens, int force);
 
 extern "C" int appfis_gol(appfis_ctx* ctx, uint8_t* buf0, uint8_t* buf1, int nx, int ny, int gens, int variant,
                           int tsteps)
@@ -185,10 +185,12 @@ extern "C" int appfis_gol(appfis_ctx* ctx, uint8_t* buf0, uint8_t* buf1, int nx,
     APPFIS_REQUIRE(gens >= 0, "gens must be >= 0");
     APPFIS_REQUIRE(((uintptr_t)buf0 & 15) == 0 && ((uintptr_t)buf1 & 15) == 0, "GoL buffers must be 16-byte aligned");
     if (gens == 0) return APPFIS_OK;
-    APPFIS_REQUIRE(variant == APPFIS_PLAIN || variant == APPFIS_CHOSEN, "gol: unknown variant %d", variant);
+    APPFIS_REQUIRE(variant >= 0, "gol: unknown variant %d", variant);
+    if (variant >= APPFIS_CANDIDATE0)  // a forced blocked candidate of the bit-packed form
+        return gol_bits_run(ctx, buf0, buf1, nx, ny, gens, variant - APPFIS_CANDIDATE0);
     // the bit-packed form needs the periodic wrap on word boundaries; other widths run
     // the byte kernel, which gives the same bytes
-    if (variant == APPFIS_CHOSEN && nx % 32 == 0) return gol_bits_run(ctx, buf0, buf1, nx, ny, gens, tsteps);
+    if (variant == APPFIS_CHOSEN && nx % 32 == 0) return gol_bits_run(ctx, buf0, buf1, nx, ny, gens, -1);
     const int W = nx + 2, H = ny + 2;
     const int nbytes = W * H;
     const int nchunks = (nbytes + 15) / 16;

@@ -270,10 +270,17 @@ int run_tb(appfis_ctx* ctx, const golbits::TbVariant& v, uint32_t* const P[2], i
 // temporally blocked
 // (tile height and depth autotuned per shape), and the last one with the
 // one-generation kernel, so that generation gens-1 is still at hand to unpack.
-int gol_bits_run(appfis_ctx* ctx, uint8_t* b0, uint8_t* b1, int nx, int ny, int gens, int tsteps)
+int gol_bits_run(appfis_ctx* ctx, uint8_t* b0, uint8_t* b1, int nx, int ny, int gens, int force)
 {
-    (void)tsteps;
+    APPFIS_REQUIRE(nx % 32 == 0, "gol: the bit-packed form needs nx %% 32 == 0 (nx = %d)", nx);
     const int NW = nx / 32;
+    const int ntb = (int)(sizeof(golbits::kTb) / sizeof(golbits::kTb[0]));
+    if (force >= 0) {
+        APPFIS_REQUIRE(force < ntb, "gol: candidate %d out of range (%d candidates)", force, ntb);
+        APPFIS_REQUIRE(tb_fits(golbits::kTb[force], NW),
+                       "gol: candidate %s needs nx a multiple of 2048 that its block of waves spans",
+                       golbits::kTb[force].name);
+    }
     const size_t words = (size_t)NW * ny;
     const size_t half = (words * 4 + 255) / 256 * 256;
     int st = ensure_work(ctx, 2 * half);
@@ -283,14 +290,13 @@ int gol_bits_run(appfis_ctx* ctx, uint8_t* b0, uint8_t* b1, int nx, int ny, int 
     // (timing does not depend on the cell values)
     std::vector<int> cand;
     std::vector<const char*> names;
-    const int ntb = (int)(sizeof(golbits::kTb) / sizeof(golbits::kTb[0]));
     for (int i = 0; i < ntb; i++)
         if (tb_fits(golbits::kTb[i], NW)) {
             cand.push_back(i);
             names.push_back(golbits::kTb[i].name);
         }
-    int pick = -1;
-    if (!cand.empty() && gens > 1) {
+    int pick = force;
+    if (force < 0 && !cand.empty() && gens > 1) {
         const int key[6] = {4, APPFIS_U8, nx, ny, 0, 0};
         int c = 0;
         st = tune_select(ctx, key, (int)cand.size(), names.data(),
@@ -306,7 +312,7 @@ int gol_bits_run(appfis_ctx* ctx, uint8_t* b0, uint8_t* b1, int nx, int ny, int 
     hipLaunchKernelGGL(golbits::k_pack, dim3(pblocks), dim3(256), 0, ctx->stream, b0, P[0], nx, ny, NW);
     APPFIS_LAUNCH_CHECK();
     int cur = 0;
-    if (pick >= 0) {
+    if (pick >= 0 && gens > 1) {
         st = run_tb(ctx, golbits::kTb[pick], P, &cur, NW, ny, gens - 1);
         if (st) return st;
         st = run_step(ctx, P, &cur, NW, ny, 1);
@@ -330,4 +336,13 @@ int gol_bits_run(appfis_ctx* ctx, uint8_t* b0, uint8_t* b1, int nx, int ny, int 
         if (st) return st;
     }
     return APPFIS_OK;
+}
+
+extern "C" int appfis_gol_candidates(char* names, int len)
+{
+    const int ntb = (int)(sizeof(golbits::kTb) / sizeof(golbits::kTb[0]));
+    std::string all;
+    for (int i = 0; i < ntb; i++) all += std::string(golbits::kTb[i].name) + "\n";
+    if (names && len > 0) snprintf(names, (size_t)len, "%s", all.c_str());
+    return ntb;
 }

@@ -241,5 +241,26 @@ extern "C" int appfis_heat2d(appfis_ctx* ctx, int dtype, void* buf0, void* buf1,
         if (st) return st;
         return tab[pick].fn(ctx, buf0, buf1, nx, ny, c, iters, tab[pick].ts);
     }
+    if (variant >= APPFIS_CANDIDATE0) {
+        if (dtype != APPFIS_F64 && dtype != APPFIS_F32) return invalid("heat2d: dtype must be F64 or F32");
+        const HeatVariant* tab = dtype == APPFIS_F64 ? kHeatF64 : kHeatF32;
+        const int ntab = dtype == APPFIS_F64 ? (int)(sizeof(kHeatF64) / sizeof(kHeatF64[0]))
+                                             : (int)(sizeof(kHeatF32) / sizeof(kHeatF32[0]));
+        const int i = variant - APPFIS_CANDIDATE0;
+        APPFIS_REQUIRE(i < ntab, "heat2d: candidate %d out of range (%d candidates)", i, ntab);
+        return tab[i].fn(ctx, buf0, buf1, nx, ny, c, iters, tab[i].ts);
+    }
     return invalid("heat2d: unknown variant %d", variant);
+}
+
+extern "C" int appfis_heat2d_candidates(int dtype, char* names, int len)
+{
+    if (dtype != APPFIS_F64 && dtype != APPFIS_F32) return -invalid("heat2d: dtype must be F64 or F32");
+    const HeatVariant* tab = dtype == APPFIS_F64 ? kHeatF64 : kHeatF32;
+    const int ntab = dtype == APPFIS_F64 ? (int)(sizeof(kHeatF64) / sizeof(kHeatF64[0]))
+                                         : (int)(sizeof(kHeatF32) / sizeof(kHeatF32[0]));
+    std::string all;
+    for (int i = 0; i < ntab; i++) all += std::string(tab[i].name) + "\n";
+    if (names && len > 0) snprintf(names, (size_t)len, "%s", all.c_str());
+    return ntab;
 }

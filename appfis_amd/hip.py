@@ -118,6 +118,8 @@ def lib():
         "appfis_gol": ([vp, vp, vp, i, i, i, i, i], i),
         "appfis_fdtd3d": ([vp, i, vp, vp, i, i, i, dp, i, i], i),
         "appfis_fdtd3d_candidates": ([i, C.c_char_p, i], i),
+        "appfis_heat2d_candidates": ([i, C.c_char_p, i], i),
+        "appfis_gol_candidates": ([C.c_char_p, i], i),
         "appfis_himeno": ([vp, i, C.POINTER(vp), i, i, i, d, i, i, i, dp], i),
         "appfis_flood_run": ([vp, i, vp, vp, vp, vp, vp, i, i, C.POINTER(FloodParams), i, i, dp, dp], i),
         "appfis_flood_ensemble_run": ([vp, i, i, ip, ip, C.POINTER(vp), C.POINTER(vp), C.POINTER(vp),
@@ -132,13 +134,27 @@ def lib():
     return L
 
 
-def fdtd3d_candidates(dtype: int):
-    """Names of the FDTD3d candidate tilings (variant CANDIDATE0 + index)."""
+def _candidates(fn, *args):
     buf = C.create_string_buffer(4096)
-    n = lib().appfis_fdtd3d_candidates(dtype, buf, 4096)
+    n = fn(*args, buf, 4096)
     if n < 0:
         raise AppfisError(lib().appfis_last_error().decode(errors="replace"))
     return buf.value.decode().splitlines()[:n]
+
+
+def fdtd3d_candidates(dtype: int):
+    """Names of the FDTD3d candidate tilings (variant CANDIDATE0 + index)."""
+    return _candidates(lib().appfis_fdtd3d_candidates, dtype)
+
+
+def heat2d_candidates(dtype: int):
+    """Names of the Heat2D temporal-blocking candidates (variant CANDIDATE0 + index)."""
+    return _candidates(lib().appfis_heat2d_candidates, dtype)
+
+
+def gol_candidates():
+    """Names of the bit-packed GoL blocked kernels (variant CANDIDATE0 + index)."""
+    return _candidates(lib().appfis_gol_candidates)
 
 
 def check(status: int, what: str = "") -> None:

@@ -154,15 +154,25 @@ int appfis_reduce_device(appfis_ctx* ctx, const void* data, int dtype, int ndim,
 int appfis_heat2d(appfis_ctx* ctx, int dtype, void* buf0, void* buf1, int nx, int ny, double c,
                   int iters, int variant, int tsteps);
 
+/* Temporal-blocking candidates of appfis_heat2d CHOSEN for `dtype` (variant
+ * APPFIS_CANDIDATE0 + i), one name per line; returns their number (< 0: bad dtype). */
+int appfis_heat2d_candidates(int dtype, char* names, int len);
+
 /* ---- GameOfLife 2D, uint8, periodic, rule B3/S23 (config form of GameOfLife/src/
  *      game_appfis.hpp:121-135 plus the periodic exchange of Communication.hpp:613-678).
  * bufs as for heat2d (ghost 1, padded (nx+2)*(ny+2)).  The ghost ring of the
  * written buffer is refreshed every generation.  PLAIN: one generation per launch;
  * CHOSEN (nx % 32 == 0): the generations run on a bit-packed copy of the field (32
  * cells per word), both buffers are left exactly as PLAIN leaves them; other widths
- * run the PLAIN kernel.  `tsteps` is reserved (pass 0).  Cells must hold 0 or 1. */
+ * run the PLAIN kernel.  `tsteps` is reserved (pass 0).  Cells must hold 0 or 1.
+ * APPFIS_CANDIDATE0 + i: the bit-packed form with blocked candidate i forced. */
 int appfis_gol(appfis_ctx* ctx, uint8_t* buf0, uint8_t* buf1, int nx, int ny, int gens,
                int variant, int tsteps);
+
+/* Blocked kernels of the bit-packed GoL form (variant APPFIS_CANDIDATE0 + i forces
+ * candidate i; it needs nx a multiple of 2048 spanned by its block), one name per
+ * line; returns their number. */
+int appfis_gol_candidates(char* names, int len);
 
 /* ---- FDTD3d radius-4 star (config form of FDTD3d/src/fdtd3d_appfis.hpp:129-143):
  * u' = c[0]*u + sum_{r=1..4} c[r]*(6 neighbours at distance r); ghost 4 held at 0.

@@ -42,6 +42,20 @@ def test_heat2d_chosen_depths(ctx, tsteps):
     assert np.array_equal((d0, d1)[iters % 2].download(), ref)
 
 
+@pytest.mark.parametrize("dt", [hip.F64, hip.F32])
+def test_heat2d_every_candidate_bit_exact(ctx, dt):
+    # every temporal-blocking candidate the autotuner may pick, forced one at a time
+    nx, ny, iters = 200, 150, 37
+    a, d0, d1 = _heat_pair(ctx, nx, ny, dt)
+    ref = oracle.heat2d(a.copy(), np.zeros_like(a), nx, ny, 0.1, iters)
+    names = hip.heat2d_candidates(dt)
+    assert len(names) >= 2
+    for i, name in enumerate(names):
+        d0.upload(a)
+        ctx.heat2d(d0, d1, nx, ny, 0.1, iters, hip.CANDIDATE0 + i)
+        assert np.array_equal((d0, d1)[iters % 2].download(), ref), name
+
+
 def test_heat2d_full_config(ctx):
     # 1024^2 FP64, 500 sweeps: the whole config against the oracle
     nx = ny = 1024
@@ -106,6 +120,27 @@ def test_gol_chosen_both_buffers(ctx, nx, ny, gens):
     assert np.array_equal((d0, d1)[gens % 2].download(), ref)
     prev = oracle.gol_u8(a.copy(), np.zeros_like(a), nx, ny, gens - 1)
     assert np.array_equal((d0, d1)[(gens + 1) % 2].download(), prev)
+
+
+@pytest.mark.parametrize("nx,ny,gens", [(2048, 45, 17), (4096, 30, 9), (8192, 21, 6)])
+def test_gol_every_blocked_candidate(ctx, nx, ny, gens):
+    # every blocked kernel of the bit-packed form, forced one at a time (candidates whose
+    # block does not span nx refuse with an error)
+    a, d0, d1 = _gol_pair(ctx, nx, ny, seed=nx + gens)
+    ref = oracle.gol_u8(a.copy(), np.zeros_like(a), nx, ny, gens)
+    prev = oracle.gol_u8(a.copy(), np.zeros_like(a), nx, ny, gens - 1)
+    ran = 0
+    for i, name in enumerate(hip.gol_candidates()):
+        d0.upload(a)
+        try:
+            ctx.gol(d0, d1, nx, ny, gens, hip.CANDIDATE0 + i)
+        except hip.AppfisError as e:
+            assert "spanned" in str(e) or "needs nx" in str(e), e
+            continue
+        ran += 1
+        assert np.array_equal((d0, d1)[gens % 2].download(), ref), name
+        assert np.array_equal((d0, d1)[(gens + 1) % 2].download(), prev), name
+    assert ran >= 3
 
 
 def test_gol_chosen_glider_period(ctx):
