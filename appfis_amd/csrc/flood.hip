@@ -3,12 +3,12 @@
 // rainfall (added as h += rain*dt right after the update, where the reference adds its
 // point sources), fused into two kernels per step:
 //
-//   K1 (flux + update + rain): one thread per interior cell evaluates the Roe flux of
-//       its four edges (X edges (i,i+1) and (i-1,i), Y edges (j,j-1) and (j+1,j)),
-//       accumulates rhs*0 / rhs*1 in the order of the reference's X-then-Y functors,
-//       applies the update with implicit Manning friction and the rain, and writes
-//       the new state to the other buffer of a ping-pong pair (the reference's reset
-//       pass and six rhs arrays never touch memory).
+//   K1 (flux + update + rain): a 64 x 8 tile solves each of its edges once (X edges,
+//       then Y edges, through LDS), every cell accumulates rhs*0 / rhs*1 from its four
+//       edges in the order of the reference's X-then-Y functors, applies the update
+//       with implicit Manning friction and the rain, and writes the new state to the
+//       other buffer of a ping-pong pair (the reference's reset pass and six rhs
+//       arrays never touch memory).
 //   K2 (wet/dry fix + CFL + volume): the momentum fix of triton_appfis.hpp:1095-1112 in
 //       place, then the CFL candidate of triton_appfis.hpp:567-586 from the fixed state
 //       (wave/block min) and a compensated FP64 sum of h, written as one partial per
@@ -38,6 +38,7 @@ using namespace appfis;
 namespace {
 
 constexpr int BX = 64, BY = 4, BT = BX * BY;
+constexpr int EBY = 8;  // rows of a flux-kernel tile (BX x EBY cells, BT threads)
 constexpr int CHUNK = 20;  // steps per captured graph (even: buffer parity returns)
 
 template <typename R> struct RealLim;
@@ -64,8 +65,11 @@ struct Scen {
     const R* n2;
     int nx, ny;
     int tiles_x;
-    int tile0;         // first global tile of this scenario
+    int tile0;         // first global tile of this scenario (BX x BY tiles: wet/dry, CFL, volume)
     int ntiles;
+    int etiles_x;      // the flux kernel's BX x EBY tiles
+    int etile0;
+    int entiles;
     ScenState* st;
     double* partials;  // 2 doubles per tile (compensated volume partial)
     double* rec_time;  // per-step records (nsteps each) or null
@@ -78,6 +82,15 @@ struct Params {
     int nsteps_rec;  // length of the record arrays
 };
 
+// Division and square root: correctly rounded for R = double (the reference's
+// arithmetic: the FP64 form reproduces the oracle); for the FP32 bulk form the hardware
+// reciprocal and square-root approximations (v_rcp_f32 / v_sqrt_f32, about 1 ulp), well
+// inside that form's stated error against FP64.
+template <typename R> __device__ __forceinline__ R fdiv(R a, R b) { return a / b; }
+template <> __device__ __forceinline__ float fdiv<float>(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
+template <typename R> __device__ __forceinline__ R fsqrt(R a) { return sqrt(a); }
+template <> __device__ __forceinline__ float fsqrt<float>(float a) { return __builtin_amdgcn_sqrtf(a); }
+
 // Roe flux with entropy fix, bed-slope source and wet/dry treatment -- the shared body
 // of the X functor (triton_appfis.hpp:608-824) and the Y functor (827-1043).
 template <typename R>
@@ -88,15 +101,15 @@ __device__ inline bool roe(R h1, R h2, R qx1, R qy1, R z1, R qx2, R qy2, R z2, R
     if (!(h1 > (R)0.0 || h2 > (R)0.0)) return false;
     R alpha[3], beta[3], eigen[3], eigenE[3], eigev[3][3];
     R u1, u2, v1, v2;
-    if (h1 >= hextra) { u1 = qx1 / h1; v1 = qy1 / h1; }
+    if (h1 >= hextra) { u1 = fdiv(qx1, h1); v1 = fdiv(qy1, h1); }
     else { u1 = (R)0.0; v1 = (R)0.0; }
-    if (h2 >= hextra) { u2 = qx2 / h2; v2 = qy2 / h2; }
+    if (h2 >= hextra) { u2 = fdiv(qx2, h2); v2 = fdiv(qy2, h2); }
     else { u2 = (R)0.0; v2 = (R)0.0; }
     const R h12 = (R)0.5 * (h1 + h2);
     const R dh = h2 - h1;
-    const R c12 = sqrt(G * h12);
-    const R u12 = (u1 * sqh1 + u2 * sqh2) / (sqh1 + sqh2);
-    const R v12 = (v1 * sqh1 + v2 * sqh2) / (sqh1 + sqh2);
+    const R c12 = fsqrt(G * h12);
+    const R u12 = fdiv(u1 * sqh1 + u2 * sqh2, sqh1 + sqh2);
+    const R v12 = fdiv(v1 * sqh1 + v2 * sqh2, sqh1 + sqh2);
     const R un = u12 * nx_ + v12 * ny_;
     eigen[0] = un - c12;
     eigen[1] = un;
@@ -106,14 +119,14 @@ __device__ inline bool roe(R h1, R h2, R qx1, R qy1, R z1, R qx2, R qy2, R z2, R
     R e1 = u1 * nx_ + v1 * ny_ - SQRTG * sqh1;
     R e2 = u2 * nx_ + v2 * ny_ - SQRTG * sqh2;
     if (e1 < (R)0.0 && e2 > (R)0.0) {
-        eigenE[0] = eigen[0] - e1 * (e2 - eigen[0]) / (e2 - e1);
-        eigen[0] = e1 * (e2 - eigen[0]) / (e2 - e1);
+        eigenE[0] = eigen[0] - fdiv(e1 * (e2 - eigen[0]), e2 - e1);
+        eigen[0] = fdiv(e1 * (e2 - eigen[0]), e2 - e1);
     }
     e1 = u1 * nx_ + v1 * ny_ + SQRTG * sqh1;
     e2 = u2 * nx_ + v2 * ny_ + SQRTG * sqh2;
     if (e1 < (R)0.0 && e2 > (R)0.0) {
-        eigenE[2] = eigen[2] - e2 * (eigen[2] - e1) / (e2 - e1);
-        eigen[2] = e2 * (eigen[2] - e1) / (e2 - e1);
+        eigenE[2] = eigen[2] - fdiv(e2 * (eigen[2] - e1), e2 - e1);
+        eigen[2] = fdiv(e2 * (eigen[2] - e1), e2 - e1);
     }
     eigev[0][0] = (R)1.0;
     eigev[0][1] = u12 - c12 * nx_;
@@ -124,9 +137,9 @@ __device__ inline bool roe(R h1, R h2, R qx1, R qy1, R z1, R qx2, R qy2, R z2, R
     eigev[2][0] = (R)1.0;
     eigev[2][1] = u12 + c12 * nx_;
     eigev[2][2] = v12 + c12 * ny_;
-    alpha[0] = (R)0.5 * (dh - (((qx2 - qx1) * nx_ + (qy2 - qy1) * ny_) - un * dh) / c12);
-    alpha[1] = (((qy2 - qy1) - v12 * dh) * nx_ - ((qx2 - qx1) - u12 * dh) * ny_) / c12;
-    alpha[2] = (R)0.5 * (dh + (((qx2 - qx1) * nx_ + (qy2 - qy1) * ny_) - un * dh) / c12);
+    alpha[0] = (R)0.5 * (dh - fdiv(((qx2 - qx1) * nx_ + (qy2 - qy1) * ny_) - un * dh, c12));
+    alpha[1] = fdiv(((qy2 - qy1) - v12 * dh) * nx_ - ((qx2 - qx1) - u12 * dh) * ny_, c12);
+    alpha[2] = (R)0.5 * (dh + fdiv(((qx2 - qx1) * nx_ + (qy2 - qy1) * ny_) - un * dh, c12));
     const R dz = z2 - z1;
     R l1 = z1 + h1;
     R l2 = z2 + h2;
@@ -139,18 +152,18 @@ __device__ inline bool roe(R h1, R h2, R qx1, R qy1, R z1, R qx2, R qy2, R z2, R
         hp = h2;
         if (l2 < z1) dzp = -h2;
     }
-    beta[0] = (R)0.5 * G * (hp - (R)0.5 * fabs(dzp)) * dzp / c12;
+    beta[0] = fdiv((R)0.5 * G * (hp - (R)0.5 * fabs(dzp)) * dzp, c12);
     hp = h1 + alpha[0];
     if (eigen[0] * eigen[2] < (R)0.0 && hp > (R)0.0 && h1 > (R)0.0 && h2 > (R)0.0) {
-        beta[0] = fmax(beta[0], alpha[0] * eigen[0] - h1 * dx * (R)0.5 / dt);
-        beta[0] = fmin(beta[0], -alpha[2] * eigen[2] + h2 * dx * (R)0.5 / dt);
+        beta[0] = fmax(beta[0], alpha[0] * eigen[0] - fdiv(h1 * dx * (R)0.5, dt));
+        beta[0] = fmin(beta[0], -alpha[2] * eigen[2] + fdiv(h2 * dx * (R)0.5, dt));
     }
     beta[1] = (R)0.0;
     beta[2] = -beta[0];
 #pragma unroll
     for (int i = 0; i < 3; i++) { d1[i] = (R)0.0; d2[i] = (R)0.0; }
-    l1 = hp - beta[0] / eigen[0];
-    l2 = hp + beta[2] / eigen[2];
+    l1 = hp - fdiv(beta[0], eigen[0]);
+    l2 = hp + fdiv(beta[2], eigen[2]);
     if (l2 < -EPS12 && h2 < EPS12) {
         d1[0] += (eigen[0] * alpha[0] - beta[0]) + (eigen[2] * alpha[2] - beta[2]);
     } else if (l1 < -EPS12 && h1 < EPS12) {
@@ -189,97 +202,159 @@ __device__ inline bool roe(R h1, R h2, R qx1, R qy1, R z1, R qx2, R qy2, R z2, R
     return true;
 }
 
-template <typename R> __device__ inline R sqrt_pos(R h) { return h > (R)0.0 ? sqrt(h) : (R)0.0; }
+template <typename R> __device__ inline R sqrt_pos(R h) { return h > (R)0.0 ? fsqrt(h) : (R)0.0; }
 
-// scenario of a global tile: binary search over tile0 (uniform per block)
-template <typename R>
+// scenario of a global tile: binary search over tile0 (or etile0 for the flux tiles),
+// uniform per block
+template <typename R, bool FLUX = false>
 __device__ inline int find_scen(const Scen<R>* sc, int nscen, int tile)
 {
     int lo = 0, hi = nscen - 1;
     while (lo < hi) {
         int mid = (lo + hi + 1) >> 1;
-        if (sc[mid].tile0 <= tile) lo = mid;
+        if ((FLUX ? sc[mid].etile0 : sc[mid].tile0) <= tile) lo = mid;
         else hi = mid - 1;
     }
     return lo;
 }
 
+// K1 on a BX x EBY tile, each edge solved once: phase X solves the (BX+1) x EBY edges
+// between horizontal neighbours into LDS and every cell picks up its two (the edge to
+// its right gives rhs*0, the edge to its left rhs*1); phase Y does the same with the
+// BX x (EBY+1) edges between vertical neighbours (the edge below gives rhs*0 +=, the
+// edge above rhs*1 +=).  The arguments of every Roe solve and the order of the rhs
+// accumulation are those of the per-cell form, so the results are identical; each
+// edge is solved once instead of twice.
+template <typename R>
+__device__ inline void edge_store(R (*buf)[BX * (EBY + 1)], unsigned char* flag, int e, bool ok, const R d1[3],
+                                  const R d2[3])
+{
+    flag[e] = ok ? 1 : 0;
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+        buf[k][e] = d1[k];
+        buf[3 + k][e] = d2[k];
+    }
+}
+
 template <typename R>
 __global__ __launch_bounds__(BT) void k_flux_update(const Scen<R>* __restrict__ sc, int nscen, int par, Params P)
 {
+    constexpr int NXE = (BX + 1) * EBY, NYE = BX * (EBY + 1);
+    constexpr int NE = NXE > NYE ? NXE : NYE;
+    static_assert(NE <= BX * (EBY + 1), "edge buffer size");
+    __shared__ R ebuf[6][BX * (EBY + 1)];  // d1[0..2], d2[0..2] per edge (structure of arrays)
+    __shared__ unsigned char eok[BX * (EBY + 1)];
+    constexpr int CPT = BX * EBY / BT;  // cells per thread
     const int tile = blockIdx.x;
-    const int s = find_scen(sc, nscen, tile);
-    const Scen<R>& S = sc[s];  // a reference: a copy indexed by `par` would live in scratch
-    const int t = tile - S.tile0;
-    const int tyi = t / S.tiles_x, txi = t - tyi * S.tiles_x;
-    const int i = txi * BX + (threadIdx.x & (BX - 1)) + 1;
-    const int j = tyi * BY + (threadIdx.x >> 6) + 1;
-    if (i > S.nx || j > S.ny) return;
-    const int W = S.nx + 2;
+    const int s = find_scen<R, true>(sc, nscen, tile);
+    const Scen<R>& S = sc[s];
+    const int t = tile - S.etile0;
+    const int tyi = t / S.etiles_x, txi = t - tyi * S.etiles_x;
+    const int i0 = txi * BX + 1, j0 = tyi * EBY + 1;  // first interior cell of the tile
+    const int nx = S.nx, ny = S.ny, W = nx + 2;
     const R* h = S.h[par];
     const R* qx = S.qx[par];
     const R* qy = S.qy[par];
     const R* z = S.z;
     const R dt = (R)S.st->dt;
     const R dx = (R)P.dx, hextra = (R)P.hextra, rain = (R)P.rain;
-    const size_t c = (size_t)i + (size_t)W * (size_t)j;
-    const size_t e = c + 1, w = c - 1, dn = c - W, up = c + W;  // up = j+1, dn = j-1
-    const R hc = h[c], he = h[e], hw = h[w], hd = h[dn], hu = h[up];
-    const R qxc = qx[c], qyc = qy[c], zc = z[c];
-    const R sc_ = sqrt_pos(hc);
-    R rh0 = (R)0.0, rqx0 = (R)0.0, rqy0 = (R)0.0, rh1 = (R)0.0, rqx1 = (R)0.0, rqy1 = (R)0.0;
-    R d1[3], d2[3];
-    // X edge (i, i+1): this cell is side 1 -> rhs*0 (overwrite, triton_appfis.hpp:817-819)
-    if (roe(hc, he, qxc, qyc, zc, qx[e], qy[e], z[e], sc_, sqrt_pos(he), (R)1.0, (R)0.0, dx, dt, hextra, d1, d2)) {
-        rh0 = dt * d1[0] / dx;
-        rqx0 = dt * d1[1] / dx;
-        rqy0 = dt * d1[2] / dx;
+    const int lx = threadIdx.x & (BX - 1);
+    R rh0[CPT], rqx0[CPT], rqy0[CPT], rh1[CPT], rqx1[CPT], rqy1[CPT];
+    // ---- X edges: edge (col, row) between cells (i0-1+col, j) and (i0+col, j)
+    for (int e = threadIdx.x; e < NXE; e += BT) {
+        const int row = e / (BX + 1), col = e - row * (BX + 1);
+        const int iL = i0 - 1 + col, j = j0 + row;
+        R d1[3] = {(R)0, (R)0, (R)0}, d2[3] = {(R)0, (R)0, (R)0};
+        bool ok = false;
+        if (j <= ny && iL <= nx) {
+            const size_t cL = (size_t)iL + (size_t)W * (size_t)j, cR = cL + 1;
+            const R hL = h[cL], hR = h[cR];
+            ok = roe(hL, hR, qx[cL], qy[cL], z[cL], qx[cR], qy[cR], z[cR], sqrt_pos(hL), sqrt_pos(hR), (R)1.0,
+                     (R)0.0, dx, dt, hextra, d1, d2);
+        }
+        edge_store(ebuf, eok, e, ok, d1, d2);
     }
-    // X edge (i-1, i): this cell is side 2 -> rhs*1 (overwrite, triton_appfis.hpp:821-823)
-    if (roe(hw, hc, qx[w], qy[w], z[w], qxc, qyc, zc, sqrt_pos(hw), sc_, (R)1.0, (R)0.0, dx, dt, hextra, d1, d2)) {
-        rh1 = dt * d2[0] / dx;
-        rqx1 = dt * d2[1] / dx;
-        rqy1 = dt * d2[2] / dx;
-    }
-    // Y edge jj = j (id1 = j, id2 = j-1): this cell is side 1 -> rhs*0 += (1034-1036)
-    if (roe(hc, hd, qxc, qyc, zc, qx[dn], qy[dn], z[dn], sc_, sqrt_pos(hd), (R)0.0, (R)1.0, dx, dt, hextra, d1, d2)) {
-        rh0 = rh0 + dt * d1[0] / dx;
-        rqx0 = rqx0 + dt * d1[1] / dx;
-        rqy0 = rqy0 + dt * d1[2] / dx;
-    }
-    // Y edge jj = j+1 (id1 = j+1, id2 = j): this cell is side 2 -> rhs*1 += (1038-1040)
-    if (roe(hu, hc, qx[up], qy[up], z[up], qxc, qyc, zc, sqrt_pos(hu), sc_, (R)0.0, (R)1.0, dx, dt, hextra, d1, d2)) {
-        rh1 = rh1 + dt * d2[0] / dx;
-        rqx1 = rqx1 + dt * d2[1] / dx;
-        rqy1 = rqy1 + dt * d2[2] / dx;
-    }
-    // update with implicit Manning friction (triton_appfis.hpp:1046-1092)
-    const R G = (R)9.81, EPS12 = (R)1e-12;
-    R hij, qxij, qyij;
-    const R hn = hc;
-    hij = hn - rh0 - rh1;
-    if (hij < (R)0.0) hij = (R)0.0;
-    if (hij < hextra) {
-        qxij = (R)0.0;
-        qyij = (R)0.0;
-    } else {
-        const R mx = qxc - rqx0 - rqx1;
-        const R my = qyc - rqy0 - rqy1;
-        const R modM = sqrt(mx * mx + my * my);
-        const R nn = S.n2[c];
-        if (nn > EPS12 && hn >= hextra && modM > EPS12) {
-            const R tt = dt * G * nn * modM / (hn * hn * cbrt(hn));
-            qxij = (R)-0.5 * (mx - mx * sqrt((R)1.0 + (R)4.0 * tt)) / tt;
-            qyij = (R)-0.5 * (my - my * sqrt((R)1.0 + (R)4.0 * tt)) / tt;
-        } else {
-            qxij = mx;
-            qyij = my;
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < CPT; k++) {
+        const int ly = (threadIdx.x >> 6) + k * (BT / BX);
+        const int eR = ly * (BX + 1) + lx + 1, eL = eR - 1;  // edges (i, i+1) and (i-1, i)
+        rh0[k] = (R)0.0; rqx0[k] = (R)0.0; rqy0[k] = (R)0.0;
+        rh1[k] = (R)0.0; rqx1[k] = (R)0.0; rqy1[k] = (R)0.0;
+        if (eok[eR]) {  // cell is side 1: rhs*0 = (triton_appfis.hpp:817-819)
+            rh0[k] = fdiv(dt * ebuf[0][eR], dx);
+            rqx0[k] = fdiv(dt * ebuf[1][eR], dx);
+            rqy0[k] = fdiv(dt * ebuf[2][eR], dx);
+        }
+        if (eok[eL]) {  // cell is side 2: rhs*1 = (triton_appfis.hpp:821-823)
+            rh1[k] = fdiv(dt * ebuf[3][eL], dx);
+            rqx1[k] = fdiv(dt * ebuf[4][eL], dx);
+            rqy1[k] = fdiv(dt * ebuf[5][eL], dx);
         }
     }
+    __syncthreads();
+    // ---- Y edges: edge (col, row) between cells (i, jU) (side 1) and (i, jU-1) (side 2),
+    //      jU = j0 + row
+    for (int e = threadIdx.x; e < NYE; e += BT) {
+        const int row = e / BX, col = e - row * BX;
+        const int i = i0 + col, jU = j0 + row;
+        R d1[3] = {(R)0, (R)0, (R)0}, d2[3] = {(R)0, (R)0, (R)0};
+        bool ok = false;
+        if (i <= nx && jU <= ny + 1) {
+            const size_t cU = (size_t)i + (size_t)W * (size_t)jU, cD = cU - W;
+            const R hU = h[cU], hD = h[cD];
+            ok = roe(hU, hD, qx[cU], qy[cU], z[cU], qx[cD], qy[cD], z[cD], sqrt_pos(hU), sqrt_pos(hD), (R)0.0,
+                     (R)1.0, dx, dt, hextra, d1, d2);
+        }
+        edge_store(ebuf, eok, e, ok, d1, d2);
+    }
+    __syncthreads();
+    const R G = (R)9.81, EPS12 = (R)1e-12;
     const int o = par ^ 1;
-    S.h[o][c] = hij + rain * dt;
-    S.qx[o][c] = qxij;
-    S.qy[o][c] = qyij;
+#pragma unroll
+    for (int k = 0; k < CPT; k++) {
+        const int ly = (threadIdx.x >> 6) + k * (BT / BX);
+        const int i = i0 + lx, j = j0 + ly;
+        if (i > nx || j > ny) continue;
+        const int eD = ly * BX + lx, eU = eD + BX;  // edges (j, j-1) and (j+1, j)
+        if (eok[eD]) {  // cell is side 1: rhs*0 += (triton_appfis.hpp:1034-1036)
+            rh0[k] = rh0[k] + fdiv(dt * ebuf[0][eD], dx);
+            rqx0[k] = rqx0[k] + fdiv(dt * ebuf[1][eD], dx);
+            rqy0[k] = rqy0[k] + fdiv(dt * ebuf[2][eD], dx);
+        }
+        if (eok[eU]) {  // cell is side 2: rhs*1 += (triton_appfis.hpp:1038-1040)
+            rh1[k] = rh1[k] + fdiv(dt * ebuf[3][eU], dx);
+            rqx1[k] = rqx1[k] + fdiv(dt * ebuf[4][eU], dx);
+            rqy1[k] = rqy1[k] + fdiv(dt * ebuf[5][eU], dx);
+        }
+        // update with implicit Manning friction (triton_appfis.hpp:1046-1092)
+        const size_t c = (size_t)i + (size_t)W * (size_t)j;
+        const R hn = h[c], qxc = qx[c], qyc = qy[c];
+        R hij, qxij, qyij;
+        hij = hn - rh0[k] - rh1[k];
+        if (hij < (R)0.0) hij = (R)0.0;
+        if (hij < hextra) {
+            qxij = (R)0.0;
+            qyij = (R)0.0;
+        } else {
+            const R mx = qxc - rqx0[k] - rqx1[k];
+            const R my = qyc - rqy0[k] - rqy1[k];
+            const R modM = fsqrt(mx * mx + my * my);
+            const R nn = S.n2[c];
+            if (nn > EPS12 && hn >= hextra && modM > EPS12) {
+                const R tt = fdiv(dt * G * nn * modM, hn * hn * cbrt(hn));
+                qxij = fdiv((R)-0.5 * (mx - mx * fsqrt((R)1.0 + (R)4.0 * tt)), tt);
+                qyij = fdiv((R)-0.5 * (my - my * fsqrt((R)1.0 + (R)4.0 * tt)), tt);
+            } else {
+                qxij = mx;
+                qyij = my;
+            }
+        }
+        S.h[o][c] = hij + rain * dt;
+        S.qx[o][c] = qxij;
+        S.qy[o][c] = qyij;
+    }
 }
 
 __device__ inline void two_sum(double& s, double& cmp, double x)
@@ -330,8 +405,8 @@ __global__ __launch_bounds__(BT) void k_fix_cfl(const Scen<R>* __restrict__ sc, 
             }
         }
         if (hij > hextra) {
-            const R dtx = (dx / (fabs(qxv / hij) + sqrt(G * hij)));
-            const R dty = (dx / (fabs(qyv / hij) + sqrt(G * hij)));
+            const R dtx = fdiv(dx, fabs(fdiv(qxv, hij)) + fsqrt(G * hij));
+            const R dty = fdiv(dx, fabs(fdiv(qyv, hij)) + fsqrt(G * hij));
             cfl = dtx < dty ? cn * dtx : cn * dty;
         }
         vs = (double)hij;
@@ -453,7 +528,7 @@ int run_flood(appfis_ctx* ctx, int nscen, const int* nx, const int* ny, void* co
     APPFIS_REQUIRE(prm->dx > 0.0, "dx must be positive");
     // tile work-list
     std::vector<Scen<R>> hs((size_t)nscen);
-    long long total_tiles = 0;
+    long long total_tiles = 0, total_etiles = 0;
     size_t field_bytes = 0;
     for (int s = 0; s < nscen; s++) {
         int tx = (int)ceil_div(nx[s], BX), ty = (int)ceil_div(ny[s], BY);
@@ -463,9 +538,14 @@ int run_flood(appfis_ctx* ctx, int nscen, const int* nx, const int* ny, void* co
         hs[s].tile0 = (int)total_tiles;
         hs[s].ntiles = tx * ty;
         total_tiles += (long long)tx * ty;
+        const int ety = (int)ceil_div(ny[s], EBY);
+        hs[s].etiles_x = tx;
+        hs[s].etile0 = (int)total_etiles;
+        hs[s].entiles = tx * ety;
+        total_etiles += (long long)tx * ety;
         field_bytes += ((size_t)(nx[s] + 2) * (ny[s] + 2) * sizeof(R) + 255) / 256 * 256;
     }
-    APPFIS_REQUIRE(total_tiles < (1LL << 31), "too many tiles");
+    APPFIS_REQUIRE(total_tiles < (1LL << 31) && total_etiles < (1LL << 31), "too many tiles");
     const int rec = steps > 0 ? steps : 1;
     // workspace: second buffers (h, qx, qy) | states | partials | records | descriptor table
     const size_t st_bytes = ((size_t)nscen * sizeof(ScenState) + 255) / 256 * 256;
@@ -512,14 +592,14 @@ int run_flood(appfis_ctx* ctx, int nscen, const int* nx, const int* ny, void* co
     Params P{prm->dx, prm->courant, prm->hextra, prm->dt_fallback, prm->print_interval, prm->rain_rate, rec};
     hipLaunchKernelGGL(k_init_state, dim3(ceil_div(nscen, 64)), dim3(64), 0, ctx->stream, st, nscen, prm->start_time);
     APPFIS_LAUNCH_CHECK();
-    const dim3 grid((unsigned)total_tiles);
+    const dim3 grid((unsigned)total_tiles), egrid((unsigned)total_etiles);
     // dt of the first step from the initial state
     hipLaunchKernelGGL((k_fix_cfl<R, 0>), grid, dim3(BT), 0, ctx->stream, tab, nscen, 0, P);
     APPFIS_LAUNCH_CHECK();
     hipLaunchKernelGGL((k_finalize<R, 0>), dim3(nscen), dim3(BT), 0, ctx->stream, tab, P);
     APPFIS_LAUNCH_CHECK();
     auto step_pair = [&](int par) -> int {
-        hipLaunchKernelGGL((k_flux_update<R>), grid, dim3(BT), 0, ctx->stream, tab, nscen, par, P);
+        hipLaunchKernelGGL((k_flux_update<R>), egrid, dim3(BT), 0, ctx->stream, tab, nscen, par, P);
         APPFIS_LAUNCH_CHECK();
         hipLaunchKernelGGL((k_fix_cfl<R, 1>), grid, dim3(BT), 0, ctx->stream, tab, nscen, par ^ 1, P);
         APPFIS_LAUNCH_CHECK();
