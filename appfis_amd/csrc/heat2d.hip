@@ -57,9 +57,10 @@ __global__ __launch_bounds__(256) void k_heat2d_plain(const T* __restrict__ u, T
 }
 
 // Temporal blocking: TX x TY output tile, TS sweeps per launch, 256 threads.
+// One TX x TY output tile (tile coordinates tbx, tby) of one grid, `steps` <= TS sweeps.
 template <typename T, int TX, int TY, int TS>
-__global__ __launch_bounds__(256) void k_heat2d_tb(const T* __restrict__ u, T* __restrict__ v, int nx, int ny, T c,
-                                                   int steps)
+__device__ inline void tb_tile(const T* __restrict__ u, T* __restrict__ v, int nx, int ny, T c, int steps, int tbx,
+                               int tby)
 {
     constexpr int RX = TX + 2 * TS, RY = TY + 2 * TS;
     constexpr int NCELL = RX * RY;
@@ -67,9 +68,6 @@ __global__ __launch_bounds__(256) void k_heat2d_tb(const T* __restrict__ u, T* _
     __shared__ T tile[RY][RX];
     const int W = nx + 2, H = ny + 2;
     // window origin in padded coordinates (may be negative / past the array)
-    const unsigned nbx = gridDim.x;
-    const unsigned t = xcd_tile(blockIdx.x + nbx * blockIdx.y, nbx * gridDim.y);
-    const int tbx = (int)(t % nbx), tby = (int)(t / nbx);
     const int gx0 = tbx * TX + 1 - TS;
     const int gy0 = tby * TY + 1 - TS;
     const int tid = threadIdx.x;
@@ -119,6 +117,39 @@ __global__ __launch_bounds__(256) void k_heat2d_tb(const T* __restrict__ u, T* _
         int gx = tbx * TX + 1 + lx, gy = tby * TY + 1 + ly;
         if (gx <= nx && gy <= ny) v[(size_t)gx + (size_t)W * (size_t)gy] = tile[ly + TS][lx + TS];
     }
+}
+
+template <typename T, int TX, int TY, int TS>
+__global__ __launch_bounds__(256) void k_heat2d_tb(const T* __restrict__ u, T* __restrict__ v, int nx, int ny, T c,
+                                                   int steps)
+{
+    const unsigned nbx = gridDim.x;
+    const unsigned t = xcd_tile(blockIdx.x + nbx * blockIdx.y, nbx * gridDim.y);
+    tb_tile<T, TX, TY, TS>(u, v, nx, ny, c, steps, (int)(t % nbx), (int)(t / nbx));
+}
+
+// Ensemble: grid s owns tiles tile0 .. tile0 + tiles_x * tiles_y - 1 of one launch.
+template <typename T>
+struct HeatGrid {
+    T* b[2];
+    int nx, ny, tiles_x, tile0;
+};
+
+template <typename T, int TX, int TY, int TS>
+__global__ __launch_bounds__(256) void k_heat2d_tb_ens(const HeatGrid<T>* __restrict__ tab, int ngrid, int par, T c,
+                                                       int steps)
+{
+    const int tile = (int)xcd_tile(blockIdx.x, gridDim.x);
+    int lo = 0, hi = ngrid - 1;  // the grid of this tile: binary search over tile0 (uniform)
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (tab[mid].tile0 <= tile) lo = mid;
+        else hi = mid - 1;
+    }
+    const HeatGrid<T>& G = tab[lo];
+    const int t = tile - G.tile0;
+    const int tby = t / G.tiles_x;
+    tb_tile<T, TX, TY, TS>(G.b[par], G.b[par ^ 1], G.nx, G.ny, c, steps, t - tby * G.tiles_x, tby);
 }
 
 constexpr int PLAIN_RY = 4;
@@ -251,6 +282,69 @@ extern "C" int appfis_heat2d(appfis_ctx* ctx, int dtype, void* buf0, void* buf1,
         return tab[i].fn(ctx, buf0, buf1, nx, ny, c, iters, tab[i].ts);
     }
     return invalid("heat2d: unknown variant %d", variant);
+}
+
+template <typename T, int TX, int TY, int TS>
+int run_ensemble(appfis_ctx* ctx, int ngrid, const int* nx, const int* ny, void* const* buf0, void* const* buf1,
+                 double c, int iters)
+{
+    std::vector<HeatGrid<T>> hg((size_t)ngrid);
+    long long tiles = 0;
+    for (int g = 0; g < ngrid; g++) {
+        APPFIS_REQUIRE(nx[g] > 0 && ny[g] > 0, "heat2d ensemble: grid %d is empty", g);
+        APPFIS_REQUIRE((double)(nx[g] + 2) * (ny[g] + 2) < 2147483647.0, "grid %d exceeds the int index range", g);
+        APPFIS_REQUIRE(buf0[g] && buf1[g] && buf0[g] != buf1[g], "heat2d ensemble: grid %d buffers", g);
+        hg[g].b[0] = (T*)buf0[g];
+        hg[g].b[1] = (T*)buf1[g];
+        hg[g].nx = nx[g];
+        hg[g].ny = ny[g];
+        hg[g].tiles_x = (int)ceil_div(nx[g], TX);
+        hg[g].tile0 = (int)tiles;
+        tiles += (long long)hg[g].tiles_x * ceil_div(ny[g], TY);
+    }
+    APPFIS_REQUIRE(tiles < (1LL << 31), "heat2d ensemble: too many tiles");
+    const size_t tab_bytes = (size_t)ngrid * sizeof(HeatGrid<T>);
+    int st = ensure_work(ctx, tab_bytes);
+    if (st) return st;
+    HeatGrid<T>* tab = (HeatGrid<T>*)ctx->work;
+    APPFIS_HIP_CHECK(hipMemcpyAsync(tab, hg.data(), tab_bytes, hipMemcpyHostToDevice, ctx->stream));
+    APPFIS_HIP_CHECK(hipStreamSynchronize(ctx->stream));  // the host table is a local
+    std::vector<int> counts((size_t)iters);
+    const int n = plan_launches(iters, TS, counts.data(), iters);
+    if (n <= 0) return invalid("heat2d: cannot split %d sweeps", iters);
+    // the whole launch chain is captured once and replayed as one graph
+    APPFIS_HIP_CHECK(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+    for (int l = 0; l < n; l++)
+        hipLaunchKernelGGL((k_heat2d_tb_ens<T, TX, TY, TS>), dim3((unsigned)tiles), dim3(256), 0, ctx->stream, tab,
+                           ngrid, l & 1, (T)c, counts[l]);
+    hipGraph_t graph = nullptr;
+    hipError_t e = hipStreamEndCapture(ctx->stream, &graph);
+    if (e != hipSuccess) return hip_fail(e, "heat2d ensemble capture", __FILE__, __LINE__);
+    hipGraphExec_t exec = nullptr;
+    e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+    hipError_t e2 = hipGraphDestroy(graph);
+    if (e != hipSuccess) return hip_fail(e, "heat2d ensemble instantiate", __FILE__, __LINE__);
+    if (e2 != hipSuccess) return hip_fail(e2, "heat2d ensemble graph destroy", __FILE__, __LINE__);
+    e = hipGraphLaunch(exec, ctx->stream);
+    // the table stays in use until the graph ran: wait before the workspace can be reused
+    hipError_t e3 = hipStreamSynchronize(ctx->stream);
+    hipError_t e4 = hipGraphExecDestroy(exec);
+    if (e != hipSuccess) return hip_fail(e, "heat2d ensemble launch", __FILE__, __LINE__);
+    if (e3 != hipSuccess) return hip_fail(e3, "heat2d ensemble sync", __FILE__, __LINE__);
+    if (e4 != hipSuccess) return hip_fail(e4, "heat2d ensemble exec destroy", __FILE__, __LINE__);
+    return APPFIS_OK;
+}
+
+extern "C" int appfis_heat2d_ensemble(appfis_ctx* ctx, int dtype, int ngrid, const int* nx, const int* ny,
+                                      void* const* buf0, void* const* buf1, double c, int iters)
+{
+    APPFIS_REQUIRE(ctx && nx && ny && buf0 && buf1, "NULL argument");
+    APPFIS_REQUIRE(ngrid >= 1, "heat2d ensemble: need at least one grid");
+    APPFIS_REQUIRE(iters >= 0, "iters must be >= 0");
+    if (iters == 0) return APPFIS_OK;
+    if (dtype == APPFIS_F64) return run_ensemble<double, 32, 32, 8>(ctx, ngrid, nx, ny, buf0, buf1, c, iters);
+    if (dtype == APPFIS_F32) return run_ensemble<float, 64, 32, 8>(ctx, ngrid, nx, ny, buf0, buf1, c, iters);
+    return invalid("heat2d ensemble: dtype must be F64 or F32");
 }
 
 extern "C" int appfis_heat2d_candidates(int dtype, char* names, int len)

@@ -119,6 +119,7 @@ def lib():
         "appfis_fdtd3d": ([vp, i, vp, vp, i, i, i, dp, i, i], i),
         "appfis_fdtd3d_candidates": ([i, C.c_char_p, i], i),
         "appfis_heat2d_candidates": ([i, C.c_char_p, i], i),
+        "appfis_heat2d_ensemble": ([vp, i, i, ip, ip, C.POINTER(vp), C.POINTER(vp), d, i], i),
         "appfis_gol_candidates": ([C.c_char_p, i], i),
         "appfis_himeno": ([vp, i, C.POINTER(vp), i, i, i, d, i, i, i, dp], i),
         "appfis_flood_run": ([vp, i, vp, vp, vp, vp, vp, i, i, C.POINTER(FloodParams), i, i, dp, dp], i),
@@ -321,6 +322,15 @@ class Context:
     def heat2d(self, b0: DeviceArray, b1: DeviceArray, nx, ny, c, iters, variant=PLAIN, tsteps=0):
         check(lib().appfis_heat2d(self.handle, b0.dtype, b0.ptr, b1.ptr, nx, ny, c, iters, variant, tsteps),
               "heat2d")
+
+    def heat2d_ensemble(self, grids, c, iters):
+        """grids: list of (buf0, buf1, nx, ny); all of one dtype."""
+        n = len(grids)
+        dt = grids[0][0].dtype
+        b0 = (C.c_void_p * n)(*[g[0].ptr for g in grids])
+        b1 = (C.c_void_p * n)(*[g[1].ptr for g in grids])
+        check(lib().appfis_heat2d_ensemble(self.handle, dt, n, _ints([g[2] for g in grids]),
+                                           _ints([g[3] for g in grids]), b0, b1, c, iters), "heat2d_ensemble")
 
     def gol(self, b0: DeviceArray, b1: DeviceArray, nx, ny, gens, variant=PLAIN, tsteps=0):
         check(lib().appfis_gol(self.handle, b0.ptr, b1.ptr, nx, ny, gens, variant, tsteps), "gol")

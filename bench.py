@@ -8,8 +8,9 @@ device before the timed region (resident in HBM).  Each timed region is brackete
 device synchronisation on both sides and measured with HIP events recorded on the
 library's stream (the stream every kernel of the call is launched on).
 
-The same line carries `suite`: the other configs measured the same way (Heat2D FP64,
-FDTD3d FP32/FP64/FP16/BF16, Himeno FP32/FP64/defect-correction, the flood ensemble in
+The same line carries `suite`: the other configs measured the same way (Heat2D FP64, a
+64-grid Heat2D ensemble, FDTD3d FP32/FP64/FP16/BF16, Himeno FP32/FP64/defect-correction,
+the flood ensemble in
 plain FP64 one scenario at a time and in the batched FP32 form), each with the
 roofline of its dominant kernel; `--headline-only` skips them.
 
@@ -180,6 +181,59 @@ def bench_heat(ctx, hip, n=1024, iters=500):
             r["tuned"] = tname
         out[name] = r
     out["workload"] = f"Heat2D {n}x{n} FP64 5-point, {iters} sweeps (16 B/LUP)"
+    return out
+
+
+def _splitmix_sides(n, seed, lo, hi):
+    """side_s = lo + floor(u_s * (hi - lo + 1)), u_s = (r(seed, s) >> 11) * 2^-53 (splitmix64)."""
+    out = []
+    for s in range(n):
+        z = (seed + (s + 1) * 0x9E3779B97F4A7C15) & ((1 << 64) - 1)
+        z = ((z ^ (z >> 30)) * 0xBF58476D1CE4E5B9) & ((1 << 64) - 1)
+        z = ((z ^ (z >> 27)) * 0x94D049BB133111EB) & ((1 << 64) - 1)
+        z ^= z >> 31
+        out.append(lo + int((z >> 11) * (hi - lo + 1) / 2 ** 53))
+    return out
+
+
+def bench_heat_ensemble(ctx, hip, ngrid=64, iters=500):
+    """64 FP64 grids, sides uniform in [128, 1024] (splitmix64 seed 1006), 500 sweeps:
+    one grid at a time (PLAIN; CHOSEN at depth 8) against the batched ensemble."""
+    sides = _splitmix_sides(ngrid, 1006, 128, 1024)
+    grids = []
+    for k, n in enumerate(sides):
+        a, b = ctx.array((n + 2, n + 2), hip.F64), ctx.array((n + 2, n + 2), hip.F64)
+        grids.append((a, b, n, n))
+    cells = sum(n * n for n in sides)
+
+    def reset():
+        for k, (a, b, n, _) in enumerate(grids):
+            ctx.fill_uniform(a, 1, 3000 + k)
+
+    out = {"workload": f"Heat2D ensemble: {ngrid} FP64 grids, sides in [128,1024] (sum N^2 = {cells}), "
+                       f"{iters} sweeps", "sides": sides}
+    res = {}
+    for name, fn in (("plain_one_at_a_time", lambda: [ctx.heat2d(a, b, n, n, 0.1, iters, hip.PLAIN)
+                                                      for a, b, n, _ in grids]),
+                     ("chosen_one_at_a_time", lambda: [ctx.heat2d(a, b, n, n, 0.1, iters, hip.CHOSEN, 8)
+                                                       for a, b, n, _ in grids]),
+                     ("ensemble", lambda: ctx.heat2d_ensemble(grids, 0.1, iters))):
+        reset()
+        fn()  # warm-up
+        reset()
+        ms, _ = timed(ctx, fn)
+        res[name] = [(a, b)[iters % 2].download().sum() for a, b, _, _ in grids[:2]]
+        r = {"ms": round(ms, 3), "glups": round(cells * iters / (ms * 1e-3) / 1e9, 2)}
+        if name == "ensemble":
+            launches = len(_plan(iters, 8))
+            r["roofline"] = roofline(16.0 * cells * iters / launches, ms / launches,
+                                     "k_heat2d_tb_ens<double, 32, 32, 8>")
+        out[name] = r
+    assert res["ensemble"] == res["plain_one_at_a_time"] == res["chosen_one_at_a_time"]
+    out["ensemble"]["speedup_vs_plain"] = round(out["plain_one_at_a_time"]["ms"] / out["ensemble"]["ms"], 3)
+    for a, b, _, _ in grids:
+        a.free()
+        b.free()
     return out
 
 
@@ -369,7 +423,7 @@ def bench_flood(ctx, hip, steps=2000, nscen=24):
 PROFILE_GROUPS = [
     ("gol_8192x8192", "GameOfLife 8192^2 uint8 (byte kernel, bit-packed kernels, ring exchange)",
      ("k_gol_plain", "k_pack", "k_unpack", "k_step", "k_wrap_axis<unsigned char>")),
-    ("heat2d_1024x1024_f64", "Heat2D 1024^2 FP64", ("k_heat2d",)),
+    ("heat2d_1024x1024_f64", "Heat2D 1024^2 FP64 and the 64-grid FP64 ensemble", ("k_heat2d",)),
     ("fdtd3d_512cubed", "FDTD3d 512^3 r=4 (FP32/FP64/FP16/BF16, every autotuner candidate)", ("k_fdtd3d",)),
     ("himeno_257x257x513", "Himeno 257x257x513 (FP32, FP64, defect correction)",
      ("k_himeno", "k_add_correction", "k_f64_to_f32")),
@@ -458,7 +512,7 @@ def main():
     ap.add_argument("--steps", type=int, default=1000)
     ap.add_argument("--warmup", type=int, default=50)
     ap.add_argument("--headline-only", action="store_true")
-    ap.add_argument("--only", default="", help="comma list of suite entries (heat,fdtd,himeno,flood)")
+    ap.add_argument("--only", default="", help="comma list of suite entries (heat,heat_ensemble,fdtd,himeno,flood)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--quick", action="store_true",
                     help="suite entries at their config sizes but few sweeps (for profiler runs)")
@@ -498,11 +552,11 @@ def main():
         variants["chosen"] = {"error": str(e)}
     suite = {}
     if not args.headline_only:
-        only = set(x for x in args.only.split(",") if x) or {"heat", "fdtd", "himeno", "flood"}
-        quick = {"heat": {"iters": 40}, "fdtd": {"steps": 4}, "himeno": {"iters": 10},
+        only = set(x for x in args.only.split(",") if x) or {"heat", "heat_ensemble", "fdtd", "himeno", "flood"}
+        quick = {"heat": {"iters": 40}, "heat_ensemble": {"iters": 40}, "fdtd": {"steps": 4}, "himeno": {"iters": 10},
                  "flood": {"steps": 40}} if args.quick else {}
-        for name, fn in (("heat", bench_heat), ("fdtd", bench_fdtd), ("himeno", bench_himeno),
-                         ("flood", bench_flood)):
+        for name, fn in (("heat", bench_heat), ("heat_ensemble", bench_heat_ensemble), ("fdtd", bench_fdtd),
+                         ("himeno", bench_himeno), ("flood", bench_flood)):
             if name in only:
                 print(f"[bench] {name}", file=sys.stderr, flush=True)
                 suite[name] = fn(ctx, hip, **quick.get(name, {}))

@@ -158,6 +158,15 @@ int appfis_heat2d(appfis_ctx* ctx, int dtype, void* buf0, void* buf1, int nx, in
  * APPFIS_CANDIDATE0 + i), one name per line; returns their number (< 0: bad dtype). */
 int appfis_heat2d_candidates(int dtype, char* names, int len);
 
+/* Heat2D ensemble: `ngrid` independent grids of any sizes (grid g: interior
+ * nx[g] x ny[g], buffers buf0[g] / buf1[g], padded as above) advance `iters` sweeps
+ * together: one temporal-blocking launch per chunk of sweeps over a tile work-list that
+ * spans every grid, the whole launch chain captured and replayed as one hipGraph.
+ * Values are identical to appfis_heat2d on each grid; grid g ends in its buffer
+ * iters % 2.  Pointer arrays are host arrays.  Waits for completion. */
+int appfis_heat2d_ensemble(appfis_ctx* ctx, int dtype, int ngrid, const int* nx, const int* ny,
+                           void* const* buf0, void* const* buf1, double c, int iters);
+
 /* ---- GameOfLife 2D, uint8, periodic, rule B3/S23 (config form of GameOfLife/src/
  *      game_appfis.hpp:121-135 plus the periodic exchange of Communication.hpp:613-678).
  * bufs as for heat2d (ghost 1, padded (nx+2)*(ny+2)).  The ghost ring of the

@@ -56,6 +56,21 @@ def test_heat2d_every_candidate_bit_exact(ctx, dt):
         assert np.array_equal((d0, d1)[iters % 2].download(), ref), name
 
 
+@pytest.mark.parametrize("dt", [hip.F64, hip.F32])
+@pytest.mark.parametrize("iters", [1, 8, 23, 64])
+def test_heat2d_ensemble_matches_oracle(ctx, dt, iters):
+    # uneven grids advanced together (one launch per chunk over all grids, graph-replayed)
+    sizes = [(1, 5), (37, 29), (130, 67), (300, 250), (64, 64), (2, 1), (513, 31)]
+    grids, refs = [], []
+    for k, (nx, ny) in enumerate(sizes):
+        a, d0, d1 = _heat_pair(ctx, nx, ny, dt, seed=2000 + k)
+        grids.append((d0, d1, nx, ny))
+        refs.append(oracle.heat2d(a.copy(), np.zeros_like(a), nx, ny, 0.1, iters))
+    ctx.heat2d_ensemble(grids, 0.1, iters)
+    for (d0, d1, nx, ny), ref in zip(grids, refs):
+        assert np.array_equal((d0, d1)[iters % 2].download(), ref), (nx, ny)
+
+
 def test_heat2d_full_config(ctx):
     # 1024^2 FP64, 500 sweeps: the whole config against the oracle
     nx = ny = 1024
