@@ -5,10 +5,11 @@
 // Layout: 14 dense arrays of dims (nx, ny, nz) including the 1-cell boundary shell.
 //
 // Kernel shape (2.5D k-march): a block owns full x rows (TX = 256 threads along x) and
-// RY rows per thread, and marches a chunk of ZC planes in k.  p is staged through a
-// 3-plane LDS ring (planes k-1, k, k+1 with their 1-cell x/y halo), so each p value is
-// read from memory about once; the 12 coefficient arrays and wrk1 are pure coalesced
-// streams.  gosa: per-thread FP64 accumulation, wave64 shuffle tree, one LDS stage, one
+// RY rows per thread, and marches a chunk of zc planes in k (zc autotuned per shape:
+// it sets how evenly the blocks fill the chip).  p is staged through an LDS ring of
+// planes (k-1, k, k+1 with their 1-cell x/y halo), so each p value is read from memory
+// about once; the 12 coefficient arrays and wrk1 are pure coalesced streams.  Tiles
+// are ordered XCD-contiguously.  gosa: per-thread FP64 accumulation, wave64 shuffle tree, one LDS stage, one
 // partial per block, and the last block to arrive (atomic ticket) sums the partials in
 // block order into gosa[it] -- deterministic, on the device, no host round trip.
 //
@@ -31,7 +32,6 @@ namespace {
 
 constexpr int TX = 256;  // threads along x (one block spans 256 interior columns)
 constexpr int RY = 2;    // rows per thread
-constexpr int ZC = 32;   // planes per block
 constexpr int LW = TX + 2, LH = RY + 2;
 
 struct Coef {
@@ -48,6 +48,18 @@ struct Coef {
     const void* bnd;
     const void* wrk1;  // PLAIN: wrk1;  inner sweep: r (FP32 residual of the base)
 };
+
+// Block -> (x, y, z) tile of a 1-D launch: tiles x fastest, then y, then z chunks, and
+// every XCD takes a contiguous run, so y-neighbour tiles (which share p rows) share an L2.
+__device__ inline void tile_of(int nx, int ny, int& bx, int& by, int& bz)
+{
+    const int gx = (nx - 2 + TX - 1) / TX, gy = (ny - 2 + RY - 1) / RY;
+    const int t = (int)xcd_tile(blockIdx.x, gridDim.x);
+    bx = t % gx;
+    const int r = t / gx;
+    by = r % gy;
+    bz = r / gy;
+}
 
 __device__ inline double block_sum_last(double v, double* partials, unsigned* ticket, double* out, double* lds)
 {
@@ -88,70 +100,116 @@ __device__ inline double block_sum_last(double v, double* partials, unsigned* ti
 // MODE 0 (PLAIN):  ss = (s0*a3 - p)*bnd with s0 including wrk1; pn = p + omega*ss
 // MODE 1 (inner):  p is the correction q, wrk1 slot holds r;
 //                  ss = r + (s0*a3 - q)*bnd with s0 excluding wrk1; qn = q + omega*ss
+//
+// Software pipeline along k: while plane k is computed, the p values of plane k+2 (for
+// the LDS ring) and the 13 coefficient values of plane k+1 are already in flight in
+// registers.  The ring has 4 slots, so one barrier per plane separates the staging of
+// plane k+1 from its use (the slot it overwrites was last read two planes ago).
 template <typename T, int MODE>
 __global__ __launch_bounds__(TX) void k_himeno(const T* __restrict__ p, T* __restrict__ pn, Coef cf, int nx, int ny,
-                                               int nz, T omega, double* partials, unsigned* ticket, double* gosa_out)
+                                               int nz, int zc, T omega, double* partials, unsigned* ticket,
+                                               double* gosa_out)
 {
-    __shared__ T ring[3][LH][LW];
+    constexpr int NR = 4;                    // ring slots
+    constexpr int PPT = (LW * LH + TX - 1) / TX;  // staged p values per thread
+    constexpr int NC = 13;                   // coefficient streams per point
+    __shared__ T ring[NR][LH][LW];
     __shared__ double red[TX / 64];
     const int tx = threadIdx.x;
-    const int i = blockIdx.x * TX + tx + 1;          // interior x of this thread
-    const int j0 = blockIdx.y * RY + 1;              // first interior row of the block
-    const int k0 = blockIdx.z * ZC + 1;
-    const int k1 = min(k0 + ZC, nz - 1);
-    const size_t sy = (size_t)nx, sz = (size_t)nx * (size_t)ny;
+    int bx, by, bz;
+    tile_of(nx, ny, bx, by, bz);
+    const int i = bx * TX + tx + 1;  // interior x of this thread
+    const int j0 = by * RY + 1;      // first interior row of the block
+    const int k0 = bz * zc + 1;
+    const int k1 = min(k0 + zc, nz - 1);
+    const int sy = nx, sz = nx * ny;         // < 2^31 elements (checked on the host)
     const bool xin = i < nx - 1;
-    const T* a0 = (const T*)cf.a0; const T* a1 = (const T*)cf.a1; const T* a2 = (const T*)cf.a2;
-    const T* a3 = (const T*)cf.a3;
-    const T* b0 = (const T*)cf.b0; const T* b1 = (const T*)cf.b1; const T* b2 = (const T*)cf.b2;
-    const T* c0 = (const T*)cf.c0; const T* c1 = (const T*)cf.c1; const T* c2 = (const T*)cf.c2;
-    const T* bnd = (const T*)cf.bnd; const T* w1 = (const T*)cf.wrk1;
-
-    // stage plane k of p (block rows j0-1 .. j0+RY, columns blockIdx.x*TX .. +TX+1)
-    auto stage = [&](int k, int slot) {
-        const int xb = blockIdx.x * TX;
-        for (int q = tx; q < LW * LH; q += TX) {
-            int ly = q / LW, lx = q - ly * LW;
-            int gx = xb + lx, gy = j0 - 1 + ly;
-            T v = (T)0;
-            if (gx < nx && gy < ny) v = p[(size_t)gx + sy * gy + sz * k];
-            ring[slot][ly][lx] = v;
-        }
+    const T* arr[NC] = {(const T*)cf.a0, (const T*)cf.a1, (const T*)cf.a2, (const T*)cf.a3, (const T*)cf.b0,
+                        (const T*)cf.b1, (const T*)cf.b2, (const T*)cf.c0, (const T*)cf.c1, (const T*)cf.c2,
+                        (const T*)cf.bnd, (const T*)cf.wrk1, p};
+    // staged cells of this thread: in-plane offsets (clamped in bounds) and validity
+    int soff[PPT], slds[PPT];
+    bool sval[PPT];
+    const int xb = bx * TX;
+#pragma unroll
+    for (int q = 0; q < PPT; q++) {
+        const int e = tx + q * TX;
+        const int ly = e / LW, lx = e - ly * LW;
+        const int gx = xb + lx, gy = j0 - 1 + ly;
+        sval[q] = e < LW * LH && gx < nx && gy < ny;
+        soff[q] = sval[q] ? gx + sy * gy : 0;
+        slds[q] = e < LW * LH ? e : 0;
+    }
+    bool rin[RY];
+    int own[RY];
+#pragma unroll
+    for (int r = 0; r < RY; r++) {
+        rin[r] = xin && j0 + r < ny - 1;
+        own[r] = rin[r] ? i + sy * (j0 + r) : 0;
+    }
+    T* ringf = &ring[0][0][0];
+    auto stage_load = [&](int k, T(&v)[PPT]) {
+#pragma unroll
+        for (int q = 0; q < PPT; q++) v[q] = sval[q] ? p[soff[q] + sz * k] : (T)0;
     };
-    stage(k0 - 1, (k0 - 1) % 3);
-    stage(k0, k0 % 3);
+    auto stage_store = [&](int slot, const T(&v)[PPT]) {
+#pragma unroll
+        for (int q = 0; q < PPT; q++)
+            if (tx + q * TX < LW * LH) ringf[slot * (LH * LW) + slds[q]] = v[q];
+    };
+    auto coef_load = [&](int k, T(&c)[RY][NC - 1]) {
+#pragma unroll
+        for (int r = 0; r < RY; r++)
+#pragma unroll
+            for (int a = 0; a < NC - 1; a++) c[r][a] = rin[r] ? arr[a][own[r] + sz * k] : (T)0;
+    };
+    {
+        T v[PPT];
+        stage_load(k0 - 1, v);
+        stage_store((k0 - 1) % NR, v);
+        stage_load(k0, v);
+        stage_store(k0 % NR, v);
+    }
+    T pnext[PPT];
+    stage_load(k0 + 1, pnext);  // k0 + 1 <= nz - 1: always a plane of the array
+    T cc[RY][NC - 1], cn[RY][NC - 1];
+    coef_load(k0, cc);
     double g = 0.0;
     for (int k = k0; k < k1; k++) {
-        stage(k + 1, (k + 1) % 3);
+        stage_store((k + 1) % NR, pnext);
+        if (k + 2 < nz) stage_load(k + 2, pnext);
+        if (k + 1 < k1) coef_load(k + 1, cn);
         __syncthreads();
-        const int sm = (k - 1) % 3, s0s = k % 3, sp = (k + 1) % 3;
+        const int sm = (k - 1) % NR, s0s = k % NR, sp = (k + 1) % NR;
 #pragma unroll
         for (int r = 0; r < RY; r++) {
-            const int jj = j0 + r;
             const int ly = r + 1, lx = tx + 1;
-            if (xin && jj < ny - 1) {
-                const size_t o = (size_t)i + sy * jj + sz * k;
+            if (rin[r]) {
+                const T* c = cc[r];
                 const T pc = ring[s0s][ly][lx];
-                T s0 = a0[o] * ring[s0s][ly][lx + 1] + a1[o] * ring[s0s][ly + 1][lx] + a2[o] * ring[sp][ly][lx]
-                       + b0[o] * (ring[s0s][ly + 1][lx + 1] - ring[s0s][ly - 1][lx + 1] - ring[s0s][ly + 1][lx - 1]
-                                  + ring[s0s][ly - 1][lx - 1])
-                       + b1[o] * (ring[sp][ly + 1][lx] - ring[sp][ly - 1][lx] - ring[sm][ly + 1][lx]
-                                  + ring[sm][ly - 1][lx])
-                       + b2[o] * (ring[sp][ly][lx + 1] - ring[sp][ly][lx - 1] - ring[sm][ly][lx + 1]
-                                  + ring[sm][ly][lx - 1])
-                       + c0[o] * ring[s0s][ly][lx - 1] + c1[o] * ring[s0s][ly - 1][lx] + c2[o] * ring[sm][ly][lx];
+                T s0 = c[0] * ring[s0s][ly][lx + 1] + c[1] * ring[s0s][ly + 1][lx] + c[2] * ring[sp][ly][lx]
+                       + c[4] * (ring[s0s][ly + 1][lx + 1] - ring[s0s][ly - 1][lx + 1] - ring[s0s][ly + 1][lx - 1]
+                                 + ring[s0s][ly - 1][lx - 1])
+                       + c[5] * (ring[sp][ly + 1][lx] - ring[sp][ly - 1][lx] - ring[sm][ly + 1][lx]
+                                 + ring[sm][ly - 1][lx])
+                       + c[6] * (ring[sp][ly][lx + 1] - ring[sp][ly][lx - 1] - ring[sm][ly][lx + 1]
+                                 + ring[sm][ly][lx - 1])
+                       + c[7] * ring[s0s][ly][lx - 1] + c[8] * ring[s0s][ly - 1][lx] + c[9] * ring[sm][ly][lx];
                 T ss;
                 if constexpr (MODE == 0) {
-                    s0 = s0 + w1[o];
-                    ss = (s0 * a3[o] - pc) * bnd[o];
+                    s0 = s0 + c[11];
+                    ss = (s0 * c[3] - pc) * c[10];
                 } else {
-                    ss = w1[o] + (s0 * a3[o] - pc) * bnd[o];
+                    ss = c[11] + (s0 * c[3] - pc) * c[10];
                 }
-                pn[o] = pc + omega * ss;
+                pn[own[r] + sz * k] = pc + omega * ss;
                 g += (double)ss * (double)ss;
             }
         }
-        __syncthreads();
+#pragma unroll
+        for (int r = 0; r < RY; r++)
+#pragma unroll
+            for (int a = 0; a < NC - 1; a++) cc[r][a] = cn[r][a];
     }
     block_sum_last(g, partials, ticket, gosa_out, red);
 }
@@ -160,17 +218,19 @@ __global__ __launch_bounds__(TX) void k_himeno(const T* __restrict__ p, T* __res
 // writes B = P, r = (float)ss, qn = (float)(omega*ss); gosa from the FP64 ss.
 __global__ __launch_bounds__(TX) void k_himeno_rebase(const double* __restrict__ B, const float* __restrict__ q,
                                                       double* __restrict__ Bn, float* __restrict__ r,
-                                                      float* __restrict__ qn, Coef cf, int nx, int ny, int nz,
+                                                      float* __restrict__ qn, Coef cf, int nx, int ny, int nz, int zc,
                                                       double omega, double* partials, unsigned* ticket,
                                                       double* gosa_out)
 {
     __shared__ double ring[3][LH][LW];
     __shared__ double red[TX / 64];
     const int tx = threadIdx.x;
-    const int i = blockIdx.x * TX + tx + 1;
-    const int j0 = blockIdx.y * RY + 1;
-    const int k0 = blockIdx.z * ZC + 1;
-    const int k1 = min(k0 + ZC, nz - 1);
+    int bx, by, bz;
+    tile_of(nx, ny, bx, by, bz);
+    const int i = bx * TX + tx + 1;
+    const int j0 = by * RY + 1;
+    const int k0 = bz * zc + 1;
+    const int k1 = min(k0 + zc, nz - 1);
     const size_t sy = (size_t)nx, sz = (size_t)nx * (size_t)ny;
     const bool xin = i < nx - 1;
     const double* a0 = (const double*)cf.a0; const double* a1 = (const double*)cf.a1;
@@ -182,7 +242,7 @@ __global__ __launch_bounds__(TX) void k_himeno_rebase(const double* __restrict__
     const double* bnd = (const double*)cf.bnd; const double* w1 = (const double*)cf.wrk1;
 
     auto stage = [&](int k, int slot) {
-        const int xb = blockIdx.x * TX;
+        const int xb = bx * TX;
         for (int t = tx; t < LW * LH; t += TX) {
             int ly = t / LW, lx = t - ly * LW;
             int gx = xb + lx, gy = j0 - 1 + ly;
@@ -245,10 +305,16 @@ __global__ void k_add_correction(const double* __restrict__ B, const float* __re
     for (; t < n; t += stride) out[t] = __dadd_rn(B[t], (double)q[t]);
 }
 
-dim3 grid_of(int nx, int ny, int nz)
+// 1-D launch over (x tiles) * (y tiles) * (z chunks of zc planes)
+dim3 grid_of(int nx, int ny, int nz, int zc)
 {
-    return dim3(ceil_div(nx - 2, TX), ceil_div(ny - 2, RY), ceil_div(nz - 2, ZC));
+    return dim3(ceil_div(nx - 2, TX) * ceil_div(ny - 2, RY) * ceil_div(nz - 2, zc));
 }
+
+// candidate z-chunk lengths of the sweep kernels (planes per block)
+constexpr int kZc[] = {8, 16, 32, 64, 128};
+constexpr int NZC = (int)(sizeof(kZc) / sizeof(kZc[0]));
+const char* const kZcNames[NZC] = {"zc8", "zc16", "zc32", "zc64", "zc128"};
 
 }  // namespace
 
@@ -262,8 +328,7 @@ extern "C" int appfis_himeno(appfis_ctx* ctx, int dtype, void* const* arrays, in
     APPFIS_REQUIRE((double)nx * ny * nz < 2147483647.0, "grid exceeds the int index range");
     APPFIS_REQUIRE(iters >= 0, "iters must be >= 0");
     if (iters == 0) return APPFIS_OK;
-    const dim3 grid = grid_of(nx, ny, nz);
-    const size_t nblk = (size_t)grid.x * grid.y * grid.z;
+    const size_t nblk = grid_of(nx, ny, nz, kZc[0]).x;  // the most blocks any candidate launches
     // scratch: ticket | partials[nblk] | gosa[iters]
     const size_t off_part = 256, off_gosa = off_part + ((nblk * 8 + 255) / 256) * 256;
     int st = ensure_scratch(ctx, off_gosa + (size_t)iters * 8 + 256);
@@ -274,20 +339,34 @@ extern "C" int appfis_himeno(appfis_ctx* ctx, int dtype, void* const* arrays, in
     Coef cf{arrays[2], arrays[3], arrays[4], arrays[5], arrays[6], arrays[7], arrays[8],
             arrays[9], arrays[10], arrays[11], arrays[12], arrays[13]};
 
+    // planes per block: timed once per (dtype, form, shape) -- a sweep from p0 into p1
+    // (PLAIN; the first real sweep rewrites p1) or between the correction buffers (CHOSEN)
+    auto tune_zc = [&](int form, const std::function<int(int)>& sweep) -> int {
+        const int key[6] = {5, dtype, nx, ny, nz, form};
+        int pick = 2;
+        const int tst = tune_select(ctx, key, NZC, kZcNames, sweep, 2, &pick);
+        return tst ? -tst : kZc[pick];
+    };
     if (variant == APPFIS_PLAIN) {
-        for (int it = 0; it < iters; it++) {
+        if (dtype != APPFIS_F32 && dtype != APPFIS_F64) return invalid("himeno PLAIN: dtype must be F32 or F64");
+        auto sweep = [&](int zc, int it) -> int {
             void* pin = arrays[it & 1];
             void* pout = arrays[(it + 1) & 1];
-            if (dtype == APPFIS_F32) {
+            const dim3 grid = grid_of(nx, ny, nz, zc);
+            if (dtype == APPFIS_F32)
                 hipLaunchKernelGGL((k_himeno<float, 0>), grid, dim3(TX), 0, ctx->stream, (const float*)pin,
-                                   (float*)pout, cf, nx, ny, nz, (float)omega, partials, ticket, gosa_dev + it);
-            } else if (dtype == APPFIS_F64) {
+                                   (float*)pout, cf, nx, ny, nz, zc, (float)omega, partials, ticket, gosa_dev + it);
+            else
                 hipLaunchKernelGGL((k_himeno<double, 0>), grid, dim3(TX), 0, ctx->stream, (const double*)pin,
-                                   (double*)pout, cf, nx, ny, nz, omega, partials, ticket, gosa_dev + it);
-            } else {
-                return invalid("himeno PLAIN: dtype must be F32 or F64");
-            }
+                                   (double*)pout, cf, nx, ny, nz, zc, omega, partials, ticket, gosa_dev + it);
             APPFIS_LAUNCH_CHECK();
+            return APPFIS_OK;
+        };
+        const int zc = tune_zc(0, [&](int c) { return sweep(kZc[c], 0); });
+        if (zc < 0) return -zc;
+        for (int it = 0; it < iters; it++) {
+            st = sweep(zc, it);
+            if (st) return st;
         }
     } else if (variant == APPFIS_CHOSEN) {
         APPFIS_REQUIRE(dtype == APPFIS_F64, "himeno CHOSEN (defect correction) takes the F64 problem");
@@ -316,15 +395,25 @@ extern "C" int appfis_himeno(appfis_ctx* ctx, int dtype, void* const* arrays, in
         APPFIS_HIP_CHECK(hipMemsetAsync(q[0], 0, n * 4, ctx->stream));
         APPFIS_HIP_CHECK(hipMemsetAsync(q[1], 0, n * 4, ctx->stream));
         Coef c32s{c32[0], c32[1], c32[2], c32[3], c32[4], c32[5], c32[6], c32[7], c32[8], c32[9], c32[10], r32};
+        // z-chunk of the inner FP32 sweep, timed between the correction buffers (q[1] is
+        // rewritten by the first rebase; q[0] stays 0)
+        const int zc = tune_zc(1, [&](int c) {
+            hipLaunchKernelGGL((k_himeno<float, 1>), grid_of(nx, ny, nz, kZc[c]), dim3(TX), 0, ctx->stream, q[0],
+                               q[1], c32s, nx, ny, nz, kZc[c], (float)omega, partials, ticket, gosa_dev);
+            APPFIS_LAUNCH_CHECK();
+            return APPFIS_OK;
+        });
+        if (zc < 0) return -zc;
+        const dim3 grid = grid_of(nx, ny, nz, zc);
         int bi = 0, qi = 0;
         for (int it = 0; it < iters; it++) {
             if (it % K == 0) {
                 hipLaunchKernelGGL(k_himeno_rebase, grid, dim3(TX), 0, ctx->stream, B[bi], q[qi], B[bi ^ 1], r32,
-                                   q[qi ^ 1], cf, nx, ny, nz, omega, partials, ticket, gosa_dev + it);
+                                   q[qi ^ 1], cf, nx, ny, nz, zc, omega, partials, ticket, gosa_dev + it);
                 bi ^= 1;
             } else {
                 hipLaunchKernelGGL((k_himeno<float, 1>), grid, dim3(TX), 0, ctx->stream, q[qi], q[qi ^ 1], c32s, nx,
-                                   ny, nz, (float)omega, partials, ticket, gosa_dev + it);
+                                   ny, nz, zc, (float)omega, partials, ticket, gosa_dev + it);
             }
             APPFIS_LAUNCH_CHECK();
             qi ^= 1;

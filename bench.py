@@ -327,6 +327,7 @@ def bench_himeno(ctx, hip, nx=257, ny=257, nz=513, iters=200):
     for name in ("f32", "dc"):
         out[name]["gosa_max_rel_err_vs_f64"] = float(np.max(np.abs(gosas[name] - ref) / ref))
     out["dc"]["speedup_vs_f64"] = round(out["f64"]["ms"] / out["dc"]["ms"], 3)
+    out["autotune"] = [l for l in ctx.tune_report().splitlines() if l.startswith("autotune kernel 5 ")]
     return out
 
 
