@@ -81,37 +81,44 @@ __global__ __launch_bounds__(256) void k_gol_plain(const uint8_t* __restrict__ u
     const bool valid = c < nchunks;
     const bool full = valid && p0 + 16 <= nbytes;
 
-    // centre chunk (every valid full lane loads it; shared with neighbours below)
+    // rows of the first and last cell of the chunk, and whether the fast path applies
+    // (all 16 cells in interior rows and every load in bounds) -- before any load, so
+    // that all loads of the lane are in flight together
+    const int y0 = (unsigned)p0 / (unsigned)W;
+    const int y1 = (unsigned)(p0 + 15) / (unsigned)W;
+    const int below_end = ((p0 + W) & ~15) + 32;
+    const bool fast = full && y0 >= 1 && y1 <= H - 2 && p0 - W >= 16 && below_end <= nbytes;
+    // centre chunk (every valid full lane loads it; shared with neighbours below) and,
+    // on the fast path, the two aligned 32-byte spans holding the rows above and below
     uint32_t m[4] = {0u, 0u, 0u, 0u};
+    uint32_t a8[8], b8[8];
+    uint32_t mlb = 0u, mrb = 0u;
     if (full) {
         uint4 q = *(const uint4*)(u + p0);
         m[0] = q.x; m[1] = q.y; m[2] = q.z; m[3] = q.w;
+    }
+    if (fast) {
+        const int pa = (p0 - W) & ~15;
+        const uint4 qa0 = *(const uint4*)(u + pa);
+        const uint4 qa1 = *(const uint4*)(u + pa + 16);
+        a8[0] = qa0.x; a8[1] = qa0.y; a8[2] = qa0.z; a8[3] = qa0.w;
+        a8[4] = qa1.x; a8[5] = qa1.y; a8[6] = qa1.z; a8[7] = qa1.w;
+        const int pb = (p0 + W) & ~15;
+        const uint4 qb0 = *(const uint4*)(u + pb);
+        const uint4 qb1 = *(const uint4*)(u + pb + 16);
+        b8[0] = qb0.x; b8[1] = qb0.y; b8[2] = qb0.z; b8[3] = qb0.w;
+        b8[4] = qb1.x; b8[5] = qb1.y; b8[6] = qb1.z; b8[7] = qb1.w;
+        // the centre row's outer neighbours for lanes whose adjacent lane does not hold them
+        if (lane == 0 || c == 0) mlb = u[p0 - 1];
+        if (lane == 63 || c + 1 >= nchunks || p0 + 32 > nbytes) mrb = u[p0 + 16];
     }
     // left / right neighbour bytes of the centre row from the adjacent lanes
     uint32_t prev3 = __shfl_up(m[3], 1, 64);
     uint32_t next0 = __shfl_down(m[0], 1, 64);
 
-    const int y0 = (unsigned)p0 / (unsigned)W;
-    const int y1 = (unsigned)(p0 + 15) / (unsigned)W;
-    // fast path: all 16 cells in interior rows and every load in bounds
-    const int below_end = ((p0 + W) & ~15) + 32;
-    bool fast = full && y0 >= 1 && y1 <= H - 2 && p0 - W >= 16 && below_end <= nbytes;
     if (fast) {
-        uint32_t ml = (lane == 0 || c == 0) ? (uint32_t)u[p0 - 1] : (prev3 >> 24);
-        uint32_t mr = (lane == 63 || c + 1 >= nchunks || p0 + 32 > nbytes) ? (uint32_t)u[p0 + 16] : (next0 & 0xFFu);
-        uint32_t a8[8], b8[8];
-        {
-            const int pa = (p0 - W) & ~15;
-            const uint4 qa0 = *(const uint4*)(u + pa);
-            const uint4 qa1 = *(const uint4*)(u + pa + 16);
-            a8[0] = qa0.x; a8[1] = qa0.y; a8[2] = qa0.z; a8[3] = qa0.w;
-            a8[4] = qa1.x; a8[5] = qa1.y; a8[6] = qa1.z; a8[7] = qa1.w;
-            const int pb = (p0 + W) & ~15;
-            const uint4 qb0 = *(const uint4*)(u + pb);
-            const uint4 qb1 = *(const uint4*)(u + pb + 16);
-            b8[0] = qb0.x; b8[1] = qb0.y; b8[2] = qb0.z; b8[3] = qb0.w;
-            b8[4] = qb1.x; b8[5] = qb1.y; b8[6] = qb1.z; b8[7] = qb1.w;
-        }
+        uint32_t ml = (lane == 0 || c == 0) ? mlb : (prev3 >> 24);
+        uint32_t mr = (lane == 63 || c + 1 >= nchunks || p0 + 32 > nbytes) ? mrb : (next0 & 0xFFu);
         uint32_t up[4], dn[4], ul, ur, dl, dr;
         if constexpr (DA == 0) {
             // W multiple of 16: the window is the aligned chunk; edge bytes come from the
