@@ -22,10 +22,17 @@ using namespace appfis;
 
 namespace golbits {
 
+// 3-input bitwise functions, one v_bitop3_b32 each (gfx950).  Both are symmetric in
+// their inputs, so the lookup table does not depend on the operand order.  Written as
+// separate instructions on purpose: sharing a ^ b between the two lets the compiler
+// fall back to two-input operations.
+__device__ __forceinline__ uint32_t xor3(uint32_t a, uint32_t b, uint32_t c)
+{
+    return __builtin_amdgcn_bitop3_b32(a, b, c, 0x96);  // a ^ b ^ c
+}
 __device__ __forceinline__ uint32_t maj(uint32_t a, uint32_t b, uint32_t c)
 {
-    const uint32_t m = a ^ b;
-    return (m & c) | (~m & a);  // a == b ? a : c
+    return __builtin_amdgcn_bitop3_b32(a, b, c, 0xE8);  // at least two of a, b, c
 }
 
 // cells x = 32w .. 32w+31 of interior row y (uint8, 0/1) -> one word
@@ -83,7 +90,7 @@ __global__ __launch_bounds__(256) void k_step(const uint32_t* __restrict__ P, ui
         const uint32_t c = row[w], l = row[wl], rr = row[wr];
         const uint32_t W = __builtin_amdgcn_alignbit(c, l, 31);  // cell x-1 at bit x
         const uint32_t E = __builtin_amdgcn_alignbit(rr, c, 1);  // cell x+1 at bit x
-        h0[r] = W ^ c ^ E;
+        h0[r] = xor3(W, c, E);
         h1[r] = maj(W, c, E);
         cen[r] = c;
     }
@@ -92,9 +99,9 @@ __global__ __launch_bounds__(256) void k_step(const uint32_t* __restrict__ P, ui
         const int y = y0 + k;
         if (y >= ny) break;
         // S = h[k] + h[k+1] + h[k+2]: bits s0 (1), s1 (2), s2 (4); the 8s bit is implied
-        const uint32_t s0 = h0[k] ^ h0[k + 1] ^ h0[k + 2];
+        const uint32_t s0 = xor3(h0[k], h0[k + 1], h0[k + 2]);
         const uint32_t c0 = maj(h0[k], h0[k + 1], h0[k + 2]);
-        const uint32_t u1 = h1[k] ^ h1[k + 1] ^ h1[k + 2];
+        const uint32_t u1 = xor3(h1[k], h1[k + 1], h1[k + 2]);
         const uint32_t c1 = maj(h1[k], h1[k + 1], h1[k + 2]);
         const uint32_t s1 = u1 ^ c0;
         const uint32_t s2 = c1 ^ (u1 & c0);
@@ -167,7 +174,7 @@ __global__ __launch_bounds__(64 * NWAVES) void k_step_tb(const uint32_t* __restr
             const uint32_t c = R[k];
             const uint32_t Wc = __builtin_amdgcn_alignbit(c, l, 31);
             const uint32_t Ec = __builtin_amdgcn_alignbit(r, c, 1);
-            h0[k] = Wc ^ c ^ Ec;
+            h0[k] = xor3(Wc, c, Ec);
             h1[k] = maj(Wc, c, Ec);
         }
         sum_top[0][wv][lane] = h0[0];
@@ -188,9 +195,9 @@ __global__ __launch_bounds__(64 * NWAVES) void k_step_tb(const uint32_t* __restr
         for (int k = 0; k < K; k++) {
             const uint32_t u0 = k == 0 ? a0 : h0[k - 1], u1 = k == 0 ? a1 : h1[k - 1];
             const uint32_t d0 = k == K - 1 ? z0 : h0[k + 1], d1 = k == K - 1 ? z1 : h1[k + 1];
-            const uint32_t s0 = u0 ^ h0[k] ^ d0;
+            const uint32_t s0 = xor3(u0, h0[k], d0);
             const uint32_t c0 = maj(u0, h0[k], d0);
-            const uint32_t v1 = u1 ^ h1[k] ^ d1;
+            const uint32_t v1 = xor3(u1, h1[k], d1);
             const uint32_t c1 = maj(u1, h1[k], d1);
             const uint32_t s1 = v1 ^ c0;
             const uint32_t s2 = c1 ^ (v1 & c0);
