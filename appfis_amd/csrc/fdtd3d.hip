@@ -143,9 +143,11 @@ __global__ __launch_bounds__(TXT* TY) void k_fdtd3d(const S* __restrict__ u, S* 
             lx = p - hy * PX + R;
             ly = hy < R ? hy : PY + hy;
         } else if (huse[s]) {
+            // side halos row by row: 8 consecutive slots are the R cells left and the R
+            // cells right of one row, so a wave touches few cache lines
             const int p2 = p - 2 * R * PX;
-            const int hx = p2 / PY;
-            ly = p2 - hx * PY + R;
+            const int hy = p2 / (2 * R), hx = p2 - hy * (2 * R);
+            ly = hy + R;
             lx = hx < R ? hx : PX + hx;
         }
         const int xg = bx - R + lx, yg = by - R + ly;
@@ -316,6 +318,7 @@ int run_v(appfis_ctx* ctx, void* b0, void* b1, int nx, int ny, int nz, const dou
 // planes per block, D-deep prefetch, U-step unroll: 18 rotates the queue).  Index 0 of
 // each table is the PLAIN default: the fastest measured tiling that fits any shape.
 static const FdtdVariant kF32[] = {
+    {"vx1 ty16 ry2 zc256 d4 u4", run_v<float, 1, 16, 2, 256, 4, 4>, false},
     {"vx1 ty8 ry4 zc256 d2 u2", run_v<float, 1, 8, 4, 256, 2, 2>, false},
     {"vx1 ty4 ry4 zc64 d2 u2", run_v<float, 1, 4, 4, 64, 2, 2>, false},
     {"vx1 ty8 ry4 zc64 d2 u2", run_v<float, 1, 8, 4, 64, 2, 2>, false},
@@ -328,8 +331,12 @@ static const FdtdVariant kF32[] = {
     {"vx1 ty16 ry2 zc64 d2 u2", run_v<float, 1, 16, 2, 64, 2, 2>, false},
     {"vx1 ty8 ry4 zc128 d3 u6", run_v<float, 1, 8, 4, 128, 3, 6>, false},
     {"vx1 ty16 ry2 zc256 d2 u2", run_v<float, 1, 16, 2, 256, 2, 2>, false},
+    {"vx1 ty8 ry4 zc256 d4 u4", run_v<float, 1, 8, 4, 256, 4, 4>, false},
+    {"vx1 ty8 ry4 zc256 d6 u6", run_v<float, 1, 8, 4, 256, 6, 6>, false},
+    {"vx1 ty8 ry2 zc256 d4 u4", run_v<float, 1, 8, 2, 256, 4, 4>, false},
 };
 static const FdtdVariant kF64[] = {
+    {"vx1 ty8 ry2 zc256 d4 u4", run_v<double, 1, 8, 2, 256, 4, 4>, false},
     {"vx1 ty8 ry2 zc64 d2 u2", run_v<double, 1, 8, 2, 64, 2, 2>, false},
     {"vx1 ty4 ry2 zc64 d2 u2", run_v<double, 1, 4, 2, 64, 2, 2>, false},
     {"vx1 ty4 ry4 zc64 d2 u2", run_v<double, 1, 4, 4, 64, 2, 2>, false},
@@ -339,6 +346,8 @@ static const FdtdVariant kF64[] = {
     {"vx1 ty8 ry2 zc256 d2 u2", run_v<double, 1, 8, 2, 256, 2, 2>, false},
     {"vx1 ty4 ry4 zc256 d2 u2", run_v<double, 1, 4, 4, 256, 2, 2>, false},
     {"vx1 ty8 ry2 zc128 d2 u18", run_v<double, 1, 8, 2, 128, 2, 18>, false},
+    {"vx1 ty4 ry4 zc256 d4 u4", run_v<double, 1, 4, 4, 256, 4, 4>, false},
+    {"vx1 ty16 ry1 zc256 d4 u4", run_v<double, 1, 16, 1, 256, 4, 4>, false},
 };
 #define FDTD16_VARIANTS(S)                                                 \
     {"vx1 ty4 ry2 zc64 d2 u18", run_v<S, 1, 4, 2, 64, 2, 18>, false},      \

@@ -267,7 +267,7 @@ def _plan(iters, ts):
     return [0] * min(n, iters)
 
 
-FDTD_PLAIN = {"f32": "vx1 ty8 ry4 zc256 d2 u2", "f64": "vx1 ty8 ry2 zc64 d2 u2", "f16": "vx1 ty4 ry2 zc64 d2 u18",
+FDTD_PLAIN = {"f32": "vx1 ty16 ry2 zc256 d4 u4", "f64": "vx1 ty8 ry2 zc256 d4 u4", "f16": "vx1 ty4 ry2 zc64 d2 u18",
               "bf16": "vx1 ty4 ry2 zc64 d2 u18"}
 FDTD_CTYPE = {"f32": "float", "f64": "double", "f16": "__half", "bf16": "__hip_bfloat16"}
 
