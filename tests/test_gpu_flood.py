@@ -85,6 +85,23 @@ def test_flood_graph_form_matches_direct_launches(ctx):
     assert np.array_equal(ta, tb) and np.array_equal(va, vb)
 
 
+def test_flood_graph_replays_repeatedly(ctx):
+    # 5 replays of the captured 20-step chunk per call, three calls in a row with eager
+    # work and synchronisation between them: every per-step record of every call must
+    # equal the direct-launch run
+    arrs = scenario(64, 3211, wet=0.2)
+    ref, tr, vr = gpu_run(ctx, arrs, 100, hip.PLAIN)
+    for call in range(3):
+        scratch = ctx.array((1000,), hip.F64)
+        ctx.fill_value(scratch, float(call))
+        ctx.sync()
+        got, tg, vg = gpu_run(ctx, arrs, 100, hip.CHOSEN)
+        for x, y in zip(ref, got):
+            assert np.array_equal(x, y), call
+        assert np.array_equal(tr, tg) and np.array_equal(vr, vg), call
+        scratch.free()
+
+
 def test_flood_print_interval_clamp(ctx):
     arrs = scenario(40, 3301, wet=0.3)
     arrs[4][:] = 0.0

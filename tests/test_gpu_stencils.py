@@ -71,6 +71,30 @@ def test_heat2d_ensemble_matches_oracle(ctx, dt, iters):
         assert np.array_equal((d0, d1)[iters % 2].download(), ref), (nx, ny)
 
 
+def test_heat2d_ensemble_repeated_calls(ctx):
+    # each call captures and replays its launch chain as a graph; three calls in a row on
+    # the same grids (eager reductions between them) continue the same trajectory
+    sizes = [(40, 33), (129, 70), (7, 200)]
+    grids, arrs = [], []
+    for k, (nx, ny) in enumerate(sizes):
+        a, d0, d1 = _heat_pair(ctx, nx, ny, hip.F64, seed=2100 + k)
+        grids.append((d0, d1, nx, ny))
+        arrs.append(a)
+    done, prev = 0, 0
+    for call, iters in enumerate((9, 16, 7)):
+        if prev % 2:  # the last call left the field in buffer 1: bring it to buffer 0
+            for d0, d1, _, _ in grids:
+                d0.upload(d1.download())
+        ctx.heat2d_ensemble(grids, 0.1, iters)
+        done += iters
+        prev = iters
+        for (d0, d1, nx, ny), a in zip(grids, arrs):
+            ref = oracle.heat2d(a.copy(), np.zeros_like(a), nx, ny, 0.1, done)
+            got = (d0, d1)[iters % 2].download()
+            assert np.array_equal(got, ref), (call, nx, ny)
+            ctx.reduce_sum_wide(d0, [1, nx + 1, 1, ny + 1])
+
+
 def test_heat2d_full_config(ctx):
     # 1024^2 FP64, 500 sweeps: the whole config against the oracle
     nx = ny = 1024
