@@ -1,0 +1,64 @@
+"""Host logic of bench.py (no GPU): the seeded workload sizes, the roofline arithmetic,
+the kernel-name normalisation of the profile summaries, and the one-GPU refusal."""
+import json
+import os
+import subprocess
+import sys
+
+import pytest
+
+import oracle
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+sys.path.insert(0, ROOT)
+import bench  # noqa: E402
+
+
+def test_flood_sizes_follow_the_oracle_generator():
+    # N_s = 512 + floor(u_s * 3585), u_s = (r(1005, s) >> 11) * 2^-53
+    sizes = bench.flood_sizes(24, 1005)
+    assert len(sizes) == 24 and all(512 <= n <= 4096 for n in sizes)
+    for s, n in enumerate(sizes):
+        u = (oracle.rng_u64(1005, s) >> 11) * 2.0 ** -53
+        assert n == 512 + int(u * 3585)
+    assert bench._splitmix_sides(24, 1005, 512, 4096) == sizes
+
+
+def test_heat_ensemble_sides_in_range_and_seeded():
+    a = bench._splitmix_sides(64, 1006, 128, 1024)
+    assert a == bench._splitmix_sides(64, 1006, 128, 1024)
+    assert min(a) >= 128 and max(a) <= 1024 and len(set(a)) > 50
+
+
+def test_roofline_arithmetic():
+    r = bench.roofline(8e9, 2.0, "k")  # 8 GB in 2 ms = 4000 GB/s
+    assert r["achieved"] == 4000.0 and r["frac"] == 0.5 and r["peak"] == 8000.0
+    assert r["unit"] == "GB/s" and r["bound"] == "hbm" and r["kernel"] == "k"
+
+
+def test_kernel_name_normalisation():
+    s = bench._short
+    assert s("void (anonymous namespace)::k_fdtd3d<float, 1, 8, 4, 256, 2, 2>(float const*, float*, int)") == \
+        "k_fdtd3d<float, 1, 8, 4, 256, 2, 2>"
+    assert s("void gol::k_gol_plain<2, 16>(unsigned char const*, unsigned char*, int, int, int, int)") == \
+        "k_gol_plain<2, 16>"
+    assert s("void golbits::k_step_tb<10, 4, 16>(unsigned int const*, unsigned int*, int)") == "k_step_tb<10, 4, 16>"
+    assert s("(anonymous namespace)::k_himeno_rebase(double const*, float const*)") == "k_himeno_rebase"
+    assert s("k_copy16(uint4 const*, uint4*, unsigned long)") == "k_copy16"
+
+
+def test_traffic_lookup_weights(tmp_path, monkeypatch):
+    prof = tmp_path / "profiles"
+    prof.mkdir()
+    (prof / "traffic.json").write_text(json.dumps({"a": {"bytes_per_launch": 100.0},
+                                                   "b": {"bytes_per_launch": 10.0}}))
+    monkeypatch.setattr(bench, "ROOT", str(tmp_path))
+    assert bench._traffic({"a": 1.0}) == 100.0
+    assert bench._traffic({"a": 0.5, "b": 2.0}) == 70.0
+    assert bench._traffic({"missing": 1.0}) is None
+
+
+def test_more_than_one_gpu_is_refused():
+    r = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--gpus", "2"], capture_output=True,
+                       text=True, timeout=120)
+    assert r.returncode == 2 and "one GPU" in r.stderr
