@@ -263,6 +263,16 @@ __global__ __launch_bounds__(TXT* TY) void k_fdtd3d(const S* __restrict__ u, S* 
 #pragma unroll
             for (int r = 0; r < RY; r++) {
                 const int ly = ty * RY + r + R;
+                // x-window of this row, positions -R .. VX+R-1 around the lane's points: the
+                // lane's own cells from its registers, the 2R outside cells from LDS
+                C xw[VX + 2 * R];
+#pragma unroll
+                for (int j = 0; j < R; j++) {
+                    xw[j] = buf[ly * LX + tx * VX + j];
+                    xw[R + VX + j] = buf[ly * LX + tx * VX + R + VX + j];
+                }
+#pragma unroll
+                for (int e = 0; e < VX; e++) xw[R + e] = q[r][e][QS(R)];
                 C out[VX];
 #pragma unroll
                 for (int e = 0; e < VX; e++) {
@@ -270,7 +280,7 @@ __global__ __launch_bounds__(TXT* TY) void k_fdtd3d(const S* __restrict__ u, S* 
                     C acc = cc[0] * q[r][e][QS(R)];
 #pragma unroll
                     for (int k = 1; k <= R; k++) {
-                        const C sx = buf[ly * LX + lx - k] + buf[ly * LX + lx + k];
+                        const C sx = xw[R + e - k] + xw[R + e + k];
                         const C ym = (r - k >= 0) ? q[r - k][e][QS(R)] : buf[(ly - k) * LX + lx];
                         const C yp = (r + k < RY) ? q[r + k][e][QS(R)] : buf[(ly + k) * LX + lx];
                         const C sY = ym + yp;
@@ -305,7 +315,8 @@ typedef int (*fdtd_fn)(appfis_ctx*, void*, void*, int, int, int, const double*, 
 struct FdtdVariant {
     const char* name;
     fdtd_fn fn;
-    bool pairs;  // needs the 32-bit pair path (even nx, 4-byte aligned buffers)
+    bool pairs;  // 16-bit storage moved in 32-bit pairs: needs 4-byte aligned buffers
+    int vx;      // x points per thread: needs nx % vx == 0 (a lane stores all vx or none)
 };
 
 template <typename S, int VX, int TY, int RY, int ZC, int D, int U>
@@ -318,47 +329,52 @@ int run_v(appfis_ctx* ctx, void* b0, void* b1, int nx, int ny, int nz, const dou
 // planes per block, D-deep prefetch, U-step unroll: 18 rotates the queue).  Index 0 of
 // each table is the PLAIN default: the fastest measured tiling that fits any shape.
 static const FdtdVariant kF32[] = {
-    {"vx1 ty16 ry2 zc256 d4 u4", run_v<float, 1, 16, 2, 256, 4, 4>, false},
-    {"vx1 ty8 ry4 zc256 d2 u2", run_v<float, 1, 8, 4, 256, 2, 2>, false},
-    {"vx1 ty4 ry4 zc64 d2 u2", run_v<float, 1, 4, 4, 64, 2, 2>, false},
-    {"vx1 ty8 ry4 zc64 d2 u2", run_v<float, 1, 8, 4, 64, 2, 2>, false},
-    {"vx1 ty4 ry4 zc64 d2 u18", run_v<float, 1, 4, 4, 64, 2, 18>, false},
-    {"vx1 ty4 ry2 zc64 d2 u18", run_v<float, 1, 4, 2, 64, 2, 18>, false},
-    {"vx1 ty8 ry2 zc64 d2 u18", run_v<float, 1, 8, 2, 64, 2, 18>, false},
-    {"vx1 ty4 ry4 zc32 d2 u2", run_v<float, 1, 4, 4, 32, 2, 2>, false},
-    {"vx1 ty8 ry4 zc128 d2 u2", run_v<float, 1, 8, 4, 128, 2, 2>, false},
-    {"vx1 ty4 ry4 zc64 d3 u6", run_v<float, 1, 4, 4, 64, 3, 6>, false},
-    {"vx1 ty16 ry2 zc64 d2 u2", run_v<float, 1, 16, 2, 64, 2, 2>, false},
-    {"vx1 ty8 ry4 zc128 d3 u6", run_v<float, 1, 8, 4, 128, 3, 6>, false},
-    {"vx1 ty16 ry2 zc256 d2 u2", run_v<float, 1, 16, 2, 256, 2, 2>, false},
-    {"vx1 ty8 ry4 zc256 d4 u4", run_v<float, 1, 8, 4, 256, 4, 4>, false},
-    {"vx1 ty8 ry4 zc256 d6 u6", run_v<float, 1, 8, 4, 256, 6, 6>, false},
-    {"vx1 ty8 ry2 zc256 d4 u4", run_v<float, 1, 8, 2, 256, 4, 4>, false},
+    {"vx1 ty16 ry2 zc256 d4 u4", run_v<float, 1, 16, 2, 256, 4, 4>, false, 1},
+    {"vx1 ty8 ry4 zc256 d2 u2", run_v<float, 1, 8, 4, 256, 2, 2>, false, 1},
+    {"vx1 ty4 ry4 zc64 d2 u2", run_v<float, 1, 4, 4, 64, 2, 2>, false, 1},
+    {"vx1 ty8 ry4 zc64 d2 u2", run_v<float, 1, 8, 4, 64, 2, 2>, false, 1},
+    {"vx1 ty4 ry4 zc64 d2 u18", run_v<float, 1, 4, 4, 64, 2, 18>, false, 1},
+    {"vx1 ty4 ry2 zc64 d2 u18", run_v<float, 1, 4, 2, 64, 2, 18>, false, 1},
+    {"vx1 ty8 ry2 zc64 d2 u18", run_v<float, 1, 8, 2, 64, 2, 18>, false, 1},
+    {"vx1 ty4 ry4 zc32 d2 u2", run_v<float, 1, 4, 4, 32, 2, 2>, false, 1},
+    {"vx1 ty8 ry4 zc128 d2 u2", run_v<float, 1, 8, 4, 128, 2, 2>, false, 1},
+    {"vx1 ty4 ry4 zc64 d3 u6", run_v<float, 1, 4, 4, 64, 3, 6>, false, 1},
+    {"vx1 ty16 ry2 zc64 d2 u2", run_v<float, 1, 16, 2, 64, 2, 2>, false, 1},
+    {"vx1 ty8 ry4 zc128 d3 u6", run_v<float, 1, 8, 4, 128, 3, 6>, false, 1},
+    {"vx1 ty16 ry2 zc256 d2 u2", run_v<float, 1, 16, 2, 256, 2, 2>, false, 1},
+    {"vx1 ty8 ry4 zc256 d4 u4", run_v<float, 1, 8, 4, 256, 4, 4>, false, 1},
+    {"vx1 ty8 ry4 zc256 d6 u6", run_v<float, 1, 8, 4, 256, 6, 6>, false, 1},
+    {"vx1 ty8 ry2 zc256 d4 u4", run_v<float, 1, 8, 2, 256, 4, 4>, false, 1},
+    {"vx2 ty8 ry2 zc256 d4 u4", run_v<float, 2, 8, 2, 256, 4, 4>, false, 2},
+    {"vx2 ty16 ry1 zc256 d4 u4", run_v<float, 2, 16, 1, 256, 4, 4>, false, 2},
+    {"vx2 ty16 ry1 zc256 d6 u6", run_v<float, 2, 16, 1, 256, 6, 6>, false, 2},
 };
 static const FdtdVariant kF64[] = {
-    {"vx1 ty8 ry2 zc256 d4 u4", run_v<double, 1, 8, 2, 256, 4, 4>, false},
-    {"vx1 ty8 ry2 zc64 d2 u2", run_v<double, 1, 8, 2, 64, 2, 2>, false},
-    {"vx1 ty4 ry2 zc64 d2 u2", run_v<double, 1, 4, 2, 64, 2, 2>, false},
-    {"vx1 ty4 ry4 zc64 d2 u2", run_v<double, 1, 4, 4, 64, 2, 2>, false},
-    {"vx1 ty4 ry2 zc128 d2 u2", run_v<double, 1, 4, 2, 128, 2, 2>, false},
-    {"vx1 ty4 ry2 zc64 d2 u18", run_v<double, 1, 4, 2, 64, 2, 18>, false},
-    {"vx1 ty8 ry2 zc64 d2 u18", run_v<double, 1, 8, 2, 64, 2, 18>, false},
-    {"vx1 ty8 ry2 zc256 d2 u2", run_v<double, 1, 8, 2, 256, 2, 2>, false},
-    {"vx1 ty4 ry4 zc256 d2 u2", run_v<double, 1, 4, 4, 256, 2, 2>, false},
-    {"vx1 ty8 ry2 zc128 d2 u18", run_v<double, 1, 8, 2, 128, 2, 18>, false},
-    {"vx1 ty4 ry4 zc256 d4 u4", run_v<double, 1, 4, 4, 256, 4, 4>, false},
-    {"vx1 ty16 ry1 zc256 d4 u4", run_v<double, 1, 16, 1, 256, 4, 4>, false},
+    {"vx1 ty8 ry2 zc256 d4 u4", run_v<double, 1, 8, 2, 256, 4, 4>, false, 1},
+    {"vx1 ty8 ry2 zc64 d2 u2", run_v<double, 1, 8, 2, 64, 2, 2>, false, 1},
+    {"vx1 ty4 ry2 zc64 d2 u2", run_v<double, 1, 4, 2, 64, 2, 2>, false, 1},
+    {"vx1 ty4 ry4 zc64 d2 u2", run_v<double, 1, 4, 4, 64, 2, 2>, false, 1},
+    {"vx1 ty4 ry2 zc128 d2 u2", run_v<double, 1, 4, 2, 128, 2, 2>, false, 1},
+    {"vx1 ty4 ry2 zc64 d2 u18", run_v<double, 1, 4, 2, 64, 2, 18>, false, 1},
+    {"vx1 ty8 ry2 zc64 d2 u18", run_v<double, 1, 8, 2, 64, 2, 18>, false, 1},
+    {"vx1 ty8 ry2 zc256 d2 u2", run_v<double, 1, 8, 2, 256, 2, 2>, false, 1},
+    {"vx1 ty4 ry4 zc256 d2 u2", run_v<double, 1, 4, 4, 256, 2, 2>, false, 1},
+    {"vx1 ty8 ry2 zc128 d2 u18", run_v<double, 1, 8, 2, 128, 2, 18>, false, 1},
+    {"vx1 ty4 ry4 zc256 d4 u4", run_v<double, 1, 4, 4, 256, 4, 4>, false, 1},
+    {"vx2 ty8 ry2 zc256 d2 u2", run_v<double, 2, 8, 2, 256, 2, 2>, false, 2},
+    {"vx2 ty16 ry1 zc256 d2 u2", run_v<double, 2, 16, 1, 256, 2, 2>, false, 2},
+    {"vx1 ty16 ry1 zc256 d4 u4", run_v<double, 1, 16, 1, 256, 4, 4>, false, 1},
 };
 #define FDTD16_VARIANTS(S)                                                 \
-    {"vx1 ty4 ry2 zc64 d2 u18", run_v<S, 1, 4, 2, 64, 2, 18>, false},      \
-    {"vx1 ty4 ry4 zc64 d2 u18", run_v<S, 1, 4, 4, 64, 2, 18>, false},      \
-    {"vx1 ty4 ry4 zc64 d2 u2", run_v<S, 1, 4, 4, 64, 2, 2>, false},        \
-    {"vx1 ty8 ry4 zc64 d2 u2", run_v<S, 1, 8, 4, 64, 2, 2>, false},        \
-    {"vx2 ty4 ry4 zc64 d2 u18", run_v<S, 2, 4, 4, 64, 2, 18>, true},       \
-    {"vx2 ty4 ry2 zc64 d2 u18", run_v<S, 2, 4, 2, 64, 2, 18>, true},       \
-    {"vx2 ty8 ry2 zc64 d2 u18", run_v<S, 2, 8, 2, 64, 2, 18>, true},       \
-    {"vx2 ty4 ry4 zc64 d2 u2", run_v<S, 2, 4, 4, 64, 2, 2>, true},        \
-    {"vx2 ty8 ry2 zc128 d2 u18", run_v<S, 2, 8, 2, 128, 2, 18>, true}
+    {"vx1 ty4 ry2 zc64 d2 u18", run_v<S, 1, 4, 2, 64, 2, 18>, false, 1},      \
+    {"vx1 ty4 ry4 zc64 d2 u18", run_v<S, 1, 4, 4, 64, 2, 18>, false, 1},      \
+    {"vx1 ty4 ry4 zc64 d2 u2", run_v<S, 1, 4, 4, 64, 2, 2>, false, 1},        \
+    {"vx1 ty8 ry4 zc64 d2 u2", run_v<S, 1, 8, 4, 64, 2, 2>, false, 1},        \
+    {"vx2 ty4 ry4 zc64 d2 u18", run_v<S, 2, 4, 4, 64, 2, 18>, true, 2},       \
+    {"vx2 ty4 ry2 zc64 d2 u18", run_v<S, 2, 4, 2, 64, 2, 18>, true, 2},       \
+    {"vx2 ty8 ry2 zc64 d2 u18", run_v<S, 2, 8, 2, 64, 2, 18>, true, 2},       \
+    {"vx2 ty4 ry4 zc64 d2 u2", run_v<S, 2, 4, 4, 64, 2, 2>, true, 2},        \
+    {"vx2 ty8 ry2 zc128 d2 u18", run_v<S, 2, 8, 2, 128, 2, 18>, true, 2}
 static const FdtdVariant kF16[] = {FDTD16_VARIANTS(__half)};
 static const FdtdVariant kBF16[] = {FDTD16_VARIANTS(__hip_bfloat16)};
 
@@ -399,11 +415,13 @@ extern "C" int appfis_fdtd3d(appfis_ctx* ctx, int dtype, void* buf0, void* buf1,
     case APPFIS_BF16: tab = kBF16; ntab = (int)(sizeof(kBF16) / sizeof(kBF16[0])); break;
     default: return invalid("fdtd3d: dtype must be F32, F64, F16 or BF16");
     }
-    const bool pairs = ((nx & 1) == 0) && (((uintptr_t)buf0 | (uintptr_t)buf1) & 3) == 0;
+    const bool aligned4 = (((uintptr_t)buf0 | (uintptr_t)buf1) & 3) == 0;
+    auto fits = [&](const FdtdVariant& v) { return nx % v.vx == 0 && (!v.pairs || aligned4); };
     if (variant >= APPFIS_CANDIDATE0) {
         const int i = variant - APPFIS_CANDIDATE0;
         APPFIS_REQUIRE(i < ntab, "fdtd3d: candidate %d out of range (%d candidates)", i, ntab);
-        APPFIS_REQUIRE(!tab[i].pairs || pairs, "fdtd3d: candidate %d needs even nx and 4-byte aligned buffers", i);
+        APPFIS_REQUIRE(fits(tab[i]), "fdtd3d: candidate %s needs nx %% %d == 0%s", tab[i].name, tab[i].vx,
+                       tab[i].pairs ? " and 4-byte aligned buffers" : "");
         return steps == 0 ? APPFIS_OK : tab[i].fn(ctx, buf0, buf1, nx, ny, nz, coeff5, steps);
     }
     if (steps == 0) return APPFIS_OK;
@@ -413,11 +431,11 @@ extern "C" int appfis_fdtd3d(appfis_ctx* ctx, int dtype, void* buf0, void* buf1,
     std::vector<int> cand;
     std::vector<const char*> names;
     for (int i = 0; i < ntab; i++)
-        if (!tab[i].pairs || pairs) {
+        if (fits(tab[i])) {
             cand.push_back(i);
             names.push_back(tab[i].name);
         }
-    const int key[6] = {3, dtype, nx, ny, nz, pairs ? 1 : 0};
+    const int key[6] = {3, dtype, nx, ny, nz, aligned4 ? 1 : 0};
     int pick = 0;
     const size_t bytes = (size_t)(nx + 8) * (ny + 8) * (nz + 8) * dtype_size(dtype);
     int st = APPFIS_OK;

@@ -235,18 +235,26 @@ def test_fdtd3d_bit_exact(ctx, dt, n, steps, variant):
 
 
 @pytest.mark.parametrize("dt", [hip.F32, hip.F64, hip.F16, hip.BF16])
-def test_fdtd3d_every_candidate_bit_exact(ctx, dt):
-    # every tiling the autotuner may pick, forced one at a time
-    n = (70, 33, 40)
+@pytest.mark.parametrize("n", [(70, 33, 40), (68, 21, 37), (71, 30, 19)])
+def test_fdtd3d_every_candidate_bit_exact(ctx, dt, n):
+    # every tiling the autotuner may pick, forced one at a time; nx = 70, 68, 71 covers
+    # x points per lane (1, 2, 4) that do and do not divide nx
     coeff = FDTD_COEFF if dt != hip.F16 else [c / 2.0625 for c in FDTD_COEFF]
     a, d0, d1 = _fdtd_pair(ctx, n, dt)
     ref = oracle.fdtd3d(a.copy(), np.zeros_like(a), *n, coeff, 3, bf16=(dt == hip.BF16))
     names = hip.fdtd3d_candidates(dt)
     assert len(names) >= 2
+    ran = 0
     for i, name in enumerate(names):
         d0.upload(a)
-        ctx.fdtd3d(d0, d1, *n, coeff, 3, hip.CANDIDATE0 + i)
+        try:
+            ctx.fdtd3d(d0, d1, *n, coeff, 3, hip.CANDIDATE0 + i)
+        except hip.AppfisError as e:  # a candidate whose x points per lane do not divide nx
+            assert "needs nx" in str(e) and int(name.split()[0][2:]) > 1 and n[0] % int(name.split()[0][2:]), e
+            continue
+        ran += 1
         assert np.array_equal(d1.download().view(np.uint8), ref.view(np.uint8)), name
+    assert ran >= 2
 
 
 def test_fdtd3d_full_config(ctx):
