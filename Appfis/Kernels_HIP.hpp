@@ -53,6 +53,30 @@ namespace APPFIS
             detail_k::SwapParity(g, iters);
         }
 
+        // Many HeatDiffusion 2D grids of any sizes advanced together (one launch per chunk
+        // of sweeps over a tile work-list spanning every grid, replayed as a hipGraph);
+        // values identical to Heat2D on each grid.  All grids of one element type.
+        template <typename T>
+        void Heat2DEnsemble(std::vector<Grid<T>*>& grids, double c, int iters)
+        {
+            std::vector<int> nx, ny;
+            std::vector<void*> b0, b1;
+            for (Grid<T>* g : grids) {
+                detail_k::RequireDual(*g, "Heat2DEnsemble");
+                Array<T>* h = g->GetSubGridCPU();
+                nx.push_back(h->GetDim(X) - 2);
+                ny.push_back(h->GetDim(Y) - 2);
+                b0.push_back(g->GetSubGridsGPUData());
+                b1.push_back(g->GetSubGridsNextGPUData());
+            }
+            if (grids.empty()) return;
+            APPFIS::detail::Check(appfis_heat2d_ensemble(GetContext(), APPFIS::detail::DtypeOf<T>(),
+                                                         (int)grids.size(), nx.data(), ny.data(), b0.data(),
+                                                         b1.data(), c, iters),
+                                  "appfis_heat2d_ensemble");
+            for (Grid<T>* g : grids) detail_k::SwapParity(*g, iters);
+        }
+
         // GameOfLife B3/S23 on a periodic uint8 grid (ghost 1)
         inline void GameOfLife(Grid<uint8_t>& g, int gens, int variant = APPFIS_CHOSEN, int tsteps = 0)
         {

@@ -55,3 +55,33 @@ def test_invalid_arguments_are_reported_not_fatal():
     dims = (C.c_int * 2)(4, 4)
     assert L.appfis_exchange(None, None, 0, 2, dims, 1, 1) == 1
     assert L.appfis_heat2d(None, 3, None, None, 4, 4, 0.1, 1, 0, 0) == 1
+
+
+def test_header_is_plain_c_and_links(tmp_path):
+    # the boundary is usable from C (and so from any C FFI): the header compiles as C99
+    # with warnings as errors, and a C program links against the library and calls the
+    # context-free entry points (no GPU needed)
+    src = tmp_path / "abi_c.c"
+    src.write_text(r'''
+#include <stdio.h>
+#include <string.h>
+#include "appfis_hip.h"
+int main(void) {
+    char names[4096];
+    if (appfis_abi_version() != APPFIS_ABI_VERSION) return 1;
+    if (strcmp(appfis_status_string(APPFIS_OK), "ok") != 0) return 2;
+    if (appfis_fdtd3d_candidates(APPFIS_F32, names, (int)sizeof names) < 2) return 3;
+    if (appfis_heat2d_candidates(APPFIS_F64, names, (int)sizeof names) < 2) return 4;
+    if (appfis_gol_candidates(names, (int)sizeof names) < 2) return 5;
+    if (appfis_sync(NULL) != APPFIS_ERR_INVALID) return 6;
+    printf("ok %s\n", appfis_last_error());
+    return 0;
+}
+''')
+    exe = tmp_path / "abi_c"
+    subprocess.run(["gcc", "-std=c99", "-Wall", "-Wextra", "-Werror", "-pedantic", "-I", os.path.dirname(HEADER),
+                    str(src), "-o", str(exe), "-L", os.path.dirname(LIB), "-lappfis_hip",
+                    "-Wl,-rpath," + os.path.dirname(LIB)], check=True)
+    r = subprocess.run([str(exe)], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 0, (r.returncode, r.stdout, r.stderr)
+    assert r.stdout.startswith("ok")
