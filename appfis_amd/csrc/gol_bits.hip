@@ -140,13 +140,31 @@ __global__ __launch_bounds__(64 * NWAVES) void k_step_tb(const uint32_t* __restr
     const int left = wy * WX + (wx == 0 ? WX - 1 : wx - 1);
     const int right = wy * WX + (wx == WX - 1 ? 0 : wx + 1);
     const int y0 = (int)blockIdx.x * OUT - G + wy * K;  // global row of register row 0
+    // periodic row of register row 0 (the window may start up to G rows below 0 and end
+    // past ny); then consecutive rows step by one word row, wrapping once at ny when
+    // K <= ny (the general loop otherwise).  32-bit word offsets: NW * ny < 2^31.
+    int yw = y0 % ny;
+    if (yw < 0) yw += ny;
     uint32_t R[K];
+    if (K <= ny) {
+        int y = yw, off = yw * NW + w;
 #pragma unroll
-    for (int k = 0; k < K; k++) {
-        int y = y0 + k;
-        while (y < 0) y += ny;  // window rows may reach 2G below 0 or past ny: periodic
-        while (y >= ny) y -= ny;
-        R[k] = P[(size_t)y * NW + w];
+        for (int k = 0; k < K; k++) {
+            R[k] = P[off];
+            ++y;
+            off += NW;
+            if (y == ny) {
+                y = 0;
+                off = w;
+            }
+        }
+    } else {
+#pragma unroll
+        for (int k = 0; k < K; k++) {
+            int y = yw + k;
+            while (y >= ny) y -= ny;
+            R[k] = P[y * NW + w];
+        }
     }
     for (int g = 0; g < gens; g++) {
         if (lane == 0)
@@ -204,11 +222,13 @@ __global__ __launch_bounds__(64 * NWAVES) void k_step_tb(const uint32_t* __restr
             R[k] = (s0 & s1 & ~s2) | (R[k] & s2 & ~(s0 | s1));
         }
     }
+    // output rows: window rows G .. G+OUT-1 of the block, global rows blockIdx.x*OUT + r
+    const int ybase = (int)blockIdx.x * OUT - G + wy * K;
 #pragma unroll
     for (int k = 0; k < K; k++) {
         const int r = wy * K + k - G;  // output row within the block's OUT rows
-        const int y = (int)blockIdx.x * OUT + r;
-        if (r >= 0 && r < OUT && y < ny) Q[(size_t)y * NW + w] = R[k];
+        const int y = ybase + k;
+        if (r >= 0 && r < OUT && y < ny) Q[y * NW + w] = R[k];
     }
 }
 
