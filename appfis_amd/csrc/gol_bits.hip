@@ -232,6 +232,107 @@ __global__ __launch_bounds__(64 * NWAVES) void k_step_tb(const uint32_t* __restr
     }
 }
 
+// One barrier per generation.  Each wave publishes its raw edge words (lanes 0 / 63,
+// all K rows) and its first and last rows into the LDS buffers of the generation's
+// parity; after the barrier every wave computes the horizontal sums of its K rows and
+// of the two rows next to its band (those of the waves above / below, with their edge
+// words) itself.  The buffers alternate by generation, so a wave still reading
+// generation g's buffers is never overwritten: the next writes to them come after the
+// barrier of generation g+1, which every wave reaches only after finishing g.
+template <int K, int G, int NWAVES>
+__global__ __launch_bounds__(64 * NWAVES) void k_step_tb1(const uint32_t* __restrict__ P, uint32_t* __restrict__ Q,
+                                                           int NW, int ny, int WX, int OUT, int gens)
+{
+    __shared__ uint32_t edge_lo[2][NWAVES][K], edge_hi[2][NWAVES][K];  // lane-0 / lane-63 words
+    __shared__ uint32_t row_top[2][NWAVES][64], row_bot[2][NWAVES][64];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int wx = wv % WX, wy = wv / WX;
+    const int WY = NWAVES / WX;
+    const int w = wx * 64 + lane;
+    const int left = wy * WX + (wx == 0 ? WX - 1 : wx - 1);
+    const int right = wy * WX + (wx == WX - 1 ? 0 : wx + 1);
+    const int y0 = (int)blockIdx.x * OUT - G + wy * K;
+    int yw = y0 % ny;
+    if (yw < 0) yw += ny;
+    uint32_t R[K];
+    if (K <= ny) {
+        int y = yw, off = yw * NW + w;
+#pragma unroll
+        for (int k = 0; k < K; k++) {
+            R[k] = P[off];
+            ++y;
+            off += NW;
+            if (y == ny) {
+                y = 0;
+                off = w;
+            }
+        }
+    } else {
+#pragma unroll
+        for (int k = 0; k < K; k++) {
+            int y = yw + k;
+            while (y >= ny) y -= ny;
+            R[k] = P[y * NW + w];
+        }
+    }
+    for (int g = 0; g < gens; g++) {
+        const int b = g & 1;
+        if (lane == 0)
+#pragma unroll
+            for (int k = 0; k < K; k++) edge_lo[b][wv][k] = R[k];
+        if (lane == 63)
+#pragma unroll
+            for (int k = 0; k < K; k++) edge_hi[b][wv][k] = R[k];
+        row_top[b][wv][lane] = R[0];
+        row_bot[b][wv][lane] = R[K - 1];
+        __syncthreads();
+        // horizontal sums of rows -1 .. K (rows -1 / K: the neighbouring waves' band
+        // edges, zero outside the window)
+        uint32_t h0[K + 2], h1[K + 2];
+        auto hsum = [&](uint32_t c, uint32_t lo_edge, uint32_t hi_edge, uint32_t& s0, uint32_t& s1) {
+            const uint32_t l = (uint32_t)__builtin_amdgcn_update_dpp((int)lo_edge, (int)c, 0x138, 0xF, 0xF, false);
+            const uint32_t r = (uint32_t)__builtin_amdgcn_update_dpp((int)hi_edge, (int)c, 0x130, 0xF, 0xF, false);
+            const uint32_t Wc = __builtin_amdgcn_alignbit(c, l, 31);
+            const uint32_t Ec = __builtin_amdgcn_alignbit(r, c, 1);
+            s0 = xor3(Wc, c, Ec);
+            s1 = maj(Wc, c, Ec);
+        };
+#pragma unroll
+        for (int k = 0; k < K; k++) hsum(R[k], edge_hi[b][left][k], edge_lo[b][right][k], h0[k + 1], h1[k + 1]);
+        if (wy > 0) {
+            hsum(row_bot[b][wv - WX][lane], edge_hi[b][left - WX][K - 1], edge_lo[b][right - WX][K - 1], h0[0],
+                 h1[0]);
+        } else {
+            h0[0] = 0u;
+            h1[0] = 0u;
+        }
+        if (wy < WY - 1) {
+            hsum(row_top[b][wv + WX][lane], edge_hi[b][left + WX][0], edge_lo[b][right + WX][0], h0[K + 1],
+                 h1[K + 1]);
+        } else {
+            h0[K + 1] = 0u;
+            h1[K + 1] = 0u;
+        }
+#pragma unroll
+        for (int k = 0; k < K; k++) {
+            const uint32_t s0 = xor3(h0[k], h0[k + 1], h0[k + 2]);
+            const uint32_t c0 = maj(h0[k], h0[k + 1], h0[k + 2]);
+            const uint32_t v1 = xor3(h1[k], h1[k + 1], h1[k + 2]);
+            const uint32_t c1 = maj(h1[k], h1[k + 1], h1[k + 2]);
+            const uint32_t s1 = v1 ^ c0;
+            const uint32_t s2 = c1 ^ (v1 & c0);
+            R[k] = (s0 & s1 & ~s2) | (R[k] & s2 & ~(s0 | s1));
+        }
+    }
+    const int ybase = (int)blockIdx.x * OUT - G + wy * K;
+#pragma unroll
+    for (int k = 0; k < K; k++) {
+        const int r = wy * K + k - G;
+        const int y = ybase + k;
+        if (r >= 0 && r < OUT && y < ny) Q[y * NW + w] = R[k];
+    }
+}
+
 struct TbVariant {
     const char* name;
     void (*kernel)(const uint32_t*, uint32_t*, int, int, int, int, int);
@@ -245,6 +346,9 @@ static const TbVariant kTb[] = {
     {"tb k12 g4 w16", k_step_tb<12, 4, 16>, 12, 4, 16}, {"tb k32 g8 w8", k_step_tb<32, 8, 8>, 32, 8, 8},
     {"tb k16 g4 w8", k_step_tb<16, 4, 8>, 16, 4, 8},    {"tb k11 g5 w16", k_step_tb<11, 5, 16>, 11, 5, 16},
     {"tb k13 g6 w16", k_step_tb<13, 6, 16>, 13, 6, 16},
+    {"tb1 k10 g4 w16", k_step_tb1<10, 4, 16>, 10, 4, 16}, {"tb1 k11 g5 w16", k_step_tb1<11, 5, 16>, 11, 5, 16},
+    {"tb1 k13 g6 w16", k_step_tb1<13, 6, 16>, 13, 6, 16}, {"tb1 k16 g8 w16", k_step_tb1<16, 8, 16>, 16, 8, 16},
+    {"tb1 k20 g4 w8", k_step_tb1<20, 4, 8>, 20, 4, 8},
 };
 
 }  // namespace golbits

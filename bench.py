@@ -35,7 +35,10 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
 VALU_PEAK_TOPS = 256 * 64 * 2.4e9 / 1e12  # 32-bit VALU lane-instructions per second (T)
-GOL_VALU_PER_WORD = 17.6  # k_step_tb generation loop: 176 VALU instructions per 10-word column
+# VALU instructions per word and generation in the generation loop of the blocked GoL
+# kernels, counted in the gfx950 code object (K = 10 column: 176 for k_step_tb, 183 for
+# k_step_tb1, which also sums the two rows next to its band)
+GOL_VALU_PER_WORD = {"tb": 17.6, "tb1": 18.3}
 FDTD_COEFF = [0.5, 0.125, 0.0625, 0.125 / 3, 0.03125]
 
 
@@ -109,20 +112,21 @@ def bench_gol(ctx, hip, steps, warmup, variant, nx=8192, ny=8192):
         res["autotune"] = [l for l in ctx.tune_report().splitlines() if l.startswith("autotune kernel 4 ")]
         if tname:
             k, gg, w = (int(t[1:]) for t in tname.split()[1:4])
-            kname = f"k_step_tb<{k}, {gg}, {w}>"
+            kname = f"k_step_{tname.split()[0]}<{k}, {gg}, {w}>"
         else:
             kname = "k_step<8>"
         res["roofline"] = roofline(2.0 * nx * ny / 8, launch_ms * g, kname)
         if tname:
             # the blocked kernel is bound by integer VALU issue, not HBM: lane-instructions
-            # it issues per launch (every window word, every generation; GOL_VALU_PER_WORD
+            # it issues per launch (every window word, every generation; GOL_VALU_PER_WORD[kind]
             # instructions per word and generation in its generation loop, counted in the
             # gfx950 code object) against 256 CUs x 64 lanes x 2.4 GHz
             wx = (nx // 32) // 64
             out_rows = (w // wx) * k - 2 * gg
             words = -(-ny // out_rows) * 64 * w * k * gg
-            tops = words * GOL_VALU_PER_WORD / (launch_ms * g * 1e-3) / 1e12
-            res["roofline"]["valu"] = {"lane_instr_per_word_gen": GOL_VALU_PER_WORD, "achieved": round(tops, 2),
+            per_word = GOL_VALU_PER_WORD[tname.split()[0]]
+            tops = words * per_word / (launch_ms * g * 1e-3) / 1e12
+            res["roofline"]["valu"] = {"lane_instr_per_word_gen": per_word, "achieved": round(tops, 2),
                                        "peak": VALU_PEAK_TOPS, "unit": "T lane-instr/s",
                                        "frac": round(tops / VALU_PEAK_TOPS, 4)}
     d0.free()
