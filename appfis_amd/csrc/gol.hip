@@ -45,6 +45,26 @@ __device__ __forceinline__ void window(const uint32_t (&a)[CB / 4 + 4], uint32_t
     right = (a[R >> 2] >> ((R & 3) * 8)) & 0xFFu;
 }
 
+// Row of a byte offset p < 2^31 of rows of W >= 2 bytes, by multiply-high: with
+// 2^(l-1) < W <= 2^l and m = floor(2^(31+l) / W) + 1 (< 2^32),
+// floor(p*m / 2^(31+l)) = floor(p / W), because the excess p*(m - 2^(31+l)/W)/2^(31+l)
+// is below 2^31/2^(31+l) <= 1/W.
+struct RowDiv {
+    uint32_t m;
+    int sh;  // l - 1
+    __device__ __forceinline__ int row(int p) const { return (int)(__umulhi((uint32_t)p, m) >> sh); }
+};
+
+inline RowDiv make_rowdiv(int W)
+{
+    int l = 0;
+    while ((1LL << l) < W) l++;
+    RowDiv r;
+    r.m = (uint32_t)((1ULL << (31 + l)) / (uint64_t)W + 1ULL);
+    r.sh = l - 1;
+    return r;
+}
+
 __device__ __forceinline__ uint32_t zero_bytes(uint32_t t)
 {
     // bytes of t are <= 0x0B: (t + 0x7F) sets bit 7 exactly for non-zero bytes
@@ -69,7 +89,7 @@ __device__ uint8_t cell_new(const uint8_t* __restrict__ u, int W, int H, int x, 
 
 template <int DB, int CB>
 __global__ __launch_bounds__(256) void k_gol_plain(const uint8_t* __restrict__ u, uint8_t* __restrict__ v, int W,
-                                                   int H, int nchunks, int nbytes)
+                                                   int H, int nchunks, int nbytes, RowDiv rd)
 {
     constexpr int ND = CB / 4;   // dwords of a chunk
     constexpr int NA = ND + 4;   // dwords of the aligned span holding a neighbour row's window
@@ -87,8 +107,8 @@ __global__ __launch_bounds__(256) void k_gol_plain(const uint8_t* __restrict__ u
     // rows of the first and last cell of the chunk, and whether the fast path applies
     // (all CB cells in interior rows and every load in bounds) -- before any load, so
     // that all loads of the lane are in flight together
-    const int y0 = (unsigned)p0 / (unsigned)W;
-    const int y1 = (unsigned)(p0 + CB - 1) / (unsigned)W;
+    const int y0 = rd.row(p0);
+    const int y1 = rd.row(p0 + CB - 1);
     const int below_end = ((p0 + W) & ~15) + 4 * NA;
     const bool fast = full && y0 >= 1 && y1 <= H - 2 && p0 - W >= 16 && below_end <= nbytes;
     // centre chunk (every valid full lane loads it; shared with neighbours below) and,
@@ -151,9 +171,8 @@ __global__ __launch_bounds__(256) void k_gol_plain(const uint8_t* __restrict__ u
             const uint32_t left = abyte(s[d], lo, 3);
             const uint32_t right = abyte(hi, s[d], 1);
             const uint32_t n = left + s[d] + right - m[d];
-            const uint32_t eq3 = zero_bytes(n ^ 0x03030303u);
-            const uint32_t eq2 = zero_bytes(n ^ 0x02020202u);
-            out[d] = eq3 | (eq2 & m[d]);
+            // alive' = (n == 3) | (alive & n == 2) = ((n | alive) == 3) for n <= 8, alive <= 1
+            out[d] = zero_bytes((n | m[d]) ^ 0x03030303u);
         }
         // patch ghost cells (x == 0 or x == W-1) inside the chunk
         const int x0 = p0 - y0 * W;
@@ -187,7 +206,7 @@ __global__ __launch_bounds__(256) void k_gol_plain(const uint8_t* __restrict__ u
 
 constexpr int CHUNK = 16;  // bytes (cells) per lane (32 measured slower: fewer lanes in flight)
 
-typedef void (*gol_kernel_t)(const uint8_t*, uint8_t*, int, int, int, int);
+typedef void (*gol_kernel_t)(const uint8_t*, uint8_t*, int, int, int, int, RowDiv);
 
 }  // namespace gol
 
@@ -227,9 +246,10 @@ extern "C" int appfis_gol(appfis_ctx* ctx, uint8_t* buf0, uint8_t* buf1, int nx,
     if (st) return st;
     uint8_t* buf[2] = {buf0, buf1};
     const unsigned blocks = ceil_div((size_t)nchunks, 256);
+    const gol::RowDiv rd = gol::make_rowdiv(W);
     for (int g = 0; g < gens; g++) {
         hipLaunchKernelGGL(k, dim3(blocks), dim3(256), 0, ctx->stream, buf[g & 1], buf[(g + 1) & 1], W, H, nchunks,
-                           nbytes);
+                           nbytes, rd);
         APPFIS_LAUNCH_CHECK();
     }
     return APPFIS_OK;
