@@ -198,8 +198,51 @@ def bench_heat(ctx, hip, n=1024, iters=500):
         if variant == hip.CHOSEN:
             r["tuned"] = tname
         out[name] = r
-    out["workload"] = f"Heat2D {n}x{n} FP64 5-point, {iters} sweeps (16 B/LUP)"
+    # mixed precision: the same sweeps in FP32 (temporal blocking), error against FP64
+    ctx.fill_uniform(a, 1, 1001)
+    init64 = a.download()
+    ctx.heat2d(a, b, n, n, 0.1, iters, hip.CHOSEN)
+    ref64 = a.download()
+    a32 = ctx.from_numpy(init64.astype(np.float32), hip.F32)
+    b32 = ctx.array((n + 2, n + 2), hip.F32)
+    ctx.heat2d(a32, b32, n, n, 0.1, 20, hip.CHOSEN)  # warm-up (and the one-time autotuning)
+    a32.upload(init64.astype(np.float32))
+    ms, _ = timed(ctx, lambda: ctx.heat2d(a32, b32, n, n, 0.1, iters, hip.CHOSEN))
+    f32 = a32.download().astype(np.float64)
+    out["chosen_f32"] = {"ms": round(ms, 4), "glups": round(n * n * iters / (ms * 1e-3) / 1e9, 2),
+                         "tuned": _tuned(ctx, 1, hip.F32),
+                         "max_abs_err_vs_f64": float(np.max(np.abs(f32 - ref64))),
+                         "rel_l2_err_vs_f64": float(np.linalg.norm(f32 - ref64) / np.linalg.norm(ref64)),
+                         "speedup_vs_plain_f64": round(out["plain"]["ms"] / ms, 3)}
+    a32.free()
+    b32.free()
+    out["workload"] = f"Heat2D {n}x{n} FP64 5-point, {iters} sweeps (16 B/LUP); chosen_f32 = FP32 storage"
+    if CPU_SAMPLES:
+        import oracle
+
+        h = init64.copy()
+        out["cpu"] = _cpu_rate(lambda k: oracle.heat2d(h, np.zeros_like(h), n, n, 0.1, k), n * n,
+                               f"Heat2D {n}x{n} FP64")
     return out
+
+
+CPU_SAMPLES = True  # per-app CPU oracle samples next to the suite entries (off with --no-cpu)
+
+
+def _cpu_rate(run, cells_per_step, what, target_s=3.0):
+    """GLUP/s of the oracle (OpenMP restatement of the reference CPU path) on a bounded
+    number of steps: one step to size the sample, then about target_s seconds."""
+    import oracle
+
+    t0 = time.perf_counter()
+    run(1)
+    one = time.perf_counter() - t0
+    k = max(1, min(200, int(target_s / max(one, 1e-6))))
+    t0 = time.perf_counter()
+    run(k)
+    dt = time.perf_counter() - t0
+    return {"value": round(cells_per_step * k / dt / 1e9, 4), "unit": "GLUP/s", "cores": oracle.omp_threads(),
+            "kind": "port", "sample": f"{what}: {k} steps of the oracle in {dt:.2f} s"}
 
 
 def _splitmix_sides(n, seed, lo, hi):
@@ -297,6 +340,13 @@ def bench_fdtd(ctx, hip, n=512, steps=100):
         a.free()
         b.free()
     out["autotune"] = [l for l in ctx.tune_report().splitlines() if "kernel 3 " in l]
+    if CPU_SAMPLES:
+        import oracle
+
+        u = np.zeros((n + 8, n + 8, n + 8), dtype=np.float32)
+        oracle.fill_uniform(u, 4, 1004)
+        v = np.zeros_like(u)
+        out["cpu"] = _cpu_rate(lambda k: oracle.fdtd3d(u, v, n, n, n, FDTD_COEFF, k), n ** 3, f"FDTD3d {n}^3 FP32")
     return out
 
 
@@ -345,6 +395,15 @@ def bench_himeno(ctx, hip, nx=257, ny=257, nz=513, iters=200):
     for name in ("f32", "dc"):
         out[name]["gosa_max_rel_err_vs_f64"] = float(np.max(np.abs(gosas[name] - ref) / ref))
     out["dc"]["speedup_vs_f64"] = round(out["f64"]["ms"] / out["dc"]["ms"], 3)
+    if CPU_SAMPLES:
+        import oracle
+
+        k = np.arange(nz, dtype=np.float64).reshape(nz, 1, 1)
+        p = np.ascontiguousarray(np.broadcast_to((k / (nz - 1)) ** 2, (nz, ny, nx)))
+        harr = [p.copy(), p.copy()] + [np.full((nz, ny, nx), v) for v in
+                                       (1.0, 1.0, 1.0, 1.0 / 6.0, 0.0, 0.0, 0.0, 1.0, 1.0, 1.0, 1.0, 0.0)]
+        out["cpu"] = _cpu_rate(lambda it: oracle.himeno(harr, nx, ny, nz, 0.8, it), (nx - 2) * (ny - 2) * (nz - 2),
+                               f"Himeno {nx}x{ny}x{nz} FP64")
     out["autotune"] = [l for l in ctx.tune_report().splitlines() if l.startswith("autotune kernel 5 ")]
     return out
 
@@ -433,6 +492,14 @@ def bench_flood(ctx, hip, steps=2000, nscen=24):
     for d in sc:
         for k in ("h", "qx", "qy", "z", "n2"):
             d[k].free()
+    if CPU_SAMPLES:
+        import oracle
+
+        n0 = sizes[0]
+        z = oracle.flood_dem(n0, n0, 10.0, 1005)
+        f = [np.zeros_like(z), np.zeros_like(z), np.zeros_like(z), z, np.full_like(z, 0.035 ** 2)]
+        out["cpu"] = _cpu_rate(lambda k: oracle.flood_run(*f, n0, n0, 10.0, 0.5, 1e-3, 1.0, 0.0, 50e-3 / 3600.0, 0.0,
+                                                          k), n0 * n0, f"flood scenario 0 ({n0}^2) FP64 from its dry start")
     return out
 
 
@@ -546,6 +613,8 @@ def main():
     if args.summarize:
         summarize(args.summarize, args.fetch_db, args.write_db, args.prof_cmd, os.path.join(ROOT, "profiles"))
         return
+    global CPU_SAMPLES
+    CPU_SAMPLES = not args.no_cpu
     if args.gpus != 1:
         print(f"bench.py measures one GPU: this build has no multi-GPU path (asked for --gpus {args.gpus}); "
               "DESIGN.md describes how the stencils would shard.", file=sys.stderr)

@@ -213,6 +213,26 @@ def test_gol_chosen_full_config_prefix(ctx):
     assert np.array_equal(d0.download(), c0) and np.array_equal(d1.download(), c1)
 
 
+def test_gol_full_config_every_100th_generation(ctx):
+    # the 8192^2 config for 1000 generations, checked at every 100th generation: the byte
+    # kernel and the bit-packed kernel are independent implementations of the rule and
+    # must agree on both buffers (the oracle pins the first generations above)
+    nx = ny = 8192
+    a = np.zeros((ny + 2, nx + 2), dtype=np.uint8)
+    oracle.fill_bernoulli(a, 1, 1002, 0.25)
+    p0, p1 = ctx.from_numpy(a), ctx.array(a.shape, hip.U8)
+    c0, c1 = ctx.from_numpy(a), ctx.array(a.shape, hip.U8)
+    live = []
+    for chunk in range(10):
+        ctx.gol(p0, p1, nx, ny, 100, hip.PLAIN)
+        ctx.gol(c0, c1, nx, ny, 100, hip.CHOSEN)
+        got_p, got_c = p0.download(), c0.download()
+        assert np.array_equal(got_p, got_c), (chunk + 1) * 100
+        live.append(int(got_p[1:-1, 1:-1].sum(dtype=np.int64)))
+    print(f"[gol] live cells at generations 100..1000: {live}")
+    assert all(0 < n < nx * ny for n in live)
+
+
 def _fdtd_pair(ctx, n, dt, seed=1004):
     npdt = {hip.F32: np.float32, hip.F64: np.float64, hip.F16: np.float16, hip.BF16: np.uint16}[dt]
     shape = tuple(x + 8 for x in n[::-1])
