@@ -326,7 +326,7 @@ class Context:
     def heat2d_ensemble(self, grids, c, iters):
         """grids: list of (buf0, buf1, nx, ny); all of one dtype."""
         n = len(grids)
-        dt = grids[0][0].dtype
+        dt = grids[0][0].dtype if grids else F64  # an empty list is refused by the library
         b0 = (C.c_void_p * n)(*[g[0].ptr for g in grids])
         b1 = (C.c_void_p * n)(*[g[1].ptr for g in grids])
         check(lib().appfis_heat2d_ensemble(self.handle, dt, n, _ints([g[2] for g in grids]),
