@@ -38,7 +38,8 @@ using namespace appfis;
 namespace {
 
 constexpr int BX = 64, BY = 4, BT = BX * BY;
-constexpr int EBY = 8;  // rows of a flux-kernel tile (BX x EBY cells, BT threads)
+constexpr int EBY = 8;  // rows of a flux-kernel tile (BX x EBY cells, BT threads; 16 measured slower)
+constexpr int FBY = 16; // rows of a wet/dry + CFL + volume tile (BX x FBY cells, FBY / BY per thread)
 constexpr int CHUNK = 20;  // steps per captured graph (even: buffer parity returns)
 
 template <typename R> struct RealLim;
@@ -65,7 +66,7 @@ struct Scen {
     const R* n2;
     int nx, ny;
     int tiles_x;
-    int tile0;         // first global tile of this scenario (BX x BY tiles: wet/dry, CFL, volume)
+    int tile0;         // first global tile of this scenario (BX x FBY tiles: wet/dry, CFL, volume)
     int ntiles;
     int etiles_x;      // the flux kernel's BX x EBY tiles
     int etile0;
@@ -91,12 +92,59 @@ template <> __device__ __forceinline__ float fdiv<float>(float a, float b) { ret
 template <typename R> __device__ __forceinline__ R fsqrt(R a) { return sqrt(a); }
 template <> __device__ __forceinline__ float fsqrt<float>(float a) { return __builtin_amdgcn_sqrtf(a); }
 
+// Edge normal: (1, 0) for DIR 0 (X edges), (0, 1) for DIR 1 (Y edges).  The FP64 form
+// multiplies the normal out exactly as the reference functors do (a product with the
+// zero component is +-0 and cannot be dropped without changing signed zeros, so the
+// results stay bit-identical); the FP32 bulk form drops the terms multiplied by the
+// zero component.
+template <typename R, int DIR>
+struct Normal {
+    static constexpr bool EXACT = sizeof(R) == 8;
+    static constexpr R nx = DIR == 0 ? (R)1 : (R)0, ny = DIR == 0 ? (R)0 : (R)1;
+    __device__ static R dot(R a, R b)  // a*nx + b*ny
+    {
+        if constexpr (EXACT) return a * nx + b * ny;
+        else return DIR == 0 ? a : b;
+    }
+    __device__ static R mx(R c)  // c*nx
+    {
+        if constexpr (EXACT) return c * nx;
+        else return DIR == 0 ? c : (R)0;
+    }
+    __device__ static R my(R c)  // c*ny
+    {
+        if constexpr (EXACT) return c * ny;
+        else return DIR == 1 ? c : (R)0;
+    }
+    __device__ static R addx(R a, R c)  // a + c*nx
+    {
+        if constexpr (EXACT) return a + c * nx;
+        else return DIR == 0 ? a + c : a;
+    }
+    __device__ static R addy(R a, R c)  // a + c*ny
+    {
+        if constexpr (EXACT) return a + c * ny;
+        else return DIR == 1 ? a + c : a;
+    }
+    __device__ static R subx(R a, R c)  // a - c*nx
+    {
+        if constexpr (EXACT) return a - c * nx;
+        else return DIR == 0 ? a - c : a;
+    }
+    __device__ static R suby(R a, R c)  // a - c*ny
+    {
+        if constexpr (EXACT) return a - c * ny;
+        else return DIR == 1 ? a - c : a;
+    }
+};
+
 // Roe flux with entropy fix, bed-slope source and wet/dry treatment -- the shared body
-// of the X functor (triton_appfis.hpp:608-824) and the Y functor (827-1043).
-template <typename R>
-__device__ inline bool roe(R h1, R h2, R qx1, R qy1, R z1, R qx2, R qy2, R z2, R sqh1, R sqh2, R nx_, R ny_, R dx,
-                           R dt, R hextra, R d1[3], R d2[3])
+// of the X functor (triton_appfis.hpp:608-824, DIR 0) and the Y functor (827-1043, DIR 1).
+template <typename R, int DIR>
+__device__ inline bool roe(R h1, R h2, R qx1, R qy1, R z1, R qx2, R qy2, R z2, R sqh1, R sqh2, R dx, R dt, R hextra,
+                           R d1[3], R d2[3])
 {
+    using N = Normal<R, DIR>;
     const R G = (R)9.81, SQRTG = (R)3.132091953, EPS12 = (R)1e-12;
     if (!(h1 > (R)0.0 || h2 > (R)0.0)) return false;
     R alpha[3], beta[3], eigen[3], eigenE[3], eigev[3][3];
@@ -110,36 +158,36 @@ __device__ inline bool roe(R h1, R h2, R qx1, R qy1, R z1, R qx2, R qy2, R z2, R
     const R c12 = fsqrt(G * h12);
     const R u12 = fdiv(u1 * sqh1 + u2 * sqh2, sqh1 + sqh2);
     const R v12 = fdiv(v1 * sqh1 + v2 * sqh2, sqh1 + sqh2);
-    const R un = u12 * nx_ + v12 * ny_;
+    const R un = N::dot(u12, v12);
     eigen[0] = un - c12;
     eigen[1] = un;
     eigen[2] = un + c12;
 #pragma unroll
     for (int i = 0; i < 3; i++) eigenE[i] = (R)0.0;
-    R e1 = u1 * nx_ + v1 * ny_ - SQRTG * sqh1;
-    R e2 = u2 * nx_ + v2 * ny_ - SQRTG * sqh2;
+    R e1 = N::dot(u1, v1) - SQRTG * sqh1;
+    R e2 = N::dot(u2, v2) - SQRTG * sqh2;
     if (e1 < (R)0.0 && e2 > (R)0.0) {
         eigenE[0] = eigen[0] - fdiv(e1 * (e2 - eigen[0]), e2 - e1);
         eigen[0] = fdiv(e1 * (e2 - eigen[0]), e2 - e1);
     }
-    e1 = u1 * nx_ + v1 * ny_ + SQRTG * sqh1;
-    e2 = u2 * nx_ + v2 * ny_ + SQRTG * sqh2;
+    e1 = N::dot(u1, v1) + SQRTG * sqh1;
+    e2 = N::dot(u2, v2) + SQRTG * sqh2;
     if (e1 < (R)0.0 && e2 > (R)0.0) {
         eigenE[2] = eigen[2] - fdiv(e2 * (eigen[2] - e1), e2 - e1);
         eigen[2] = fdiv(e2 * (eigen[2] - e1), e2 - e1);
     }
     eigev[0][0] = (R)1.0;
-    eigev[0][1] = u12 - c12 * nx_;
-    eigev[0][2] = v12 - c12 * ny_;
+    eigev[0][1] = N::subx(u12, c12);
+    eigev[0][2] = N::suby(v12, c12);
     eigev[1][0] = (R)0.0;
-    eigev[1][1] = -c12 * ny_;
-    eigev[1][2] = c12 * nx_;
+    eigev[1][1] = N::my(-c12);
+    eigev[1][2] = N::mx(c12);
     eigev[2][0] = (R)1.0;
-    eigev[2][1] = u12 + c12 * nx_;
-    eigev[2][2] = v12 + c12 * ny_;
-    alpha[0] = (R)0.5 * (dh - fdiv(((qx2 - qx1) * nx_ + (qy2 - qy1) * ny_) - un * dh, c12));
-    alpha[1] = fdiv(((qy2 - qy1) - v12 * dh) * nx_ - ((qx2 - qx1) - u12 * dh) * ny_, c12);
-    alpha[2] = (R)0.5 * (dh + fdiv(((qx2 - qx1) * nx_ + (qy2 - qy1) * ny_) - un * dh, c12));
+    eigev[2][1] = N::addx(u12, c12);
+    eigev[2][2] = N::addy(v12, c12);
+    alpha[0] = (R)0.5 * (dh - fdiv(N::dot(qx2 - qx1, qy2 - qy1) - un * dh, c12));
+    alpha[1] = fdiv(N::mx((qy2 - qy1) - v12 * dh) - N::my((qx2 - qx1) - u12 * dh), c12);
+    alpha[2] = (R)0.5 * (dh + fdiv(N::dot(qx2 - qx1, qy2 - qy1) - un * dh, c12));
     const R dz = z2 - z1;
     R l1 = z1 + h1;
     R l2 = z2 + h2;
@@ -270,8 +318,8 @@ __global__ __launch_bounds__(BT) void k_flux_update(const Scen<R>* __restrict__ 
         if (j <= ny && iL <= nx) {
             const size_t cL = (size_t)iL + (size_t)W * (size_t)j, cR = cL + 1;
             const R hL = h[cL], hR = h[cR];
-            ok = roe(hL, hR, qx[cL], qy[cL], z[cL], qx[cR], qy[cR], z[cR], sqrt_pos(hL), sqrt_pos(hR), (R)1.0,
-                     (R)0.0, dx, dt, hextra, d1, d2);
+            ok = roe<R, 0>(hL, hR, qx[cL], qy[cL], z[cL], qx[cR], qy[cR], z[cR], sqrt_pos(hL), sqrt_pos(hR), dx, dt,
+                           hextra, d1, d2);
         }
         edge_store(ebuf, eok, e, ok, d1, d2);
     }
@@ -304,8 +352,8 @@ __global__ __launch_bounds__(BT) void k_flux_update(const Scen<R>* __restrict__ 
         if (i <= nx && jU <= ny + 1) {
             const size_t cU = (size_t)i + (size_t)W * (size_t)jU, cD = cU - W;
             const R hU = h[cU], hD = h[cD];
-            ok = roe(hU, hD, qx[cU], qy[cU], z[cU], qx[cD], qy[cD], z[cD], sqrt_pos(hU), sqrt_pos(hD), (R)0.0,
-                     (R)1.0, dx, dt, hextra, d1, d2);
+            ok = roe<R, 1>(hU, hD, qx[cU], qy[cU], z[cU], qx[cD], qy[cD], z[cD], sqrt_pos(hU), sqrt_pos(hD), dx, dt,
+                           hextra, d1, d2);
         }
         edge_store(ebuf, eok, e, ok, d1, d2);
     }
@@ -366,8 +414,11 @@ __device__ inline void two_sum(double& s, double& cmp, double x)
 }
 
 // K2: wet/dry fix + per-tile CFL minimum and compensated volume partial (no atomics;
-// one tile of 64 x 4 cells per block, partials[3t..3t+2] = (min, sum, compensation)).
-// MODE 0: initial CFL only (no fix); MODE 1: end of a step.
+// one tile of BX x FBY cells per block, partials[3t..3t+2] = (min, sum, compensation)).
+// Each thread folds its FBY / BY cells (rows BY apart) into its own minimum and
+// compensated sum before the one wave butterfly, so the FP64 reduction is paid once
+// per thread rather than once per cell.  MODE 0: initial CFL only (no fix); MODE 1:
+// end of a step.
 template <typename R, int MODE>
 __global__ __launch_bounds__(BT) void k_fix_cfl(const Scen<R>* __restrict__ sc, int nscen, int par, Params P)
 {
@@ -379,18 +430,21 @@ __global__ __launch_bounds__(BT) void k_fix_cfl(const Scen<R>* __restrict__ sc, 
     const int t = tile - S.tile0;
     const int tyi = t / S.tiles_x, txi = t - tyi * S.tiles_x;
     const int i = txi * BX + (threadIdx.x & (BX - 1)) + 1;
-    const int j = tyi * BY + (threadIdx.x >> 6) + 1;
+    const int jb = tyi * FBY + (threadIdx.x >> 6) + 1;
     const R G = (R)9.81, EPS12 = (R)1e-12;
     const R hextra = (R)P.hextra, dx = (R)P.dx, cn = (R)P.cn;
     R cfl = RealLim<R>::max();
     double vs = 0.0, vc = 0.0;
-    if (i <= S.nx && j <= S.ny) {
-        const int W = S.nx + 2;
+    const int W = S.nx + 2;
+    const R* h = S.h[par];
+    R* qx = S.qx[par];
+    R* qy = S.qy[par];
+    const R* z = S.z;
+#pragma unroll
+    for (int k = 0; k < FBY / BY; k++) {
+        const int j = jb + k * BY;
+        if (i > S.nx || j > S.ny) continue;
         const size_t c = (size_t)i + (size_t)W * (size_t)j;
-        const R* h = S.h[par];
-        R* qx = S.qx[par];
-        R* qy = S.qy[par];
-        const R* z = S.z;
         const R hij = h[c];
         R qxv = qx[c], qyv = qy[c];
         if (MODE == 1 && hij > hextra) {
@@ -407,9 +461,10 @@ __global__ __launch_bounds__(BT) void k_fix_cfl(const Scen<R>* __restrict__ sc, 
         if (hij > hextra) {
             const R dtx = fdiv(dx, fabs(fdiv(qxv, hij)) + fsqrt(G * hij));
             const R dty = fdiv(dx, fabs(fdiv(qyv, hij)) + fsqrt(G * hij));
-            cfl = dtx < dty ? cn * dtx : cn * dty;
+            const R cc = dtx < dty ? cn * dtx : cn * dty;
+            cfl = cc < cfl ? cc : cfl;
         }
-        vs = (double)hij;
+        two_sum(vs, vc, (double)hij);
     }
     for (int o = 32; o > 0; o >>= 1) {
         R u = __shfl_xor(cfl, o, 64);
@@ -531,7 +586,7 @@ int run_flood(appfis_ctx* ctx, int nscen, const int* nx, const int* ny, void* co
     long long total_tiles = 0, total_etiles = 0;
     size_t field_bytes = 0;
     for (int s = 0; s < nscen; s++) {
-        int tx = (int)ceil_div(nx[s], BX), ty = (int)ceil_div(ny[s], BY);
+        int tx = (int)ceil_div(nx[s], BX), ty = (int)ceil_div(ny[s], FBY);
         hs[s].nx = nx[s];
         hs[s].ny = ny[s];
         hs[s].tiles_x = tx;

@@ -56,6 +56,21 @@ def _traffic(kernels):
         return None
 
 
+def _valu_busy(kernels):
+    """Fraction of the kernel's time the vector ALUs issue (rocprofv3's derived VALUBusy,
+    read from profiles/traffic.json) for each listed kernel that was profiled, or None."""
+    path = os.path.join(ROOT, "profiles", "traffic.json")
+    if not os.path.exists(path):
+        return None
+    try:
+        tab = json.load(open(path))
+    except ValueError:
+        return None
+    got = {k: round(tab[k]["valu_busy_pct"] / 100.0, 4) for k in kernels
+           if isinstance(tab.get(k), dict) and "valu_busy_pct" in tab[k]}
+    return got or None
+
+
 def roofline(alg_bytes_per_launch, launch_ms, kernels):
     """`kernels`: the dominant kernel's name, or {name: launches per roofline unit}."""
     if isinstance(kernels, str):
@@ -64,7 +79,7 @@ def roofline(alg_bytes_per_launch, launch_ms, kernels):
     return {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": _traffic(kernels),
             "kernel": " + ".join(kernels), "alg_bytes_per_launch": alg_bytes_per_launch,
-            "avg_launch_us": round(launch_ms * 1e3, 3)}
+            "avg_launch_us": round(launch_ms * 1e3, 3), "valu_busy": _valu_busy(kernels)}
 
 
 def timed(ctx, fn):
@@ -537,7 +552,7 @@ def _short(name):
     return n.strip()
 
 
-def summarize(kt_db, fetch_db, write_db, cmds, out_dir):
+def summarize(kt_db, fetch_db, write_db, cmds, out_dir, valu_db=None):
     import sqlite3
 
     durs = {}
@@ -554,6 +569,7 @@ def summarize(kt_db, fetch_db, write_db, cmds, out_dir):
         return {k: (sum(v) / len(v), len(v)) for k, v in acc.items()}
 
     fetch, write = pmc(fetch_db, "FETCH_SIZE"), pmc(write_db, "WRITE_SIZE")
+    busy, insts = pmc(valu_db, "VALUBusy"), pmc(valu_db, "SQ_INSTS_VALU")
     traffic = {}
     if "k_copy16" in fetch and "k_copy16" in write:
         cf = COPY_BYTES / (fetch["k_copy16"][0] * 1024.0)
@@ -567,6 +583,13 @@ def summarize(kt_db, fetch_db, write_db, cmds, out_dir):
             traffic[k] = {"fetch_kib": round(fetch[k][0], 1), "write_kib": round(write[k][0], 1),
                           "dispatches": fetch[k][1],
                           "bytes_per_launch": round(fetch[k][0] * 1024 * cf + write[k][0] * 1024 * cw, 1)}
+    for k in busy:
+        if not k.startswith("_"):
+            e = traffic.setdefault(k, {})
+            e["valu_busy_pct"] = round(busy[k][0], 2)
+            if k in insts:
+                e["valu_wave_insts_per_launch"] = round(insts[k][0], 0)
+    if traffic:
         with open(os.path.join(out_dir, "traffic.json"), "w") as f:
             json.dump(traffic, f, indent=1, sort_keys=True)
     for fname, title, prefixes in PROFILE_GROUPS:
@@ -578,13 +601,18 @@ def summarize(kt_db, fetch_db, write_db, cmds, out_dir):
         lines += ["", "Per-dispatch kernel durations (kernel trace) and, where measured, HBM traffic per",
                   "dispatch (FETCH_SIZE x fetch_scale + WRITE_SIZE x write_scale, KiB as read; scales "
                   "from a 1 GiB copy kernel, see traffic.json `_calibration`).", "",
-                  "| kernel | calls | avg us | min us | max us | total ms | FETCH KiB | WRITE KiB | traffic MB |",
-                  "|---|---:|---:|---:|---:|---:|---:|---:|---:|"]
+                  "VALU busy: rocprofv3's derived VALUBusy (share of the dispatch's time the vector ALUs",
+                  "issue), with SQ_INSTS_VALU (wave instructions per dispatch), from a third counter run.", "",
+                  "| kernel | calls | avg us | min us | max us | total ms | FETCH KiB | WRITE KiB | traffic MB "
+                  "| VALU busy % | VALU wave-insts |",
+                  "|---|---:|---:|---:|---:|---:|---:|---:|---:|---:|---:|"]
         for k in names:
             d = durs[k]
             t = traffic.get(k)
-            tf = (f"{t['fetch_kib']:.0f} | {t['write_kib']:.0f} | {t['bytes_per_launch'] / 1e6:.2f}" if t
-                  else "- | - | -")
+            tf = (f"{t['fetch_kib']:.0f} | {t['write_kib']:.0f} | {t['bytes_per_launch'] / 1e6:.2f}"
+                  if t and "bytes_per_launch" in t else "- | - | -")
+            tf += (f" | {t['valu_busy_pct']:.1f} | {t.get('valu_wave_insts_per_launch', 0):.0f}"
+                   if t and "valu_busy_pct" in t else " | - | -")
             lines.append(f"| `{k}` | {len(d)} | {sum(d) / len(d):.2f} | {min(d):.2f} | {max(d):.2f} | "
                          f"{sum(d) / 1e3:.3f} | {tf} |")
         with open(os.path.join(out_dir, fname + ".md"), "w") as f:
@@ -608,10 +636,12 @@ def main():
                     "--kernel-trace database (no GPU work)")
     ap.add_argument("--fetch-db", help="rocprofv3 --pmc FETCH_SIZE database for --summarize")
     ap.add_argument("--write-db", help="rocprofv3 --pmc WRITE_SIZE database for --summarize")
+    ap.add_argument("--valu-db", help="rocprofv3 --pmc VALUBusy SQ_INSTS_VALU database for --summarize")
     ap.add_argument("--prof-cmd", action="append", default=[], help="command text recorded in the summaries")
     args = ap.parse_args()
     if args.summarize:
-        summarize(args.summarize, args.fetch_db, args.write_db, args.prof_cmd, os.path.join(ROOT, "profiles"))
+        summarize(args.summarize, args.fetch_db, args.write_db, args.prof_cmd, os.path.join(ROOT, "profiles"),
+                  args.valu_db)
         return
     global CPU_SAMPLES
     CPU_SAMPLES = not args.no_cpu

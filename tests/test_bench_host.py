@@ -58,6 +58,19 @@ def test_traffic_lookup_weights(tmp_path, monkeypatch):
     assert bench._traffic({"missing": 1.0}) is None
 
 
+def test_valu_busy_lookup(tmp_path, monkeypatch):
+    prof = tmp_path / "profiles"
+    prof.mkdir()
+    (prof / "traffic.json").write_text(json.dumps({"_calibration": {"fetch_scale": 2.0},
+                                                   "a": {"bytes_per_launch": 1.0, "valu_busy_pct": 75.5},
+                                                   "b": {"bytes_per_launch": 1.0}}))
+    monkeypatch.setattr(bench, "ROOT", str(tmp_path))
+    assert bench._valu_busy({"a": 1, "b": 1}) == {"a": 0.755}
+    assert bench._valu_busy({"b": 1}) is None
+    r = bench.roofline(1e9, 1.0, {"a": 1})
+    assert r["valu_busy"] == {"a": 0.755} and r["traffic"] == 1.0
+
+
 def test_more_than_one_gpu_is_refused():
     r = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--gpus", "2"], capture_output=True,
                        text=True, timeout=120)
