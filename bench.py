@@ -34,11 +34,10 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
-VALU_PEAK_TOPS = 256 * 64 * 2.4e9 / 1e12  # 32-bit VALU lane-instructions per second (T)
-# VALU instructions per word and generation in the generation loop of the blocked GoL
-# kernels, counted in the gfx950 code object (K = 10 column: 176 for k_step_tb, 183 for
-# k_step_tb1, which also sums the two rows next to its band)
-GOL_VALU_PER_WORD = {"tb": 17.6, "tb1": 18.3}
+# VALU issue peak: 256 CUs x 64 lanes x 2.4 GHz lane-instructions per second (T); FP64
+# add/mul/FMA issue at this rate too on MI355X (78.6 TFLOP/s = this x 2 for FMA);
+# reciprocals and square roots take four issue cycles
+VALU_PEAK_TOPS = 256 * 64 * 2.4e9 / 1e12
 FDTD_COEFF = [0.5, 0.125, 0.0625, 0.125 / 3, 0.03125]
 
 
@@ -56,9 +55,12 @@ def _traffic(kernels):
         return None
 
 
-def _valu_busy(kernels):
-    """Fraction of the kernel's time the vector ALUs issue (rocprofv3's derived VALUBusy,
-    read from profiles/traffic.json) for each listed kernel that was profiled, or None."""
+def _valu(kernels, launch_ms):
+    """VALU issue rate of one roofline unit: the wave instructions rocprofv3 counted per
+    dispatch (SQ_INSTS_VALU, profiles/traffic.json) x 64 lanes, weighted like the bytes,
+    over the launch time measured here; plus rocprofv3's derived VALUBusy per kernel (the
+    share of the dispatch's time the vector ALUs issue; it includes the dispatch's ramp,
+    so it reads low for kernels of a few microseconds).  None when not profiled."""
     path = os.path.join(ROOT, "profiles", "traffic.json")
     if not os.path.exists(path):
         return None
@@ -66,9 +68,13 @@ def _valu_busy(kernels):
         tab = json.load(open(path))
     except ValueError:
         return None
-    got = {k: round(tab[k]["valu_busy_pct"] / 100.0, 4) for k in kernels
-           if isinstance(tab.get(k), dict) and "valu_busy_pct" in tab[k]}
-    return got or None
+    if not all(isinstance(tab.get(k), dict) and "valu_wave_insts_per_launch" in tab[k] for k in kernels):
+        return None
+    insts = sum(w * float(tab[k]["valu_wave_insts_per_launch"]) for k, w in kernels.items())
+    tops = insts * 64 / (launch_ms * 1e-3) / 1e12
+    return {"wave_insts_per_unit": insts, "achieved": round(tops, 2), "peak": VALU_PEAK_TOPS,
+            "unit": "T lane-instr/s", "frac": round(tops / VALU_PEAK_TOPS, 4),
+            "valu_busy": {k: round(tab[k]["valu_busy_pct"] / 100.0, 4) for k in kernels if "valu_busy_pct" in tab[k]}}
 
 
 def roofline(alg_bytes_per_launch, launch_ms, kernels):
@@ -79,7 +85,7 @@ def roofline(alg_bytes_per_launch, launch_ms, kernels):
     return {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": _traffic(kernels),
             "kernel": " + ".join(kernels), "alg_bytes_per_launch": alg_bytes_per_launch,
-            "avg_launch_us": round(launch_ms * 1e3, 3), "valu_busy": _valu_busy(kernels)}
+            "avg_launch_us": round(launch_ms * 1e3, 3), "valu": _valu(kernels, launch_ms)}
 
 
 def timed(ctx, fn):
@@ -130,20 +136,8 @@ def bench_gol(ctx, hip, steps, warmup, variant, nx=8192, ny=8192):
             kname = f"k_step_{tname.split()[0]}<{k}, {gg}, {w}>"
         else:
             kname = "k_step<8>"
+        # (the blocked kernel is bound by integer VALU issue, not HBM: see roofline.valu)
         res["roofline"] = roofline(2.0 * nx * ny / 8, launch_ms * g, kname)
-        if tname:
-            # the blocked kernel is bound by integer VALU issue, not HBM: lane-instructions
-            # it issues per launch (every window word, every generation; GOL_VALU_PER_WORD[kind]
-            # instructions per word and generation in its generation loop, counted in the
-            # gfx950 code object) against 256 CUs x 64 lanes x 2.4 GHz
-            wx = (nx // 32) // 64
-            out_rows = (w // wx) * k - 2 * gg
-            words = -(-ny // out_rows) * 64 * w * k * gg
-            per_word = GOL_VALU_PER_WORD[tname.split()[0]]
-            tops = words * per_word / (launch_ms * g * 1e-3) / 1e12
-            res["roofline"]["valu"] = {"lane_instr_per_word_gen": per_word, "achieved": round(tops, 2),
-                                       "peak": VALU_PEAK_TOPS, "unit": "T lane-instr/s",
-                                       "frac": round(tops / VALU_PEAK_TOPS, 4)}
     d0.free()
     d1.free()
     return res
@@ -672,7 +666,7 @@ def main():
     if not args.headline_only:
         only = set(x for x in args.only.split(",") if x) or {"heat", "heat_ensemble", "fdtd", "himeno", "flood"}
         quick = {"heat": {"iters": 40}, "heat_ensemble": {"iters": 40}, "fdtd": {"steps": 4}, "himeno": {"iters": 10},
-                 "flood": {"steps": 40}} if args.quick else {}
+                 "flood": {"steps": 300}} if args.quick else {}
         for name, fn in (("heat", bench_heat), ("heat_ensemble", bench_heat_ensemble), ("fdtd", bench_fdtd),
                          ("himeno", bench_himeno), ("flood", bench_flood)):
             if name in only:
