@@ -415,13 +415,16 @@ __device__ inline void two_sum(double& s, double& cmp, double x)
 
 // K2: wet/dry fix + per-tile CFL minimum and compensated volume partial (no atomics;
 // one tile of BX x FBY cells per block, partials[3t..3t+2] = (min, sum, compensation)).
-// Each thread folds its FBY / BY cells (rows BY apart) into its own minimum and
-// compensated sum before the one wave butterfly, so the FP64 reduction is paid once
-// per thread rather than once per cell.  MODE 0: initial CFL only (no fix); MODE 1:
-// end of a step.
+// Each thread owns RPT = FBY / BY consecutive rows of one column.  It loads h and z of
+// rows jb-1 .. jb+RPT (its cells and their vertical neighbours), its cells' momenta and
+// their horizontal neighbours before the first use, so all of its loads share one
+// memory round trip, then folds its cells into its own minimum and compensated sum
+// before the one wave butterfly (the FP64 reduction is paid once per thread, not once
+// per cell).  MODE 0: initial CFL only (no fix); MODE 1: end of a step.
 template <typename R, int MODE>
 __global__ __launch_bounds__(BT) void k_fix_cfl(const Scen<R>* __restrict__ sc, int nscen, int par, Params P)
 {
+    constexpr int RPT = FBY / BY;
     __shared__ R smin[BT / 64];
     __shared__ double ssum[BT / 64][2];
     const int tile = blockIdx.x;
@@ -430,37 +433,66 @@ __global__ __launch_bounds__(BT) void k_fix_cfl(const Scen<R>* __restrict__ sc, 
     const int t = tile - S.tile0;
     const int tyi = t / S.tiles_x, txi = t - tyi * S.tiles_x;
     const int i = txi * BX + (threadIdx.x & (BX - 1)) + 1;
-    const int jb = tyi * FBY + (threadIdx.x >> 6) + 1;
+    const int jb = tyi * FBY + (threadIdx.x >> 6) * RPT + 1;  // the thread's first row
     const R G = (R)9.81, EPS12 = (R)1e-12;
     const R hextra = (R)P.hextra, dx = (R)P.dx, cn = (R)P.cn;
     R cfl = RealLim<R>::max();
     double vs = 0.0, vc = 0.0;
-    const int W = S.nx + 2;
+    const int nx = S.nx, ny = S.ny, W = nx + 2;
     const R* h = S.h[par];
     R* qx = S.qx[par];
     R* qy = S.qy[par];
     const R* z = S.z;
+    const bool col = i <= nx;
+    // hv/zv[r]: row jb-1+r (rows up to ny+1 exist); qxv/qyv, and h/z left (l) and right
+    // (r) of the cells, for the thread's rows jb .. jb+RPT-1
+    R hv[RPT + 2], zv[RPT + 2], qxv[RPT], qyv[RPT], hl[RPT], hr[RPT], zl[RPT], zr[RPT];
 #pragma unroll
-    for (int k = 0; k < FBY / BY; k++) {
-        const int j = jb + k * BY;
-        if (i > S.nx || j > S.ny) continue;
-        const size_t c = (size_t)i + (size_t)W * (size_t)j;
-        const R hij = h[c];
-        R qxv = qx[c], qyv = qy[c];
-        if (MODE == 1 && hij > hextra) {
-            const R zij = z[c];
-            if (((hij + zij < z[c + 1]) && (h[c + 1] < EPS12)) || ((hij + zij < z[c - 1]) && (h[c - 1] < EPS12))) {
-                qxv = (R)0.0;
-                qx[c] = qxv;
+    for (int r = 0; r < RPT + 2; r++) {
+        const int row = jb - 1 + r;
+        hv[r] = (R)0.0;
+        zv[r] = (R)0.0;
+        if (col && row <= ny + 1) {
+            const size_t c = (size_t)i + (size_t)W * (size_t)row;
+            hv[r] = h[c];
+            if (MODE == 1) zv[r] = z[c];
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < RPT; k++) {
+        qxv[k] = qyv[k] = hl[k] = hr[k] = zl[k] = zr[k] = (R)0.0;
+        if (col && jb + k <= ny) {
+            const size_t c = (size_t)i + (size_t)W * (size_t)(jb + k);
+            qxv[k] = qx[c];
+            qyv[k] = qy[c];
+            if (MODE == 1) {
+                hl[k] = h[c - 1];
+                hr[k] = h[c + 1];
+                zl[k] = z[c - 1];
+                zr[k] = z[c + 1];
             }
-            if (((hij + zij < z[c - W]) && (h[c - W] < EPS12)) || ((hij + zij < z[c + W]) && (h[c + W] < EPS12))) {
-                qyv = (R)0.0;
-                qy[c] = qyv;
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < RPT; k++) {
+        const int j = jb + k;
+        if (!col || j > ny) continue;
+        const size_t c = (size_t)i + (size_t)W * (size_t)j;
+        const R hij = hv[k + 1];
+        if (MODE == 1 && hij > hextra) {
+            const R zij = zv[k + 1];
+            if (((hij + zij < zr[k]) && (hr[k] < EPS12)) || ((hij + zij < zl[k]) && (hl[k] < EPS12))) {
+                qxv[k] = (R)0.0;
+                qx[c] = qxv[k];
+            }
+            if (((hij + zij < zv[k]) && (hv[k] < EPS12)) || ((hij + zij < zv[k + 2]) && (hv[k + 2] < EPS12))) {
+                qyv[k] = (R)0.0;
+                qy[c] = qyv[k];
             }
         }
         if (hij > hextra) {
-            const R dtx = fdiv(dx, fabs(fdiv(qxv, hij)) + fsqrt(G * hij));
-            const R dty = fdiv(dx, fabs(fdiv(qyv, hij)) + fsqrt(G * hij));
+            const R dtx = fdiv(dx, fabs(fdiv(qxv[k], hij)) + fsqrt(G * hij));
+            const R dty = fdiv(dx, fabs(fdiv(qyv[k], hij)) + fsqrt(G * hij));
             const R cc = dtx < dty ? cn * dtx : cn * dty;
             cfl = cc < cfl ? cc : cfl;
         }

@@ -123,7 +123,7 @@ def bench_gol(ctx, hip, steps, warmup, variant, nx=8192, ny=8192):
            "glups": round(nx * ny * steps / (ms * 1e-3) / 1e9, 2), "live_count": live}
     if variant == hip.PLAIN:
         # uint8 field read + written once per generation
-        res["roofline"] = roofline(2.0 * nx * ny, launch_ms, f"k_gol_plain<{(nx + 2) & 15}>")
+        res["roofline"] = roofline(2.0 * nx * ny, launch_ms, f"k_gol_plain<{(nx + 2) & 15}, 16>")
     else:
         # bit-packed field (1 bit per cell) read + written once per launch; the blocked
         # kernel advances G generations per launch (its autotuned name says "gG")
