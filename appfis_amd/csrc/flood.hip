@@ -38,7 +38,6 @@ using namespace appfis;
 namespace {
 
 constexpr int BX = 64, BY = 4, BT = BX * BY;
-constexpr int EBY = 8;  // rows of a flux-kernel tile (BX x EBY cells, BT threads; 16 measured slower)
 constexpr int FBY = 16; // rows of a wet/dry + CFL + volume tile (BX x FBY cells, FBY / BY per thread)
 constexpr int CHUNK = 20;  // steps per captured graph (even: buffer parity returns)
 
@@ -68,7 +67,7 @@ struct Scen {
     int tiles_x;
     int tile0;         // first global tile of this scenario (BX x FBY tiles: wet/dry, CFL, volume)
     int ntiles;
-    int etiles_x;      // the flux kernel's BX x EBY tiles
+    int etiles_x;      // the flux kernel's MW x MR strips
     int etile0;
     int entiles;
     ScenState* st;
@@ -140,19 +139,28 @@ struct Normal {
 
 // Roe flux with entropy fix, bed-slope source and wet/dry treatment -- the shared body
 // of the X functor (triton_appfis.hpp:608-824, DIR 0) and the Y functor (827-1043, DIR 1).
+// Cell velocity of the functors (triton_appfis.hpp:620-631): q/h where h >= hextra, else 0.
+template <typename R> __device__ inline void velocity(R h, R qx, R qy, R hextra, R& u, R& v)
+{
+    if (h >= hextra) {
+        u = fdiv(qx, h);
+        v = fdiv(qy, h);
+    } else {
+        u = (R)0.0;
+        v = (R)0.0;
+    }
+}
+
+// The edge solve takes each side's velocities (u, v) and sqrt(h) precomputed per cell:
+// the values are the functors' own, computed once per cell instead of once per edge.
 template <typename R, int DIR>
-__device__ inline bool roe(R h1, R h2, R qx1, R qy1, R z1, R qx2, R qy2, R z2, R sqh1, R sqh2, R dx, R dt, R hextra,
-                           R d1[3], R d2[3])
+__device__ inline bool roe(R h1, R h2, R qx1, R qy1, R z1, R qx2, R qy2, R z2, R sqh1, R sqh2, R u1, R v1, R u2, R v2,
+                           R dx, R dt, R d1[3], R d2[3])
 {
     using N = Normal<R, DIR>;
     const R G = (R)9.81, SQRTG = (R)3.132091953, EPS12 = (R)1e-12;
     if (!(h1 > (R)0.0 || h2 > (R)0.0)) return false;
     R alpha[3], beta[3], eigen[3], eigenE[3], eigev[3][3];
-    R u1, u2, v1, v2;
-    if (h1 >= hextra) { u1 = fdiv(qx1, h1); v1 = fdiv(qy1, h1); }
-    else { u1 = (R)0.0; v1 = (R)0.0; }
-    if (h2 >= hextra) { u2 = fdiv(qx2, h2); v2 = fdiv(qy2, h2); }
-    else { u2 = (R)0.0; v2 = (R)0.0; }
     const R h12 = (R)0.5 * (h1 + h2);
     const R dh = h2 - h1;
     const R c12 = fsqrt(G * h12);
@@ -266,142 +274,188 @@ __device__ inline int find_scen(const Scen<R>* sc, int nscen, int tile)
     return lo;
 }
 
-// K1 on a BX x EBY tile, each edge solved once: phase X solves the (BX+1) x EBY edges
-// between horizontal neighbours into LDS and every cell picks up its two (the edge to
-// its right gives rhs*0, the edge to its left rhs*1); phase Y does the same with the
-// BX x (EBY+1) edges between vertical neighbours (the edge below gives rhs*0 +=, the
-// edge above rhs*1 +=).  The arguments of every Roe solve and the order of the rhs
-// accumulation are those of the per-cell form, so the results are identical; each
-// edge is solved once instead of twice.
-template <typename R>
-__device__ inline void edge_store(R (*buf)[BX * (EBY + 1)], unsigned char* flag, int e, bool ok, const R d1[3],
-                                  const R d2[3])
+// Lane shifts across the wave (DPP wave_shr:1 / wave_shl:1): from_left gives lane l the
+// value of lane l-1, from_right the value of lane l+1 (lanes 0 / 63 keep their own).
+__device__ __forceinline__ int from_left(int v) { return __builtin_amdgcn_update_dpp(v, v, 0x138, 0xF, 0xF, false); }
+__device__ __forceinline__ int from_right(int v) { return __builtin_amdgcn_update_dpp(v, v, 0x130, 0xF, 0xF, false); }
+template <bool LEFT> __device__ __forceinline__ float lane_shift(float v)
 {
-    flag[e] = ok ? 1 : 0;
-#pragma unroll
-    for (int k = 0; k < 3; k++) {
-        buf[k][e] = d1[k];
-        buf[3 + k][e] = d2[k];
+    const int b = __float_as_int(v);
+    return __int_as_float(LEFT ? from_left(b) : from_right(b));
+}
+template <bool LEFT> __device__ __forceinline__ double lane_shift(double v)
+{
+    const unsigned long long b = (unsigned long long)__double_as_longlong(v);
+    const int lo = (int)(unsigned)b, hi = (int)(unsigned)(b >> 32);
+    const unsigned l2 = (unsigned)(LEFT ? from_left(lo) : from_right(lo));
+    const unsigned h2 = (unsigned)(LEFT ? from_left(hi) : from_right(hi));
+    return __longlong_as_double((long long)(((unsigned long long)h2 << 32) | l2));
+}
+
+// One cell's state and the per-cell values every edge solve of it uses.
+template <typename R>
+struct CellV {
+    R h, qx, qy, z, n2, sq, u, v;
+};
+
+template <typename R>
+__device__ inline void cell_load(CellV<R>& c, const R* h, const R* qx, const R* qy, const R* z, const R* n2, bool in,
+                                 size_t off)
+{
+    c.h = c.qx = c.qy = c.z = c.n2 = (R)0.0;
+    if (in) {
+        c.h = h[off];
+        c.qx = qx[off];
+        c.qy = qy[off];
+        c.z = z[off];
+        c.n2 = n2[off];
     }
 }
 
 template <typename R>
-__global__ __launch_bounds__(BT) void k_flux_update(const Scen<R>* __restrict__ sc, int nscen, int par, Params P)
+__device__ inline void cell_derive(CellV<R>& c, R hextra)
 {
-    constexpr int NXE = (BX + 1) * EBY, NYE = BX * (EBY + 1);
-    constexpr int NE = NXE > NYE ? NXE : NYE;
-    static_assert(NE <= BX * (EBY + 1), "edge buffer size");
-    __shared__ R ebuf[6][BX * (EBY + 1)];  // d1[0..2], d2[0..2] per edge (structure of arrays)
-    __shared__ unsigned char eok[BX * (EBY + 1)];
-    constexpr int CPT = BX * EBY / BT;  // cells per thread
-    const int tile = blockIdx.x;
-    const int s = find_scen<R, true>(sc, nscen, tile);
+    c.sq = sqrt_pos(c.h);
+    velocity(c.h, c.qx, c.qy, hextra, c.u, c.v);
+}
+
+// K1 as a row march.  A wave owns a strip of MW columns x MR rows; lane l holds column
+// i0-1+l, so lanes 1..MW are the strip's cells and lanes 0 and 63 its halo columns.  The
+// wave walks the rows j0-1 .. j0+MR keeping two rows in registers:
+//   X edge of row j between columns (c-1, c): solved by the lane of c, with the left
+//     cell's values shifted in from lane l-1; the cell on the right of the edge takes
+//     rhs*1, the left cell's rhs*0 is shifted back to lane l-1;
+//   Y edge between rows j-1 (side 2) and j (side 1): solved by each cell lane from its
+//     two rows; row j takes rhs*0 +=, row j-1 rhs*1 +=, after which row j-1 is complete
+//     and is updated (implicit Manning friction, rain) into the other buffer.
+// Every edge is solved once; each cell's velocities and sqrt(h) once; the arguments of
+// every solve and the order of the rhs accumulation are those of the functors, so the
+// results are those of the per-cell form.  No LDS and no barriers.
+constexpr int MW = 62;            // strip columns (lanes 1..62)
+constexpr int MR = 32;            // strip rows
+constexpr int MWAVES = BT / 64;   // strips per block
+
+template <typename R>
+__global__ __launch_bounds__(BT) void k_flux_march(const Scen<R>* __restrict__ sc, int nscen, int nstrips, int par,
+                                                   Params P)
+{
+    const int lane = threadIdx.x & 63;
+    const int strip = blockIdx.x * MWAVES + (threadIdx.x >> 6);  // uniform per wave
+    if (strip >= nstrips) return;
+    const int s = find_scen<R, true>(sc, nscen, strip);
     const Scen<R>& S = sc[s];
-    const int t = tile - S.etile0;
-    const int tyi = t / S.etiles_x, txi = t - tyi * S.etiles_x;
-    const int i0 = txi * BX + 1, j0 = tyi * EBY + 1;  // first interior cell of the tile
-    const int nx = S.nx, ny = S.ny, W = nx + 2;
+    const int t = strip - S.etile0;
+    const int ty = t / S.etiles_x, tx = t - ty * S.etiles_x;
+    const int nx = S.nx, ny = S.ny;
+    const size_t W = (size_t)nx + 2;
+    const int col = tx * MW + lane;  // padded column of this lane
+    const int j0 = ty * MR + 1;
+    const bool colmem = col <= nx + 1;
+    const bool cell = lane >= 1 && lane <= MW && col <= nx;
+    const bool xedge = lane >= 1 && colmem;  // edge (col-1, col) with col-1 <= nx
     const R* h = S.h[par];
     const R* qx = S.qx[par];
     const R* qy = S.qy[par];
     const R* z = S.z;
+    const R* n2 = S.n2;
+    const int o = par ^ 1;
     const R dt = (R)S.st->dt;
     const R dx = (R)P.dx, hextra = (R)P.hextra, rain = (R)P.rain;
-    const int lx = threadIdx.x & (BX - 1);
-    R rh0[CPT], rqx0[CPT], rqy0[CPT], rh1[CPT], rqx1[CPT], rqy1[CPT];
-    // ---- X edges: edge (col, row) between cells (i0-1+col, j) and (i0+col, j)
-    for (int e = threadIdx.x; e < NXE; e += BT) {
-        const int row = e / (BX + 1), col = e - row * (BX + 1);
-        const int iL = i0 - 1 + col, j = j0 + row;
-        R d1[3] = {(R)0, (R)0, (R)0}, d2[3] = {(R)0, (R)0, (R)0};
-        bool ok = false;
-        if (j <= ny && iL <= nx) {
-            const size_t cL = (size_t)iL + (size_t)W * (size_t)j, cR = cL + 1;
-            const R hL = h[cL], hR = h[cR];
-            ok = roe<R, 0>(hL, hR, qx[cL], qy[cL], z[cL], qx[cR], qy[cR], z[cR], sqrt_pos(hL), sqrt_pos(hR), dx, dt,
-                           hextra, d1, d2);
-        }
-        edge_store(ebuf, eok, e, ok, d1, d2);
-    }
-    __syncthreads();
-#pragma unroll
-    for (int k = 0; k < CPT; k++) {
-        const int ly = (threadIdx.x >> 6) + k * (BT / BX);
-        const int eR = ly * (BX + 1) + lx + 1, eL = eR - 1;  // edges (i, i+1) and (i-1, i)
-        rh0[k] = (R)0.0; rqx0[k] = (R)0.0; rqy0[k] = (R)0.0;
-        rh1[k] = (R)0.0; rqx1[k] = (R)0.0; rqy1[k] = (R)0.0;
-        if (eok[eR]) {  // cell is side 1: rhs*0 = (triton_appfis.hpp:817-819)
-            rh0[k] = fdiv(dt * ebuf[0][eR], dx);
-            rqx0[k] = fdiv(dt * ebuf[1][eR], dx);
-            rqy0[k] = fdiv(dt * ebuf[2][eR], dx);
-        }
-        if (eok[eL]) {  // cell is side 2: rhs*1 = (triton_appfis.hpp:821-823)
-            rh1[k] = fdiv(dt * ebuf[3][eL], dx);
-            rqx1[k] = fdiv(dt * ebuf[4][eL], dx);
-            rqy1[k] = fdiv(dt * ebuf[5][eL], dx);
-        }
-    }
-    __syncthreads();
-    // ---- Y edges: edge (col, row) between cells (i, jU) (side 1) and (i, jU-1) (side 2),
-    //      jU = j0 + row
-    for (int e = threadIdx.x; e < NYE; e += BT) {
-        const int row = e / BX, col = e - row * BX;
-        const int i = i0 + col, jU = j0 + row;
-        R d1[3] = {(R)0, (R)0, (R)0}, d2[3] = {(R)0, (R)0, (R)0};
-        bool ok = false;
-        if (i <= nx && jU <= ny + 1) {
-            const size_t cU = (size_t)i + (size_t)W * (size_t)jU, cD = cU - W;
-            const R hU = h[cU], hD = h[cD];
-            ok = roe<R, 1>(hU, hD, qx[cU], qy[cU], z[cU], qx[cD], qy[cD], z[cD], sqrt_pos(hU), sqrt_pos(hD), dx, dt,
-                           hextra, d1, d2);
-        }
-        edge_store(ebuf, eok, e, ok, d1, d2);
-    }
-    __syncthreads();
     const R G = (R)9.81, EPS12 = (R)1e-12;
-    const int o = par ^ 1;
+    const int rend = min(MR, ny + 1 - j0);  // last row offset: rows up to ny+1 exist
+
+    CellV<R> prev, cur, nxt;
+    cell_load(prev, h, qx, qy, z, n2, colmem, (size_t)col + W * (size_t)(j0 - 1));
+    cell_load(cur, h, qx, qy, z, n2, colmem, (size_t)col + W * (size_t)j0);
+    cell_derive(prev, hextra);
+    // rhs*0 / rhs*1 of the previous row's cell (h, qx, qy)
+    R p0[3] = {(R)0, (R)0, (R)0}, p1[3] = {(R)0, (R)0, (R)0};
+    for (int r = 0; r <= rend; r++) {
+        const int j = j0 + r;
+        // prefetch row j+1 while row j is solved
+        cell_load(nxt, h, qx, qy, z, n2, colmem && r < rend, (size_t)col + W * (size_t)(j + 1));
+        cell_derive(cur, hextra);
+        R c0[3] = {(R)0, (R)0, (R)0}, c1[3] = {(R)0, (R)0, (R)0};
+        if (r < MR) {  // uniform: row j is a strip row
+            CellV<R> L;
+            L.h = lane_shift<true>(cur.h);
+            L.qx = lane_shift<true>(cur.qx);
+            L.qy = lane_shift<true>(cur.qy);
+            L.z = lane_shift<true>(cur.z);
+            L.sq = lane_shift<true>(cur.sq);
+            L.u = lane_shift<true>(cur.u);
+            L.v = lane_shift<true>(cur.v);
+            R d1[3] = {(R)0, (R)0, (R)0}, d2[3] = {(R)0, (R)0, (R)0};
+            bool ok = false;
+            if (xedge && j <= ny)
+                ok = roe<R, 0>(L.h, cur.h, L.qx, L.qy, L.z, cur.qx, cur.qy, cur.z, L.sq, cur.sq, L.u, L.v, cur.u,
+                               cur.v, dx, dt, d1, d2);
+            // side 2 of the edge on its left: rhs*1 = (triton_appfis.hpp:821-823)
+            if (ok) {
 #pragma unroll
-    for (int k = 0; k < CPT; k++) {
-        const int ly = (threadIdx.x >> 6) + k * (BT / BX);
-        const int i = i0 + lx, j = j0 + ly;
-        if (i > nx || j > ny) continue;
-        const int eD = ly * BX + lx, eU = eD + BX;  // edges (j, j-1) and (j+1, j)
-        if (eok[eD]) {  // cell is side 1: rhs*0 += (triton_appfis.hpp:1034-1036)
-            rh0[k] = rh0[k] + fdiv(dt * ebuf[0][eD], dx);
-            rqx0[k] = rqx0[k] + fdiv(dt * ebuf[1][eD], dx);
-            rqy0[k] = rqy0[k] + fdiv(dt * ebuf[2][eD], dx);
-        }
-        if (eok[eU]) {  // cell is side 2: rhs*1 += (triton_appfis.hpp:1038-1040)
-            rh1[k] = rh1[k] + fdiv(dt * ebuf[3][eU], dx);
-            rqx1[k] = rqx1[k] + fdiv(dt * ebuf[4][eU], dx);
-            rqy1[k] = rqy1[k] + fdiv(dt * ebuf[5][eU], dx);
-        }
-        // update with implicit Manning friction (triton_appfis.hpp:1046-1092)
-        const size_t c = (size_t)i + (size_t)W * (size_t)j;
-        const R hn = h[c], qxc = qx[c], qyc = qy[c];
-        R hij, qxij, qyij;
-        hij = hn - rh0[k] - rh1[k];
-        if (hij < (R)0.0) hij = (R)0.0;
-        if (hij < hextra) {
-            qxij = (R)0.0;
-            qyij = (R)0.0;
-        } else {
-            const R mx = qxc - rqx0[k] - rqx1[k];
-            const R my = qyc - rqy0[k] - rqy1[k];
-            const R modM = fsqrt(mx * mx + my * my);
-            const R nn = S.n2[c];
-            if (nn > EPS12 && hn >= hextra && modM > EPS12) {
-                const R tt = fdiv(dt * G * nn * modM, hn * hn * cbrt(hn));
-                qxij = fdiv((R)-0.5 * (mx - mx * fsqrt((R)1.0 + (R)4.0 * tt)), tt);
-                qyij = fdiv((R)-0.5 * (my - my * fsqrt((R)1.0 + (R)4.0 * tt)), tt);
-            } else {
-                qxij = mx;
-                qyij = my;
+                for (int k = 0; k < 3; k++) c1[k] = fdiv(dt * d2[k], dx);
+            }
+            // side 1 of the edge on its right (solved by lane l+1): rhs*0 = (817-819)
+            const int okr = from_right(ok ? 1 : 0);
+            R d1r[3];
+#pragma unroll
+            for (int k = 0; k < 3; k++) d1r[k] = lane_shift<false>(d1[k]);
+            if (okr && lane < 63) {
+#pragma unroll
+                for (int k = 0; k < 3; k++) c0[k] = fdiv(dt * d1r[k], dx);
             }
         }
-        S.h[o][c] = hij + rain * dt;
-        S.qx[o][c] = qxij;
-        S.qy[o][c] = qyij;
+        {
+            // Y edge (j, j-1): row j is side 1 (rhs*0 +=, 1034-1036), row j-1 side 2
+            // (rhs*1 +=, 1038-1040)
+            R d1[3] = {(R)0, (R)0, (R)0}, d2[3] = {(R)0, (R)0, (R)0};
+            bool ok = false;
+            if (cell)
+                ok = roe<R, 1>(cur.h, prev.h, cur.qx, cur.qy, cur.z, prev.qx, prev.qy, prev.z, cur.sq, prev.sq, cur.u,
+                               cur.v, prev.u, prev.v, dx, dt, d1, d2);
+            if (ok) {
+#pragma unroll
+                for (int k = 0; k < 3; k++) {
+                    c0[k] = c0[k] + fdiv(dt * d1[k], dx);
+                    p1[k] = p1[k] + fdiv(dt * d2[k], dx);
+                }
+            }
+        }
+        // row j-1 is complete: update with implicit Manning friction (1046-1092) and rain
+        if (r >= 1 && cell && j - 1 <= ny) {
+            const size_t c = (size_t)col + W * (size_t)(j - 1);
+            const R hn = prev.h, qxc = prev.qx, qyc = prev.qy;
+            R hij, qxij, qyij;
+            hij = hn - p0[0] - p1[0];
+            if (hij < (R)0.0) hij = (R)0.0;
+            if (hij < hextra) {
+                qxij = (R)0.0;
+                qyij = (R)0.0;
+            } else {
+                const R mx = qxc - p0[1] - p1[1];
+                const R my = qyc - p0[2] - p1[2];
+                const R modM = fsqrt(mx * mx + my * my);
+                const R nn = prev.n2;
+                if (nn > EPS12 && hn >= hextra && modM > EPS12) {
+                    const R tt = fdiv(dt * G * nn * modM, hn * hn * cbrt(hn));
+                    qxij = fdiv((R)-0.5 * (mx - mx * fsqrt((R)1.0 + (R)4.0 * tt)), tt);
+                    qyij = fdiv((R)-0.5 * (my - my * fsqrt((R)1.0 + (R)4.0 * tt)), tt);
+                } else {
+                    qxij = mx;
+                    qyij = my;
+                }
+            }
+            S.h[o][c] = hij + rain * dt;
+            S.qx[o][c] = qxij;
+            S.qy[o][c] = qyij;
+        }
+        prev = cur;
+        cur = nxt;
+#pragma unroll
+        for (int k = 0; k < 3; k++) {
+            p0[k] = c0[k];
+            p1[k] = c1[k];
+        }
     }
 }
 
@@ -625,11 +679,11 @@ int run_flood(appfis_ctx* ctx, int nscen, const int* nx, const int* ny, void* co
         hs[s].tile0 = (int)total_tiles;
         hs[s].ntiles = tx * ty;
         total_tiles += (long long)tx * ty;
-        const int ety = (int)ceil_div(ny[s], EBY);
-        hs[s].etiles_x = tx;
+        const int etx = (int)ceil_div(nx[s], MW), ety = (int)ceil_div(ny[s], MR);
+        hs[s].etiles_x = etx;
         hs[s].etile0 = (int)total_etiles;
-        hs[s].entiles = tx * ety;
-        total_etiles += (long long)tx * ety;
+        hs[s].entiles = etx * ety;
+        total_etiles += (long long)etx * ety;
         field_bytes += ((size_t)(nx[s] + 2) * (ny[s] + 2) * sizeof(R) + 255) / 256 * 256;
     }
     APPFIS_REQUIRE(total_tiles < (1LL << 31) && total_etiles < (1LL << 31), "too many tiles");
@@ -679,14 +733,15 @@ int run_flood(appfis_ctx* ctx, int nscen, const int* nx, const int* ny, void* co
     Params P{prm->dx, prm->courant, prm->hextra, prm->dt_fallback, prm->print_interval, prm->rain_rate, rec};
     hipLaunchKernelGGL(k_init_state, dim3(ceil_div(nscen, 64)), dim3(64), 0, ctx->stream, st, nscen, prm->start_time);
     APPFIS_LAUNCH_CHECK();
-    const dim3 grid((unsigned)total_tiles), egrid((unsigned)total_etiles);
+    const dim3 grid((unsigned)total_tiles), egrid((unsigned)ceil_div(total_etiles, MWAVES));
+    const int nstrips = (int)total_etiles;
     // dt of the first step from the initial state
     hipLaunchKernelGGL((k_fix_cfl<R, 0>), grid, dim3(BT), 0, ctx->stream, tab, nscen, 0, P);
     APPFIS_LAUNCH_CHECK();
     hipLaunchKernelGGL((k_finalize<R, 0>), dim3(nscen), dim3(BT), 0, ctx->stream, tab, P);
     APPFIS_LAUNCH_CHECK();
     auto step_pair = [&](int par) -> int {
-        hipLaunchKernelGGL((k_flux_update<R>), egrid, dim3(BT), 0, ctx->stream, tab, nscen, par, P);
+        hipLaunchKernelGGL((k_flux_march<R>), egrid, dim3(BT), 0, ctx->stream, tab, nscen, nstrips, par, P);
         APPFIS_LAUNCH_CHECK();
         hipLaunchKernelGGL((k_fix_cfl<R, 1>), grid, dim3(BT), 0, ctx->stream, tab, nscen, par ^ 1, P);
         APPFIS_LAUNCH_CHECK();
