@@ -3,16 +3,18 @@
 // rainfall (added as h += rain*dt right after the update, where the reference adds its
 // point sources), fused into two kernels per step:
 //
-//   K1 (flux + update + rain): a 64 x 8 tile solves each of its edges once (X edges,
-//       then Y edges, through LDS), every cell accumulates rhs*0 / rhs*1 from its four
-//       edges in the order of the reference's X-then-Y functors, applies the update
-//       with implicit Manning friction and the rain, and writes the new state to the
-//       other buffer of a ping-pong pair (the reference's reset pass and six rhs
-//       arrays never touch memory).
+//   K1 (flux + update + rain): a wave marches up a strip of 62 columns x 32 rows, one
+//       lane per column, solving each edge once (X edges with the left cell's values
+//       shifted in across lanes, Y edges from the row below kept in registers); every
+//       cell accumulates rhs*0 / rhs*1 from its four edges in the order of the
+//       reference's X-then-Y functors, applies the update with implicit Manning
+//       friction and the rain, and writes the new state to the other buffer of a
+//       ping-pong pair (the reference's reset pass and six rhs arrays never touch
+//       memory).
 //   K2 (wet/dry fix + CFL + volume): the momentum fix of triton_appfis.hpp:1095-1112 in
 //       place, then the CFL candidate of triton_appfis.hpp:567-586 from the fixed state
-//       (wave/block min) and a compensated FP64 sum of h, written as one partial per
-//       tile -- no atomics (per-scenario atomics serialise hundreds of thousands of
+//       (thread/wave/block min) and a compensated FP64 sum of h, written as one partial
+//       per 64 x 16 tile -- no atomics (per-scenario atomics serialise hundreds of thousands of
 //       blocks on a handful of addresses).
 //   K3 (finalise): one block per scenario combines its tile partials in a fixed order
 //       and does on the device what the reference does on the host
@@ -22,9 +24,9 @@
 //
 // No host round trip per step: the reference's Execute2DReduce(MIN) + D2H + 3 host
 // syncs per step become device-side state, so a chunk of steps can be captured in a
-// hipGraph and replayed.  Ensembles: every phase is one launch over a tile work-list
-// spanning all scenarios (64 x 4 cells per 256-thread block, scenario found by binary
-// search over tile offsets), with per-scenario dt and reductions.
+// hipGraph and replayed.  Ensembles: every phase is one launch over a work-list of
+// strips / tiles spanning all scenarios (the scenario is found by binary search over
+// per-scenario offsets), with per-scenario dt and reductions.
 //
 // REAL = double (reference dtype) or float (FP32 bulk form; the volume is still a
 // compensated FP64 sum).  No FMA contraction: the operation order is the functors'.
