@@ -349,6 +349,9 @@ static const TbVariant kTb[] = {
     {"tb1 k10 g4 w16", k_step_tb1<10, 4, 16>, 10, 4, 16}, {"tb1 k11 g5 w16", k_step_tb1<11, 5, 16>, 11, 5, 16},
     {"tb1 k13 g6 w16", k_step_tb1<13, 6, 16>, 13, 6, 16}, {"tb1 k16 g8 w16", k_step_tb1<16, 8, 16>, 16, 8, 16},
     {"tb1 k20 g4 w8", k_step_tb1<20, 4, 8>, 20, 4, 8},    {"tb1 k12 g4 w16", k_step_tb1<12, 4, 16>, 12, 4, 16},
+    // 4K - 2G = 32 output rows per block like k10 g4 (256 blocks fill 256 CUs at 8192
+    // rows), with more generations per launch to spread the launch's fixed load / store
+    {"tb1 k11 g6 w16", k_step_tb1<11, 6, 16>, 11, 6, 16}, {"tb1 k12 g8 w16", k_step_tb1<12, 8, 16>, 12, 8, 16},
 };
 
 }  // namespace golbits
