@@ -139,7 +139,10 @@ __global__ __launch_bounds__(64 * NWAVES) void k_step_tb(const uint32_t* __restr
     const int w = wx * 64 + lane;
     const int left = wy * WX + (wx == 0 ? WX - 1 : wx - 1);
     const int right = wy * WX + (wx == WX - 1 ? 0 : wx + 1);
-    const int y0 = (int)blockIdx.x * OUT - G + wy * K;  // global row of register row 0
+    // bands in XCD-contiguous order: neighbouring bands (whose rows form each other's halo)
+    // run on one XCD and share its L2
+    const int band = (int)xcd_tile(blockIdx.x, gridDim.x);
+    const int y0 = band * OUT - G + wy * K;  // global row of register row 0
     // periodic row of register row 0 (the window may start up to G rows below 0 and end
     // past ny); then consecutive rows step by one word row, wrapping once at ny when
     // K <= ny (the general loop otherwise).  32-bit word offsets: NW * ny < 2^31.
@@ -222,8 +225,8 @@ __global__ __launch_bounds__(64 * NWAVES) void k_step_tb(const uint32_t* __restr
             R[k] = (s0 & s1 & ~s2) | (R[k] & s2 & ~(s0 | s1));
         }
     }
-    // output rows: window rows G .. G+OUT-1 of the block, global rows blockIdx.x*OUT + r
-    const int ybase = (int)blockIdx.x * OUT - G + wy * K;
+    // output rows: window rows G .. G+OUT-1 of the block, global rows band*OUT + r
+    const int ybase = band * OUT - G + wy * K;
 #pragma unroll
     for (int k = 0; k < K; k++) {
         const int r = wy * K + k - G;  // output row within the block's OUT rows
@@ -251,7 +254,8 @@ __global__ __launch_bounds__(64 * NWAVES) void k_step_tb1(const uint32_t* __rest
     const int w = wx * 64 + lane;
     const int left = wy * WX + (wx == 0 ? WX - 1 : wx - 1);
     const int right = wy * WX + (wx == WX - 1 ? 0 : wx + 1);
-    const int y0 = (int)blockIdx.x * OUT - G + wy * K;
+    const int band = (int)xcd_tile(blockIdx.x, gridDim.x);  // XCD-contiguous band order
+    const int y0 = band * OUT - G + wy * K;
     int yw = y0 % ny;
     if (yw < 0) yw += ny;
     uint32_t R[K];
@@ -324,7 +328,7 @@ __global__ __launch_bounds__(64 * NWAVES) void k_step_tb1(const uint32_t* __rest
             R[k] = (s0 & s1 & ~s2) | (R[k] & s2 & ~(s0 | s1));
         }
     }
-    const int ybase = (int)blockIdx.x * OUT - G + wy * K;
+    const int ybase = band * OUT - G + wy * K;
 #pragma unroll
     for (int k = 0; k < K; k++) {
         const int r = wy * K + k - G;
