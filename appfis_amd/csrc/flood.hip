@@ -483,7 +483,7 @@ __global__ __launch_bounds__(BT) void k_fix_cfl(const Scen<R>* __restrict__ sc, 
     constexpr int RPT = FBY / BY;
     __shared__ R smin[BT / 64];
     __shared__ double ssum[BT / 64][2];
-    const int tile = blockIdx.x;
+    const int tile = (int)xcd_tile(blockIdx.x, gridDim.x);  // XCD-contiguous: halo rows share an L2
     const int s = find_scen(sc, nscen, tile);
     const Scen<R>& S = sc[s];
     const int t = tile - S.tile0;
