@@ -73,7 +73,7 @@ struct Scen {
     int etile0;
     int entiles;
     ScenState* st;
-    double* partials;  // 2 doubles per tile (compensated volume partial)
+    double* partials;  // 3 doubles per tile: CFL minimum, compensated volume sum and its error
     double* rec_time;  // per-step records (nsteps each) or null
     double* rec_vol;
     double* rec_dt;
