@@ -88,6 +88,58 @@ __device__ __forceinline__ unsigned xcd_tile(unsigned b, unsigned nb)
     return x * q + (x < r ? x : r) + (b >> 3);
 }
 
+// Wave64 reductions through DPP (the cross-lane moves are operand modifiers of VALU
+// moves; nothing goes through the LDS crossbar).  The tree is fixed: quad swaps (lane
+// xor 1, xor 2), the half-row mirror (i <-> 7-i) and the row mirror (i <-> 15-i), after
+// which every lane holds its 16-lane row's total; then row_bcast:15 adds row 0 into
+// row 1 and row 2 into row 3, and row_bcast:31 adds row 1 into row 3.  The wave total
+// ends in lane 63 (WAVE_TOTAL_LANE).  Lanes that a move does not write receive the
+// identity, so the other lanes hold partial values.  Every lane of the wave must be
+// active.
+constexpr int WAVE_TOTAL_LANE = 63;
+
+template <int CTRL, int ROWS, typename T>
+__device__ __forceinline__ T dpp_move(T identity, T v)
+{
+    static_assert(sizeof(T) % 4 == 0, "DPP moves 32-bit words");
+    constexpr int NW = (int)(sizeof(T) / 4);
+    int vi[NW], oi[NW], ri[NW];
+    __builtin_memcpy(vi, &v, sizeof(T));
+    __builtin_memcpy(oi, &identity, sizeof(T));
+#pragma unroll
+    for (int w = 0; w < NW; w++) ri[w] = __builtin_amdgcn_update_dpp(oi[w], vi[w], CTRL, ROWS, 0xF, false);
+    T r;
+    __builtin_memcpy(&r, ri, sizeof(T));
+    return r;
+}
+
+template <typename T, typename Op>
+__device__ __forceinline__ T wave_reduce(T v, T identity, Op op)
+{
+    v = op(v, dpp_move<0xB1, 0xF>(identity, v));   // quad_perm [1,0,3,2]
+    v = op(v, dpp_move<0x4E, 0xF>(identity, v));   // quad_perm [2,3,0,1]
+    v = op(v, dpp_move<0x141, 0xF>(identity, v));  // row_half_mirror
+    v = op(v, dpp_move<0x140, 0xF>(identity, v));  // row_mirror
+    v = op(v, dpp_move<0x142, 0xA>(identity, v));  // row_bcast:15 into rows 1 and 3
+    v = op(v, dpp_move<0x143, 0xC>(identity, v));  // row_bcast:31 into rows 2 and 3
+    return v;
+}
+
+// The value of lane WAVE_TOTAL_LANE, in every lane.
+template <typename T>
+__device__ __forceinline__ T wave_total(T v)
+{
+    static_assert(sizeof(T) % 4 == 0, "readlane moves 32-bit words");
+    constexpr int NW = (int)(sizeof(T) / 4);
+    int vi[NW];
+    __builtin_memcpy(vi, &v, sizeof(T));
+#pragma unroll
+    for (int w = 0; w < NW; w++) vi[w] = __builtin_amdgcn_readlane(vi[w], WAVE_TOTAL_LANE);
+    T r;
+    __builtin_memcpy(&r, vi, sizeof(T));
+    return r;
+}
+
 // splitmix64 (Steele, Lea, Flood 2014): r(seed, n) = mix(seed + (n+1)*golden)
 __host__ __device__ inline uint64_t mix64(uint64_t z)
 {

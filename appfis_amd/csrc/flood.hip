@@ -469,13 +469,33 @@ __device__ inline void two_sum(double& s, double& cmp, double x)
     s = t;
 }
 
+// A compensated partial (sum, running error) and the wave reductions of a CFL minimum
+// and such a partial (DPP tree; the result is in lane WAVE_TOTAL_LANE)
+struct VolSum {
+    double s, c;
+};
+__device__ inline VolSum vol_merge(VolSum a, VolSum b)
+{
+    two_sum(a.s, a.c, b.s);
+    a.c = __dadd_rn(a.c, b.c);
+    return a;
+}
+template <typename R>
+__device__ inline void wave_min_sum(R& m, double& s, double& c)
+{
+    m = wave_reduce(m, RealLim<R>::max(), [](R a, R b) { return b < a ? b : a; });
+    const VolSum v = wave_reduce(VolSum{s, c}, VolSum{0.0, 0.0}, vol_merge);
+    s = v.s;
+    c = v.c;
+}
+
 // K2: wet/dry fix + per-tile CFL minimum and compensated volume partial (no atomics;
 // one tile of BX x FBY cells per block, partials[3t..3t+2] = (min, sum, compensation)).
 // Each thread owns RPT = FBY / BY consecutive rows of one column.  It loads h and z of
 // rows jb-1 .. jb+RPT (its cells and their vertical neighbours), its cells' momenta and
 // their horizontal neighbours before the first use, so all of its loads share one
 // memory round trip, then folds its cells into its own minimum and compensated sum
-// before the one wave butterfly (the FP64 reduction is paid once per thread, not once
+// before the one wave reduction (the FP64 reduction is paid once per thread, not once
 // per cell).  MODE 0: initial CFL only (no fix); MODE 1: end of a step.
 template <typename R, int MODE>
 __global__ __launch_bounds__(BT) void k_fix_cfl(const Scen<R>* __restrict__ sc, int nscen, int par, Params P)
@@ -554,15 +574,9 @@ __global__ __launch_bounds__(BT) void k_fix_cfl(const Scen<R>* __restrict__ sc, 
         }
         two_sum(vs, vc, (double)hij);
     }
-    for (int o = 32; o > 0; o >>= 1) {
-        R u = __shfl_xor(cfl, o, 64);
-        cfl = u < cfl ? u : cfl;
-        double us = __shfl_xor(vs, o, 64), uc = __shfl_xor(vc, o, 64);
-        two_sum(vs, vc, us);
-        vc = __dadd_rn(vc, uc);
-    }
+    wave_min_sum(cfl, vs, vc);
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-    if (lane == 0) {
+    if (lane == WAVE_TOTAL_LANE) {
         smin[wid] = cfl;
         ssum[wid][0] = vs;
         ssum[wid][1] = vc;
@@ -600,15 +614,9 @@ __global__ __launch_bounds__(BT) void k_finalize(const Scen<R>* __restrict__ sc,
         two_sum(as, ac, pp[1]);
         ac = __dadd_rn(ac, pp[2]);
     }
-    for (int o = 32; o > 0; o >>= 1) {
-        double u = __shfl_xor(m, o, 64);
-        m = u < m ? u : m;
-        double us = __shfl_xor(as, o, 64), uc = __shfl_xor(ac, o, 64);
-        two_sum(as, ac, us);
-        ac = __dadd_rn(ac, uc);
-    }
+    wave_min_sum(m, as, ac);
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-    if (lane == 0) {
+    if (lane == WAVE_TOTAL_LANE) {
         smin[wid] = m;
         ssum[wid][0] = as;
         ssum[wid][1] = ac;

@@ -9,7 +9,7 @@
 // it sets how evenly the blocks fill the chip).  p is staged through an LDS ring of
 // planes (k-1, k, k+1 with their 1-cell x/y halo), so each p value is read from memory
 // about once; the 12 coefficient arrays and wrk1 are pure coalesced streams.  Tiles
-// are ordered XCD-contiguously.  gosa: per-thread FP64 accumulation, wave64 shuffle tree, one LDS stage, one
+// are ordered XCD-contiguously.  gosa: per-thread FP64 accumulation, wave64 DPP tree, one LDS stage, one
 // partial per block, and the last block to arrive (atomic ticket) sums the partials in
 // block order into gosa[it] -- deterministic, on the device, no host round trip.
 //
@@ -63,12 +63,13 @@ __device__ inline void tile_of(int nx, int ny, int& bx, int& by, int& bz)
 
 __device__ inline double block_sum_last(double v, double* partials, unsigned* ticket, double* out, double* lds)
 {
-    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    const auto add = [](double a, double b) { return a + b; };
+    v = wave_reduce(v, 0.0, add);
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
     __shared__ bool last;
     const unsigned nblk = gridDim.x * gridDim.y * gridDim.z;
     const unsigned bid = blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z);
-    if (lane == 0) lds[wid] = v;
+    if (lane == WAVE_TOTAL_LANE) lds[wid] = v;
     __syncthreads();
     if (threadIdx.x == 0) {
         double s = 0.0;
@@ -84,9 +85,9 @@ __device__ inline double block_sum_last(double v, double* partials, unsigned* ti
     double acc = 0.0;
     for (unsigned p = threadIdx.x; p < nblk; p += TX)
         acc += __hip_atomic_load(partials + p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o, 64);
+    acc = wave_reduce(acc, 0.0, add);
     __syncthreads();
-    if (lane == 0) lds[wid] = acc;
+    if (lane == WAVE_TOTAL_LANE) lds[wid] = acc;
     __syncthreads();
     if (threadIdx.x == 0) {
         double s = 0.0;

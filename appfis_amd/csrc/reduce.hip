@@ -4,7 +4,7 @@
 // reference (Appfis/Runtime_CUDA.hpp:554-740, 1249-1666), whose first pass scans the
 // whole padded array with %,/ index math, then loops log_256 passes and copies one
 // scalar back.  Here: grid-stride over the Config box only, per-thread accumulation
-// in a wide type, a wave64 shuffle reduction, one LDS stage per block, per-block
+// in a wide type, a wave64 DPP reduction, one LDS stage per block, per-block
 // partials, and the last block to arrive (atomic ticket) combines the partials in a
 // fixed order -- so the result is deterministic and never leaves the device unless
 // the host asked for it.
@@ -59,38 +59,24 @@ __device__ inline KSum ksum_merge(KSum a, KSum b)
     return a;
 }
 
+// wave reductions (DPP tree of common.hpp; the total is in lane WAVE_TOTAL_LANE)
 template <typename T>
-__device__ inline T wave_max(T v)
+__device__ inline T wave_max(T v, T identity)
 {
-    for (int o = 32; o > 0; o >>= 1) {
-        T u = __shfl_xor(v, o, 64);
-        v = u > v ? u : v;
-    }
-    return v;
+    return wave_reduce(v, identity, [](T a, T b) { return b > a ? b : a; });
 }
 template <typename T>
-__device__ inline T wave_min(T v)
+__device__ inline T wave_min(T v, T identity)
 {
-    for (int o = 32; o > 0; o >>= 1) {
-        T u = __shfl_xor(v, o, 64);
-        v = u < v ? u : v;
-    }
-    return v;
+    return wave_reduce(v, identity, [](T a, T b) { return b < a ? b : a; });
 }
 __device__ inline long long wave_sum_i64(long long v)
 {
-    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-    return v;
+    return wave_reduce(v, 0LL, [](long long a, long long b) { return a + b; });
 }
 __device__ inline KSum wave_sum_k(KSum v)
 {
-    for (int o = 32; o > 0; o >>= 1) {
-        KSum u;
-        u.s = __shfl_xor(v.s, o, 64);
-        u.c = __shfl_xor(v.c, o, 64);
-        v = ksum_merge(v, u);
-    }
-    return v;
+    return wave_reduce(v, KSum{0.0, 0.0}, [](KSum a, KSum b) { return ksum_merge(a, b); });
 }
 
 template <typename T> struct Lim;
@@ -109,14 +95,14 @@ template <typename T> struct Ops<T, APPFIS_MAX> {
     __device__ static A init() { return Lim<T>::lo; }
     __device__ static void add(A& a, T v) { a = (A)v > a ? (A)v : a; }
     __device__ static A merge(A a, A b) { return b > a ? b : a; }
-    __device__ static A wave(A a) { return wave_max(a); }
+    __device__ static A wave(A a) { return wave_max(a, init()); }
 };
 template <typename T> struct Ops<T, APPFIS_MIN> {
     using A = typename Wide<T>::type;
     __device__ static A init() { return Lim<T>::hi; }
     __device__ static void add(A& a, T v) { a = (A)v < a ? (A)v : a; }
     __device__ static A merge(A a, A b) { return b < a ? b : a; }
-    __device__ static A wave(A a) { return wave_min(a); }
+    __device__ static A wave(A a) { return wave_min(a, init()); }
 };
 template <typename T> struct OpsSumInt {
     using A = long long;
@@ -168,7 +154,7 @@ __global__ __launch_bounds__(RB) void k_reduce(const T* __restrict__ data, Box b
         O::add(acc, data[box_offset(b, t)]);
     acc = O::wave(acc);
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-    if (lane == 0) lds[wid] = acc;
+    if (lane == WAVE_TOTAL_LANE) lds[wid] = acc;
     __syncthreads();
     if (threadIdx.x == 0) {
         A blk = lds[0];
@@ -187,7 +173,7 @@ __global__ __launch_bounds__(RB) void k_reduce(const T* __restrict__ data, Box b
         v = O::merge(v, load_l2(partials + p));
     }
     v = O::wave(v);
-    if (lane == 0) lds[wid] = v;
+    if (lane == WAVE_TOTAL_LANE) lds[wid] = v;
     __syncthreads();
     if (threadIdx.x == 0) {
         A r = lds[0];
