@@ -16,7 +16,7 @@
 //       (thread/wave/block min) and a compensated FP64 sum of h, written as one partial
 //       per 64 x 16 tile -- no atomics (per-scenario atomics serialise hundreds of thousands of
 //       blocks on a handful of addresses).
-//   K3 (finalise): one block per scenario combines its tile partials in a fixed order
+//   K3 (finalise): one block per scenario combines its strip partials in a fixed order
 //       and does on the device what the reference does on the host
 //       (triton_appfis.hpp:322-335, 381-385): simtime += dt, the print_interval
 //       bookkeeping, the next dt (fallback when every cell is dry, clamp to the next
@@ -25,7 +25,7 @@
 // No host round trip per step: the reference's Execute2DReduce(MIN) + D2H + 3 host
 // syncs per step become device-side state, so a chunk of steps can be captured in a
 // hipGraph and replayed.  Ensembles: every phase is one launch over a work-list of
-// strips / tiles spanning all scenarios (the scenario is found by binary search over
+// strips spanning all scenarios (the scenario is found by binary search over
 // per-scenario offsets), with per-scenario dt and reductions.
 //
 // REAL = double (reference dtype) or float (FP32 bulk form; the volume is still a
@@ -39,8 +39,7 @@ using namespace appfis;
 
 namespace {
 
-constexpr int BX = 64, BY = 4, BT = BX * BY;
-constexpr int FBY = 16; // rows of a wet/dry + CFL + volume tile (BX x FBY cells, FBY / BY per thread)
+constexpr int BT = 256;  // threads per block: four waves, one strip each
 constexpr int CHUNK = 20;  // steps per captured graph (even: buffer parity returns)
 
 template <typename R> struct RealLim;
@@ -66,14 +65,11 @@ struct Scen {
     const R* z;
     const R* n2;
     int nx, ny;
-    int tiles_x;
-    int tile0;         // first global tile of this scenario (BX x FBY tiles: wet/dry, CFL, volume)
-    int ntiles;
-    int etiles_x;      // the flux kernel's MW x MR strips
-    int etile0;
-    int entiles;
+    int strips_x;      // strips of MW x MR cells (every kernel of the step works on them)
+    int strip0;        // first global strip of this scenario
+    int nstrips;
     ScenState* st;
-    double* partials;  // 3 doubles per tile: CFL minimum, compensated volume sum and its error
+    double* partials;  // 3 doubles per strip: CFL minimum, compensated volume sum and its error
     double* rec_time;  // per-step records (nsteps each) or null
     double* rec_vol;
     double* rec_dt;
@@ -262,15 +258,14 @@ __device__ inline bool roe(R h1, R h2, R qx1, R qy1, R z1, R qx2, R qy2, R z2, R
 
 template <typename R> __device__ inline R sqrt_pos(R h) { return h > (R)0.0 ? fsqrt(h) : (R)0.0; }
 
-// scenario of a global tile: binary search over tile0 (or etile0 for the flux tiles),
-// uniform per block
-template <typename R, bool FLUX = false>
-__device__ inline int find_scen(const Scen<R>* sc, int nscen, int tile)
+// scenario of a global strip: binary search over strip0 (uniform per wave)
+template <typename R>
+__device__ inline int find_scen(const Scen<R>* sc, int nscen, int strip)
 {
     int lo = 0, hi = nscen - 1;
     while (lo < hi) {
         int mid = (lo + hi + 1) >> 1;
-        if ((FLUX ? sc[mid].etile0 : sc[mid].tile0) <= tile) lo = mid;
+        if (sc[mid].strip0 <= strip) lo = mid;
         else hi = mid - 1;
     }
     return lo;
@@ -344,10 +339,10 @@ __global__ __launch_bounds__(BT) void k_flux_march(const Scen<R>* __restrict__ s
     const int lane = threadIdx.x & 63;
     const int strip = blockIdx.x * MWAVES + (threadIdx.x >> 6);  // uniform per wave
     if (strip >= nstrips) return;
-    const int s = find_scen<R, true>(sc, nscen, strip);
+    const int s = find_scen(sc, nscen, strip);
     const Scen<R>& S = sc[s];
-    const int t = strip - S.etile0;
-    const int ty = t / S.etiles_x, tx = t - ty * S.etiles_x;
+    const int t = strip - S.strip0;
+    const int ty = t / S.strips_x, tx = t - ty * S.strips_x;
     const int nx = S.nx, ny = S.ny;
     const size_t W = (size_t)nx + 2;
     const int col = tx * MW + lane;  // padded column of this lane
@@ -489,111 +484,92 @@ __device__ inline void wave_min_sum(R& m, double& s, double& c)
     c = v.c;
 }
 
-// K2: wet/dry fix + per-tile CFL minimum and compensated volume partial (no atomics;
-// one tile of BX x FBY cells per block, partials[3t..3t+2] = (min, sum, compensation)).
-// Each thread owns RPT = FBY / BY consecutive rows of one column.  It loads h and z of
-// rows jb-1 .. jb+RPT (its cells and their vertical neighbours), its cells' momenta and
-// their horizontal neighbours before the first use, so all of its loads share one
-// memory round trip, then folds its cells into its own minimum and compensated sum
-// before the one wave reduction (the FP64 reduction is paid once per thread, not once
-// per cell).  MODE 0: initial CFL only (no fix); MODE 1: end of a step.
+// K2 (wet/dry fix + CFL + volume) as a row march over the same strips as K1: a wave
+// walks the cell rows of its strip holding rows j-1, j, j+1 of h and z in registers,
+// with the left / right neighbours' values shifted in across lanes, so each cell costs
+// one load of h, z, qx and qy.  The momentum fix of each cell is written in place (it
+// reads only h and z, which nothing here writes).  Each lane folds its cells into its
+// own CFL minimum and compensated volume sum, and the wave's DPP reduction leaves one
+// partial per strip, partials[3t .. 3t+2] = (min, sum, compensation).  No LDS and no
+// barriers.  MODE 0: initial CFL only (no fix); MODE 1: end of a step.
 template <typename R, int MODE>
-__global__ __launch_bounds__(BT) void k_fix_cfl(const Scen<R>* __restrict__ sc, int nscen, int par, Params P)
+__global__ __launch_bounds__(BT) void k_fix_march(const Scen<R>* __restrict__ sc, int nscen, int nstrips, int par,
+                                                  Params P)
 {
-    constexpr int RPT = FBY / BY;
-    __shared__ R smin[BT / 64];
-    __shared__ double ssum[BT / 64][2];
-    const int tile = (int)xcd_tile(blockIdx.x, gridDim.x);  // XCD-contiguous: halo rows share an L2
-    const int s = find_scen(sc, nscen, tile);
+    const int lane = threadIdx.x & 63;
+    const int strip = blockIdx.x * MWAVES + (threadIdx.x >> 6);  // uniform per wave
+    if (strip >= nstrips) return;
+    const int s = find_scen(sc, nscen, strip);
     const Scen<R>& S = sc[s];
-    const int t = tile - S.tile0;
-    const int tyi = t / S.tiles_x, txi = t - tyi * S.tiles_x;
-    const int i = txi * BX + (threadIdx.x & (BX - 1)) + 1;
-    const int jb = tyi * FBY + (threadIdx.x >> 6) * RPT + 1;  // the thread's first row
-    const R G = (R)9.81, EPS12 = (R)1e-12;
-    const R hextra = (R)P.hextra, dx = (R)P.dx, cn = (R)P.cn;
-    R cfl = RealLim<R>::max();
-    double vs = 0.0, vc = 0.0;
-    const int nx = S.nx, ny = S.ny, W = nx + 2;
+    const int t = strip - S.strip0;
+    const int ty = t / S.strips_x, tx = t - ty * S.strips_x;
+    const int nx = S.nx, ny = S.ny;
+    const size_t W = (size_t)nx + 2;
+    const int col = tx * MW + lane;  // padded column of this lane
+    const int j0 = ty * MR + 1;
+    const bool colmem = col <= nx + 1;
+    const bool cell = lane >= 1 && lane <= MW && col <= nx;
+    const int nr = min(MR, ny + 1 - j0);  // cell rows of the strip (j0 <= ny)
     const R* h = S.h[par];
     R* qx = S.qx[par];
     R* qy = S.qy[par];
     const R* z = S.z;
-    const bool col = i <= nx;
-    // hv/zv[r]: row jb-1+r (rows up to ny+1 exist); qxv/qyv, and h/z left (l) and right
-    // (r) of the cells, for the thread's rows jb .. jb+RPT-1
-    R hv[RPT + 2], zv[RPT + 2], qxv[RPT], qyv[RPT], hl[RPT], hr[RPT], zl[RPT], zr[RPT];
-#pragma unroll
-    for (int r = 0; r < RPT + 2; r++) {
-        const int row = jb - 1 + r;
-        hv[r] = (R)0.0;
-        zv[r] = (R)0.0;
-        if (col && row <= ny + 1) {
-            const size_t c = (size_t)i + (size_t)W * (size_t)row;
-            hv[r] = h[c];
-            if (MODE == 1) zv[r] = z[c];
-        }
+    const R G = (R)9.81, EPS12 = (R)1e-12;
+    const R hextra = (R)P.hextra, dx = (R)P.dx, cn = (R)P.cn;
+    auto ld = [&](const R* a, int row) { return colmem ? a[(size_t)col + W * (size_t)row] : (R)0.0; };
+    R cfl = RealLim<R>::max();
+    double vs = 0.0, vc = 0.0;
+    R hm = ld(h, j0 - 1), hc = ld(h, j0), zm = (R)0.0, zc = (R)0.0;
+    if (MODE == 1) {
+        zm = ld(z, j0 - 1);
+        zc = ld(z, j0);
     }
-#pragma unroll
-    for (int k = 0; k < RPT; k++) {
-        qxv[k] = qyv[k] = hl[k] = hr[k] = zl[k] = zr[k] = (R)0.0;
-        if (col && jb + k <= ny) {
-            const size_t c = (size_t)i + (size_t)W * (size_t)(jb + k);
-            qxv[k] = qx[c];
-            qyv[k] = qy[c];
-            if (MODE == 1) {
-                hl[k] = h[c - 1];
-                hr[k] = h[c + 1];
-                zl[k] = z[c - 1];
-                zr[k] = z[c + 1];
-            }
+    for (int r = 0; r < nr; r++) {
+        const int j = j0 + r;
+        const R hp = ld(h, j + 1);  // j + 1 <= ny + 1
+        const R zp = MODE == 1 ? ld(z, j + 1) : (R)0.0;
+        R qxv = cell ? qx[(size_t)col + W * (size_t)j] : (R)0.0;
+        R qyv = cell ? qy[(size_t)col + W * (size_t)j] : (R)0.0;
+        R hl = (R)0.0, hr = (R)0.0, zl = (R)0.0, zr = (R)0.0;
+        if (MODE == 1) {
+            hl = lane_shift<true>(hc);
+            hr = lane_shift<false>(hc);
+            zl = lane_shift<true>(zc);
+            zr = lane_shift<false>(zc);
         }
-    }
-#pragma unroll
-    for (int k = 0; k < RPT; k++) {
-        const int j = jb + k;
-        if (!col || j > ny) continue;
-        const size_t c = (size_t)i + (size_t)W * (size_t)j;
-        const R hij = hv[k + 1];
-        if (MODE == 1 && hij > hextra) {
-            const R zij = zv[k + 1];
-            if (((hij + zij < zr[k]) && (hr[k] < EPS12)) || ((hij + zij < zl[k]) && (hl[k] < EPS12))) {
-                qxv[k] = (R)0.0;
-                qx[c] = qxv[k];
+        if (cell) {
+            const size_t c = (size_t)col + W * (size_t)j;
+            const R hij = hc;
+            if (MODE == 1 && hij > hextra) {  // triton_appfis.hpp:1095-1112
+                const R zij = zc;
+                if (((hij + zij < zr) && (hr < EPS12)) || ((hij + zij < zl) && (hl < EPS12))) {
+                    qxv = (R)0.0;
+                    qx[c] = qxv;
+                }
+                if (((hij + zij < zm) && (hm < EPS12)) || ((hij + zij < zp) && (hp < EPS12))) {
+                    qyv = (R)0.0;
+                    qy[c] = qyv;
+                }
             }
-            if (((hij + zij < zv[k]) && (hv[k] < EPS12)) || ((hij + zij < zv[k + 2]) && (hv[k + 2] < EPS12))) {
-                qyv[k] = (R)0.0;
-                qy[c] = qyv[k];
+            if (hij > hextra) {  // triton_appfis.hpp:567-586
+                const R dtx = fdiv(dx, fabs(fdiv(qxv, hij)) + fsqrt(G * hij));
+                const R dty = fdiv(dx, fabs(fdiv(qyv, hij)) + fsqrt(G * hij));
+                const R cc = dtx < dty ? cn * dtx : cn * dty;
+                cfl = cc < cfl ? cc : cfl;
             }
+            two_sum(vs, vc, (double)hij);
         }
-        if (hij > hextra) {
-            const R dtx = fdiv(dx, fabs(fdiv(qxv[k], hij)) + fsqrt(G * hij));
-            const R dty = fdiv(dx, fabs(fdiv(qyv[k], hij)) + fsqrt(G * hij));
-            const R cc = dtx < dty ? cn * dtx : cn * dty;
-            cfl = cc < cfl ? cc : cfl;
-        }
-        two_sum(vs, vc, (double)hij);
+        hm = hc;
+        hc = hp;
+        zm = zc;
+        zc = zp;
     }
     wave_min_sum(cfl, vs, vc);
-    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
     if (lane == WAVE_TOTAL_LANE) {
-        smin[wid] = cfl;
-        ssum[wid][0] = vs;
-        ssum[wid][1] = vc;
-    }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        R m = smin[0];
-        double bs = ssum[0][0], bc = ssum[0][1];
-        for (int w = 1; w < BT / 64; w++) {
-            m = smin[w] < m ? smin[w] : m;
-            two_sum(bs, bc, ssum[w][0]);
-            bc = __dadd_rn(bc, ssum[w][1]);
-        }
         double* pp = S.partials + 3 * (size_t)t;
-        pp[0] = (double)m;
-        pp[1] = bs;
-        pp[2] = bc;
+        pp[0] = (double)cfl;
+        pp[1] = vs;
+        pp[2] = vc;
     }
 }
 
@@ -608,7 +584,7 @@ __global__ __launch_bounds__(BT) void k_finalize(const Scen<R>* __restrict__ sc,
     const Scen<R>& S = sc[blockIdx.x];
     double m = (double)RealLim<R>::max();
     double as = 0.0, ac = 0.0;
-    for (int p = threadIdx.x; p < S.ntiles; p += BT) {
+    for (int p = threadIdx.x; p < S.nstrips; p += BT) {
         const double* pp = S.partials + 3 * (size_t)p;
         m = pp[0] < m ? pp[0] : m;
         two_sum(as, ac, pp[1]);
@@ -677,30 +653,25 @@ int run_flood(appfis_ctx* ctx, int nscen, const int* nx, const int* ny, void* co
         APPFIS_REQUIRE(h[s] && qx[s] && qy[s] && z[s] && n2[s], "scenario %d: NULL field", s);
     }
     APPFIS_REQUIRE(prm->dx > 0.0, "dx must be positive");
-    // tile work-list
+    // strip work-list
     std::vector<Scen<R>> hs((size_t)nscen);
-    long long total_tiles = 0, total_etiles = 0;
+    long long total_strips = 0;
     size_t field_bytes = 0;
     for (int s = 0; s < nscen; s++) {
-        int tx = (int)ceil_div(nx[s], BX), ty = (int)ceil_div(ny[s], FBY);
         hs[s].nx = nx[s];
         hs[s].ny = ny[s];
-        hs[s].tiles_x = tx;
-        hs[s].tile0 = (int)total_tiles;
-        hs[s].ntiles = tx * ty;
-        total_tiles += (long long)tx * ty;
-        const int etx = (int)ceil_div(nx[s], MW), ety = (int)ceil_div(ny[s], MR);
-        hs[s].etiles_x = etx;
-        hs[s].etile0 = (int)total_etiles;
-        hs[s].entiles = etx * ety;
-        total_etiles += (long long)etx * ety;
+        const int sx = (int)ceil_div(nx[s], MW), sy = (int)ceil_div(ny[s], MR);
+        hs[s].strips_x = sx;
+        hs[s].strip0 = (int)total_strips;
+        hs[s].nstrips = sx * sy;
+        total_strips += (long long)sx * sy;
         field_bytes += ((size_t)(nx[s] + 2) * (ny[s] + 2) * sizeof(R) + 255) / 256 * 256;
     }
-    APPFIS_REQUIRE(total_tiles < (1LL << 31) && total_etiles < (1LL << 31), "too many tiles");
+    APPFIS_REQUIRE(total_strips < (1LL << 31), "too many strips");
     const int rec = steps > 0 ? steps : 1;
     // workspace: second buffers (h, qx, qy) | states | partials | records | descriptor table
     const size_t st_bytes = ((size_t)nscen * sizeof(ScenState) + 255) / 256 * 256;
-    const size_t part_bytes = ((size_t)total_tiles * 24 + 255) / 256 * 256;
+    const size_t part_bytes = ((size_t)total_strips * 24 + 255) / 256 * 256;
     const size_t rec_bytes = ((size_t)nscen * rec * 8 + 255) / 256 * 256;
     const size_t tab_bytes = ((size_t)nscen * sizeof(Scen<R>) + 255) / 256 * 256;
     int stt = ensure_work(ctx, 3 * field_bytes + st_bytes + part_bytes + 3 * rec_bytes + tab_bytes);
@@ -733,7 +704,7 @@ int run_flood(appfis_ctx* ctx, int nscen, const int* nx, const int* ny, void* co
     Scen<R>* tab = (Scen<R>*)(base + st_bytes + part_bytes + 3 * rec_bytes);
     for (int s = 0; s < nscen; s++) {
         hs[s].st = st + s;
-        hs[s].partials = partials + 3 * (size_t)hs[s].tile0;
+        hs[s].partials = partials + 3 * (size_t)hs[s].strip0;
         hs[s].rec_time = rt + (size_t)s * rec;
         hs[s].rec_vol = rv + (size_t)s * rec;
         hs[s].rec_dt = rd + (size_t)s * rec;
@@ -743,17 +714,17 @@ int run_flood(appfis_ctx* ctx, int nscen, const int* nx, const int* ny, void* co
     Params P{prm->dx, prm->courant, prm->hextra, prm->dt_fallback, prm->print_interval, prm->rain_rate, rec};
     hipLaunchKernelGGL(k_init_state, dim3(ceil_div(nscen, 64)), dim3(64), 0, ctx->stream, st, nscen, prm->start_time);
     APPFIS_LAUNCH_CHECK();
-    const dim3 grid((unsigned)total_tiles), egrid((unsigned)ceil_div(total_etiles, MWAVES));
-    const int nstrips = (int)total_etiles;
+    const dim3 grid((unsigned)ceil_div(total_strips, MWAVES));  // MWAVES strips per block
+    const int nstrips = (int)total_strips;
     // dt of the first step from the initial state
-    hipLaunchKernelGGL((k_fix_cfl<R, 0>), grid, dim3(BT), 0, ctx->stream, tab, nscen, 0, P);
+    hipLaunchKernelGGL((k_fix_march<R, 0>), grid, dim3(BT), 0, ctx->stream, tab, nscen, nstrips, 0, P);
     APPFIS_LAUNCH_CHECK();
     hipLaunchKernelGGL((k_finalize<R, 0>), dim3(nscen), dim3(BT), 0, ctx->stream, tab, P);
     APPFIS_LAUNCH_CHECK();
     auto step_pair = [&](int par) -> int {
-        hipLaunchKernelGGL((k_flux_march<R>), egrid, dim3(BT), 0, ctx->stream, tab, nscen, nstrips, par, P);
+        hipLaunchKernelGGL((k_flux_march<R>), grid, dim3(BT), 0, ctx->stream, tab, nscen, nstrips, par, P);
         APPFIS_LAUNCH_CHECK();
-        hipLaunchKernelGGL((k_fix_cfl<R, 1>), grid, dim3(BT), 0, ctx->stream, tab, nscen, par ^ 1, P);
+        hipLaunchKernelGGL((k_fix_march<R, 1>), grid, dim3(BT), 0, ctx->stream, tab, nscen, nstrips, par ^ 1, P);
         APPFIS_LAUNCH_CHECK();
         hipLaunchKernelGGL((k_finalize<R, 1>), dim3(nscen), dim3(BT), 0, ctx->stream, tab, P);
         APPFIS_LAUNCH_CHECK();

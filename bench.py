@@ -491,14 +491,14 @@ def bench_flood(ctx, hip, steps=2000, nscen=24):
     # 64 B per cell FP64 (read h qx qy z n2, write h qx qy), 32 B FP32
     out["plain_f64"] = {"ms": round(ms64, 2), "glups": round(lups / (ms64 * 1e-3) / 1e9, 2),
                         "roofline": roofline(64.0 * cells / nscen, ms64 / (steps * nscen),
-                                             {"k_flux_march<double>": 1, "k_fix_cfl<double, 1>": 1,
+                                             {"k_flux_march<double>": 1, "k_fix_march<double, 1>": 1,
                                               "k_finalize<double, 1>": 1})}
     out["chosen_f32_ensemble"] = {"ms": round(ms32, 2), "glups": round(lups / (ms32 * 1e-3) / 1e9, 2),
                                   "speedup_vs_plain_f64": round(ms64 / ms32, 3),
                                   "h_rel_l2_vs_f64(first,last scenario)": rel,
                                   "volume_rel_err_vs_f64": vrel,
                                   "roofline": roofline(32.0 * cells, ms32 / steps,
-                                                       {"k_flux_march<float>": 1, "k_fix_cfl<float, 1>": 1,
+                                                       {"k_flux_march<float>": 1, "k_fix_march<float, 1>": 1,
                                                         "k_finalize<float, 1>": 1})}
     for d in sc:
         for k in ("h", "qx", "qy", "z", "n2"):
@@ -525,7 +525,7 @@ PROFILE_GROUPS = [
     ("himeno_257x257x513", "Himeno 257x257x513 (FP32, FP64, defect correction)",
      ("k_himeno", "k_add_correction", "k_f64_to_f32")),
     ("flood_ensemble_24", "Triton flood: 24 scenarios N in [512,4096] (plain FP64 one at a time, FP32 ensemble)",
-     ("k_flux_march", "k_fix_cfl", "k_finalize", "k_init_state", "k_dem")),
+     ("k_flux_march", "k_fix_march", "k_finalize", "k_init_state", "k_dem")),
 ]
 
 
