@@ -56,22 +56,22 @@ __global__ __launch_bounds__(256) void k_heat2d_plain(const T* __restrict__ u, T
     }
 }
 
-// Temporal blocking: TX x TY output tile, TS sweeps per launch, 256 threads.
+// Temporal blocking: TX x TY output tile, TS sweeps per launch, NT threads.
 // One TX x TY output tile (tile coordinates tbx, tby) of one grid, `steps` <= TS sweeps.
-template <typename T, int TX, int TY, int TS>
+template <typename T, int TX, int TY, int TS, int NT>
 __device__ inline void tb_tile(const T* __restrict__ u, T* __restrict__ v, int nx, int ny, T c, int steps, int tbx,
                                int tby)
 {
     constexpr int RX = TX + 2 * TS, RY = TY + 2 * TS;
     constexpr int NCELL = RX * RY;
-    constexpr int PER = (NCELL + 255) / 256;
+    constexpr int PER = (NCELL + NT - 1) / NT;
     __shared__ T tile[RY][RX];
     const int W = nx + 2, H = ny + 2;
     // window origin in padded coordinates (may be negative / past the array)
     const int gx0 = tbx * TX + 1 - TS;
     const int gy0 = tby * TY + 1 - TS;
     const int tid = threadIdx.x;
-    for (int p = tid; p < NCELL; p += 256) {
+    for (int p = tid; p < NCELL; p += NT) {
         int ly = p / RX, lx = p - ly * RX;
         int gx = gx0 + lx, gy = gy0 + ly;
         T val = (T)0;
@@ -85,7 +85,7 @@ __device__ inline void tb_tile(const T* __restrict__ u, T* __restrict__ v, int n
         const int lo = s + 1, hx = RX - s - 1, hy = RY - s - 1;
 #pragma unroll
         for (int q = 0; q < PER; q++) {
-            int p = tid + q * 256;
+            int p = tid + q * NT;
             int ly = p / RX, lx = p - ly * RX;
             int gx = gx0 + lx, gy = gy0 + ly;
             bool active = p < NCELL && lx >= lo && lx < hx && ly >= lo && ly < hy && gx >= 1 && gx <= nx &&
@@ -102,7 +102,7 @@ __device__ inline void tb_tile(const T* __restrict__ u, T* __restrict__ v, int n
         __syncthreads();
 #pragma unroll
         for (int q = 0; q < PER; q++) {
-            int p = tid + q * 256;
+            int p = tid + q * NT;
             int ly = p / RX, lx = p - ly * RX;
             int gx = gx0 + lx, gy = gy0 + ly;
             bool active = p < NCELL && lx >= lo && lx < hx && ly >= lo && ly < hy && gx >= 1 && gx <= nx &&
@@ -112,20 +112,20 @@ __device__ inline void tb_tile(const T* __restrict__ u, T* __restrict__ v, int n
         __syncthreads();
     }
     // write the TX x TY centre (interior cells only)
-    for (int p = tid; p < TX * TY; p += 256) {
+    for (int p = tid; p < TX * TY; p += NT) {
         int ly = p / TX, lx = p - ly * TX;
         int gx = tbx * TX + 1 + lx, gy = tby * TY + 1 + ly;
         if (gx <= nx && gy <= ny) v[(size_t)gx + (size_t)W * (size_t)gy] = tile[ly + TS][lx + TS];
     }
 }
 
-template <typename T, int TX, int TY, int TS>
-__global__ __launch_bounds__(256) void k_heat2d_tb(const T* __restrict__ u, T* __restrict__ v, int nx, int ny, T c,
-                                                   int steps)
+template <typename T, int TX, int TY, int TS, int NT>
+__global__ __launch_bounds__(NT) void k_heat2d_tb(const T* __restrict__ u, T* __restrict__ v, int nx, int ny, T c,
+                                                  int steps)
 {
     const unsigned nbx = gridDim.x;
     const unsigned t = xcd_tile(blockIdx.x + nbx * blockIdx.y, nbx * gridDim.y);
-    tb_tile<T, TX, TY, TS>(u, v, nx, ny, c, steps, (int)(t % nbx), (int)(t / nbx));
+    tb_tile<T, TX, TY, TS, NT>(u, v, nx, ny, c, steps, (int)(t % nbx), (int)(t / nbx));
 }
 
 // Ensemble: grid s owns tiles tile0 .. tile0 + tiles_x * tiles_y - 1 of one launch.
@@ -149,7 +149,7 @@ __global__ __launch_bounds__(256) void k_heat2d_tb_ens(const HeatGrid<T>* __rest
     const HeatGrid<T>& G = tab[lo];
     const int t = tile - G.tile0;
     const int tby = t / G.tiles_x;
-    tb_tile<T, TX, TY, TS>(G.b[par], G.b[par ^ 1], G.nx, G.ny, c, steps, t - tby * G.tiles_x, tby);
+    tb_tile<T, TX, TY, TS, 256>(G.b[par], G.b[par ^ 1], G.nx, G.ny, c, steps, t - tby * G.tiles_x, tby);
 }
 
 constexpr int PLAIN_RY = 4;
@@ -181,7 +181,7 @@ inline int plan_launches(int iters, int ts, int* counts, int max_launches)
     return n;
 }
 
-template <typename T, int TX, int TY, int TS>
+template <typename T, int TX, int TY, int TS, int NT>
 int run_tb(appfis_ctx* ctx, T* b0, T* b1, int nx, int ny, T c, int iters, int depth)
 {
     T* buf[2] = {b0, b1};
@@ -190,17 +190,17 @@ int run_tb(appfis_ctx* ctx, T* b0, T* b1, int nx, int ny, T c, int iters, int de
     int n = plan_launches(iters, depth < TS ? depth : TS, counts.data(), iters);
     if (n <= 0) return invalid("heat2d: cannot split %d sweeps", iters);
     for (int l = 0; l < n; l++) {
-        hipLaunchKernelGGL((k_heat2d_tb<T, TX, TY, TS>), grid, dim3(256), 0, ctx->stream, buf[l & 1],
+        hipLaunchKernelGGL((k_heat2d_tb<T, TX, TY, TS, NT>), grid, dim3(NT), 0, ctx->stream, buf[l & 1],
                            buf[(l + 1) & 1], nx, ny, c, counts[l]);
         APPFIS_LAUNCH_CHECK();
     }
     return APPFIS_OK;
 }
 
-template <typename T, int TX, int TY, int TS>
+template <typename T, int TX, int TY, int TS, int NT = 256>
 int run_tb_v(appfis_ctx* ctx, void* b0, void* b1, int nx, int ny, double c, int iters, int depth)
 {
-    return run_tb<T, TX, TY, TS>(ctx, (T*)b0, (T*)b1, nx, ny, (T)c, iters, depth);
+    return run_tb<T, TX, TY, TS, NT>(ctx, (T*)b0, (T*)b1, nx, ny, (T)c, iters, depth);
 }
 
 struct HeatVariant {
@@ -209,16 +209,28 @@ struct HeatVariant {
     int ts;  // sweeps per launch at most
 };
 
-// temporal-blocking candidates (output tile TX x TY, TS sweeps per launch)
+// temporal-blocking candidates (output tile TX x TY, TS sweeps per launch, NT threads).
+// A launch costs a fixed load of its (TX+2TS) x (TY+2TS) windows plus a per-sweep part;
+// larger tiles load less redundant window per output cell, and enough threads keep the
+// per-sweep part short.
 static const HeatVariant kHeatF64[] = {
-    {"tb 32x32 ts8", run_tb_v<double, 32, 32, 8>, 8},   {"tb 32x32 ts4", run_tb_v<double, 32, 32, 4>, 4},
-    {"tb 64x32 ts8", run_tb_v<double, 64, 32, 8>, 8},   {"tb 64x64 ts8", run_tb_v<double, 64, 64, 8>, 8},
-    {"tb 64x32 ts12", run_tb_v<double, 64, 32, 12>, 12}, {"tb 64x32 ts16", run_tb_v<double, 64, 32, 16>, 16},
+    {"tb 32x32 ts8 nt256", run_tb_v<double, 32, 32, 8>, 8},
+    {"tb 32x32 ts4 nt256", run_tb_v<double, 32, 32, 4>, 4},
+    {"tb 64x32 ts8 nt256", run_tb_v<double, 64, 32, 8>, 8},
+    {"tb 64x32 ts12 nt256", run_tb_v<double, 64, 32, 12>, 12},
+    {"tb 64x64 ts8 nt1024", run_tb_v<double, 64, 64, 8, 1024>, 8},
+    {"tb 64x64 ts12 nt1024", run_tb_v<double, 64, 64, 12, 1024>, 12},
+    {"tb 64x32 ts8 nt512", run_tb_v<double, 64, 32, 8, 512>, 8},
+    {"tb 64x32 ts16 nt256", run_tb_v<double, 64, 32, 16>, 16},
 };
 static const HeatVariant kHeatF32[] = {
-    {"tb 64x32 ts8", run_tb_v<float, 64, 32, 8>, 8},    {"tb 32x32 ts4", run_tb_v<float, 32, 32, 4>, 4},
-    {"tb 64x64 ts8", run_tb_v<float, 64, 64, 8>, 8},    {"tb 64x64 ts16", run_tb_v<float, 64, 64, 16>, 16},
-    {"tb 128x64 ts16", run_tb_v<float, 128, 64, 16>, 16},
+    {"tb 64x32 ts8 nt256", run_tb_v<float, 64, 32, 8>, 8},
+    {"tb 32x32 ts4 nt256", run_tb_v<float, 32, 32, 4>, 4},
+    {"tb 64x64 ts8 nt256", run_tb_v<float, 64, 64, 8>, 8},
+    {"tb 64x64 ts16 nt256", run_tb_v<float, 64, 64, 16>, 16},
+    {"tb 64x64 ts8 nt1024", run_tb_v<float, 64, 64, 8, 1024>, 8},
+    {"tb 64x64 ts16 nt1024", run_tb_v<float, 64, 64, 16, 1024>, 16},
+    {"tb 128x64 ts16 nt1024", run_tb_v<float, 128, 64, 16, 1024>, 16},
 };
 
 }  // namespace

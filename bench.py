@@ -196,14 +196,16 @@ def bench_heat(ctx, hip, n=1024, iters=500):
         if variant == hip.PLAIN:
             launches = iters
         else:
-            tname = _tuned(ctx, 1, hip.F64)  # e.g. "tb 64x32 ts8"
-            launches = len(_plan(iters, int(tname.rsplit("ts", 1)[1])))
+            tname = _tuned(ctx, 1, hip.F64)  # e.g. "tb 64x64 ts8 nt1024"
+            parts = tname.split()
+            ts, nt = int(parts[2][2:]), int(parts[3][2:])
+            launches = len(_plan(iters, ts))
         per_launch_sweeps = iters / launches
         if variant == hip.PLAIN:
             kname = "k_heat2d_plain<double, 4>"
         else:
-            tx, ty = tname.split()[1].split("x")
-            kname = f"k_heat2d_tb<double, {tx}, {ty}, {tname.rsplit('ts', 1)[1]}>"
+            tx, ty = parts[1].split("x")
+            kname = f"k_heat2d_tb<double, {tx}, {ty}, {ts}, {nt}>"
         r = {"ms": round(ms, 4), "glups": round(n * n * iters / (ms * 1e-3) / 1e9, 2), "checksum": chk,
              "roofline": roofline(16.0 * n * n * per_launch_sweeps, ms / launches, kname)}
         if variant == hip.CHOSEN:
