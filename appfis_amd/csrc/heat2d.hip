@@ -135,9 +135,9 @@ struct HeatGrid {
     int nx, ny, tiles_x, tile0;
 };
 
-template <typename T, int TX, int TY, int TS>
-__global__ __launch_bounds__(256) void k_heat2d_tb_ens(const HeatGrid<T>* __restrict__ tab, int ngrid, int par, T c,
-                                                       int steps)
+template <typename T, int TX, int TY, int TS, int NT>
+__global__ __launch_bounds__(NT) void k_heat2d_tb_ens(const HeatGrid<T>* __restrict__ tab, int ngrid, int par, T c,
+                                                      int steps)
 {
     const int tile = (int)xcd_tile(blockIdx.x, gridDim.x);
     int lo = 0, hi = ngrid - 1;  // the grid of this tile: binary search over tile0 (uniform)
@@ -149,7 +149,7 @@ __global__ __launch_bounds__(256) void k_heat2d_tb_ens(const HeatGrid<T>* __rest
     const HeatGrid<T>& G = tab[lo];
     const int t = tile - G.tile0;
     const int tby = t / G.tiles_x;
-    tb_tile<T, TX, TY, TS, 256>(G.b[par], G.b[par ^ 1], G.nx, G.ny, c, steps, t - tby * G.tiles_x, tby);
+    tb_tile<T, TX, TY, TS, NT>(G.b[par], G.b[par ^ 1], G.nx, G.ny, c, steps, t - tby * G.tiles_x, tby);
 }
 
 constexpr int PLAIN_RY = 4;
@@ -296,7 +296,7 @@ extern "C" int appfis_heat2d(appfis_ctx* ctx, int dtype, void* buf0, void* buf1,
     return invalid("heat2d: unknown variant %d", variant);
 }
 
-template <typename T, int TX, int TY, int TS>
+template <typename T, int TX, int TY, int TS, int NT>
 int run_ensemble(appfis_ctx* ctx, int ngrid, const int* nx, const int* ny, void* const* buf0, void* const* buf1,
                  double c, int iters)
 {
@@ -327,7 +327,7 @@ int run_ensemble(appfis_ctx* ctx, int ngrid, const int* nx, const int* ny, void*
     // the whole launch chain is captured once and replayed as one graph
     APPFIS_HIP_CHECK(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
     for (int l = 0; l < n; l++)
-        hipLaunchKernelGGL((k_heat2d_tb_ens<T, TX, TY, TS>), dim3((unsigned)tiles), dim3(256), 0, ctx->stream, tab,
+        hipLaunchKernelGGL((k_heat2d_tb_ens<T, TX, TY, TS, NT>), dim3((unsigned)tiles), dim3(NT), 0, ctx->stream, tab,
                            ngrid, l & 1, (T)c, counts[l]);
     hipGraph_t graph = nullptr;
     hipError_t e = hipStreamEndCapture(ctx->stream, &graph);
@@ -354,8 +354,8 @@ extern "C" int appfis_heat2d_ensemble(appfis_ctx* ctx, int dtype, int ngrid, con
     APPFIS_REQUIRE(ngrid >= 1, "heat2d ensemble: need at least one grid");
     APPFIS_REQUIRE(iters >= 0, "iters must be >= 0");
     if (iters == 0) return APPFIS_OK;
-    if (dtype == APPFIS_F64) return run_ensemble<double, 32, 32, 8>(ctx, ngrid, nx, ny, buf0, buf1, c, iters);
-    if (dtype == APPFIS_F32) return run_ensemble<float, 64, 32, 8>(ctx, ngrid, nx, ny, buf0, buf1, c, iters);
+    if (dtype == APPFIS_F64) return run_ensemble<double, 64, 64, 12, 1024>(ctx, ngrid, nx, ny, buf0, buf1, c, iters);
+    if (dtype == APPFIS_F32) return run_ensemble<float, 64, 64, 8, 1024>(ctx, ngrid, nx, ny, buf0, buf1, c, iters);
     return invalid("heat2d ensemble: dtype must be F64 or F32");
 }
 

@@ -299,9 +299,10 @@ def bench_heat_ensemble(ctx, hip, ngrid=64, iters=500):
         res[name] = [(a, b)[iters % 2].download().sum() for a, b, _, _ in grids[:2]]
         r = {"ms": round(ms, 3), "glups": round(cells * iters / (ms * 1e-3) / 1e9, 2)}
         if name == "ensemble":
-            launches = len(_plan(iters, 8))
+            # the FP64 ensemble runs 64 x 64 tiles, 12 sweeps per launch, 1024 threads
+            launches = len(_plan(iters, 12))
             r["roofline"] = roofline(16.0 * cells * iters / launches, ms / launches,
-                                     "k_heat2d_tb_ens<double, 32, 32, 8>")
+                                     "k_heat2d_tb_ens<double, 64, 64, 12, 1024>")
         out[name] = r
     assert res["ensemble"] == res["plain_one_at_a_time"] == res["chosen_one_at_a_time"]
     out["ensemble"]["speedup_vs_plain"] = round(out["plain_one_at_a_time"]["ms"] / out["ensemble"]["ms"], 3)
