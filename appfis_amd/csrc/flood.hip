@@ -88,6 +88,28 @@ template <typename R> __device__ __forceinline__ R fdiv(R a, R b) { return a / b
 template <> __device__ __forceinline__ float fdiv<float>(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
 template <typename R> __device__ __forceinline__ R fsqrt(R a) { return sqrt(a); }
 template <> __device__ __forceinline__ float fsqrt<float>(float a) { return __builtin_amdgcn_sqrtf(a); }
+// Cube root of a positive depth: libm's for R = double; 2^(log2(a)/3) from the hardware
+// log2 / exp2 approximations for the FP32 form (a few ulp).
+template <typename R> __device__ __forceinline__ R fcbrt(R a) { return cbrt(a); }
+template <> __device__ __forceinline__ float fcbrt<float>(float a)
+{
+    return __builtin_amdgcn_exp2f(__builtin_amdgcn_logf(a) * (1.0f / 3.0f));
+}
+
+// dt * d / dx of an edge's contribution: the functors' expression for R = double; one
+// multiply by a per-thread dt/dx for the FP32 form.
+template <typename R>
+struct RhsScale {
+    R dt, dx;
+    __device__ RhsScale(R dt_, R dx_) : dt(dt_), dx(dx_) {}
+    __device__ R operator()(R d) const { return fdiv(dt * d, dx); }
+};
+template <>
+struct RhsScale<float> {
+    float f;
+    __device__ RhsScale(float dt_, float dx_) : f(dt_ * __builtin_amdgcn_rcpf(dx_)) {}
+    __device__ float operator()(float d) const { return d * f; }
+};
 
 // Edge normal: (1, 0) for DIR 0 (X edges), (0, 1) for DIR 1 (Y edges).  The FP64 form
 // multiplies the normal out exactly as the reference functors do (a product with the
@@ -360,6 +382,7 @@ __global__ __launch_bounds__(BT) void k_flux_march(const Scen<R>* __restrict__ s
     const R dx = (R)P.dx, hextra = (R)P.hextra, rain = (R)P.rain;
     const R G = (R)9.81, EPS12 = (R)1e-12;
     const int rend = min(MR, ny + 1 - j0);  // last row offset: rows up to ny+1 exist
+    const RhsScale<R> scale(dt, dx);
 
     CellV<R> prev, cur, nxt;
     cell_load(prev, h, qx, qy, z, n2, colmem, (size_t)col + W * (size_t)(j0 - 1));
@@ -390,7 +413,7 @@ __global__ __launch_bounds__(BT) void k_flux_march(const Scen<R>* __restrict__ s
             // side 2 of the edge on its left: rhs*1 = (triton_appfis.hpp:821-823)
             if (ok) {
 #pragma unroll
-                for (int k = 0; k < 3; k++) c1[k] = fdiv(dt * d2[k], dx);
+                for (int k = 0; k < 3; k++) c1[k] = scale(d2[k]);
             }
             // side 1 of the edge on its right (solved by lane l+1): rhs*0 = (817-819)
             const int okr = from_right(ok ? 1 : 0);
@@ -399,7 +422,7 @@ __global__ __launch_bounds__(BT) void k_flux_march(const Scen<R>* __restrict__ s
             for (int k = 0; k < 3; k++) d1r[k] = lane_shift<false>(d1[k]);
             if (okr && lane < 63) {
 #pragma unroll
-                for (int k = 0; k < 3; k++) c0[k] = fdiv(dt * d1r[k], dx);
+                for (int k = 0; k < 3; k++) c0[k] = scale(d1r[k]);
             }
         }
         {
@@ -413,8 +436,8 @@ __global__ __launch_bounds__(BT) void k_flux_march(const Scen<R>* __restrict__ s
             if (ok) {
 #pragma unroll
                 for (int k = 0; k < 3; k++) {
-                    c0[k] = c0[k] + fdiv(dt * d1[k], dx);
-                    p1[k] = p1[k] + fdiv(dt * d2[k], dx);
+                    c0[k] = c0[k] + scale(d1[k]);
+                    p1[k] = p1[k] + scale(d2[k]);
                 }
             }
         }
@@ -434,7 +457,7 @@ __global__ __launch_bounds__(BT) void k_flux_march(const Scen<R>* __restrict__ s
                 const R modM = fsqrt(mx * mx + my * my);
                 const R nn = prev.n2;
                 if (nn > EPS12 && hn >= hextra && modM > EPS12) {
-                    const R tt = fdiv(dt * G * nn * modM, hn * hn * cbrt(hn));
+                    const R tt = fdiv(dt * G * nn * modM, hn * hn * fcbrt(hn));
                     qxij = fdiv((R)-0.5 * (mx - mx * fsqrt((R)1.0 + (R)4.0 * tt)), tt);
                     qyij = fdiv((R)-0.5 * (my - my * fsqrt((R)1.0 + (R)4.0 * tt)), tt);
                 } else {
