@@ -3,9 +3,16 @@
  * reference Appfis/Runtime_CUDA.hpp; a new design, not a translation).
  *
  * User stencils are compile-time functors, so the generic launch stays a template in
- * the application's translation unit (built by hipcc for gfx950): one thread per point
- * of the Config box, x along the 64-wide wavefront so consecutive lanes touch
- * consecutive addresses (blocks of 64 x 4 in 2D, 64 x 4 x 1 in 3D, 256 in 1D).
+ * the application's translation unit (built by hipcc for gfx950), with x along the
+ * 64-wide wavefront so consecutive lanes touch consecutive addresses.  In 2D a block of
+ * 64 x 4 threads covers 64 x 16 points (each thread 4 rows, 4 apart); in 3D a block of
+ * 64 x 4 threads covers 64 x 4 points on 8 consecutive planes (each thread marches them
+ * in order, so the z-neighbours a functor reads are still in its CU's L1); in 1D one
+ * thread per point.  Blocks are taken in XCD-contiguous order: workgroup b runs on XCD
+ * b % 8, and the remap gives each XCD one contiguous run of tiles, so tiles that read
+ * each other's halo points share an L2.  The points of one Execute are independent
+ * (the functor reads the current buffers and writes the next ones), so the order in
+ * which they are computed does not change a result.
  * Everything else goes through the C ABI (include/appfis_hip.h): allocation and copies,
  * the single-rank halo exchange, boundary fill and the single-pass reductions.
  *
@@ -38,21 +45,44 @@ namespace APPFIS
             if (i < ex) STENCIL(i, args...);
         }
 
-        template <typename... TArgs>
-        __global__ __launch_bounds__(256) void Kernel2D(int sx, int ex, int sy, int ey, TArgs... args)
+        constexpr int ROWS_2D = 4;    // rows per thread in 2D (block tile 64 x 16)
+        constexpr int PLANES_3D = 8;  // planes per thread in 3D (block tile 64 x 4 x 8)
+
+        // Tile of flat block b of nb: every XCD (b % 8) takes one contiguous run.
+        __device__ __forceinline__ unsigned XcdTile(unsigned b, unsigned nb)
         {
-            const int i = sx + (int)(blockIdx.x * 64 + threadIdx.x);
-            const int j = sy + (int)(blockIdx.y * 4 + threadIdx.y);
-            if (i < ex && j < ey) STENCIL(i, j, args...);
+            const unsigned x = b & 7u, q = nb >> 3, r = nb & 7u;
+            return x * q + (x < r ? x : r) + (b >> 3);
         }
 
         template <typename... TArgs>
-        __global__ __launch_bounds__(256) void Kernel3D(int sx, int ex, int sy, int ey, int sz, int ez, TArgs... args)
+        __global__ __launch_bounds__(256) void Kernel2D(int sx, int ex, int sy, int ey, int tiles_x, TArgs... args)
         {
-            const int i = sx + (int)(blockIdx.x * 64 + threadIdx.x);
-            const int j = sy + (int)(blockIdx.y * 4 + threadIdx.y);
-            const int k = sz + (int)blockIdx.z;
-            if (i < ex && j < ey && k < ez) STENCIL(i, j, k, args...);
+            const int t = (int)XcdTile(blockIdx.x, gridDim.x);
+            const int ty = t / tiles_x, tx = t - ty * tiles_x;
+            const int i = sx + tx * 64 + (int)threadIdx.x;
+            if (i >= ex) return;
+            const int j0 = sy + ty * (4 * ROWS_2D) + (int)threadIdx.y;
+#pragma unroll
+            for (int r = 0; r < ROWS_2D; r++) {
+                const int j = j0 + 4 * r;
+                if (j < ey) STENCIL(i, j, args...);
+            }
+        }
+
+        template <typename... TArgs>
+        __global__ __launch_bounds__(256) void Kernel3D(int sx, int ex, int sy, int ey, int sz, int ez, int tiles_x,
+                                                        int tiles_y, TArgs... args)
+        {
+            const int t = (int)XcdTile(blockIdx.x, gridDim.x);
+            const int txy = t % (tiles_x * tiles_y), tz = t / (tiles_x * tiles_y);
+            const int ty = txy / tiles_x, tx = txy - ty * tiles_x;
+            const int i = sx + tx * 64 + (int)threadIdx.x;
+            const int j = sy + ty * 4 + (int)threadIdx.y;
+            if (i >= ex || j >= ey) return;
+            const int k0 = sz + tz * PLANES_3D;
+            const int k1 = k0 + PLANES_3D < ez ? k0 + PLANES_3D : ez;
+            for (int k = k0; k < k1; k++) STENCIL(i, j, k, args...);
         }
 
         inline void CheckLaunch(const char* what)
@@ -140,9 +170,9 @@ namespace APPFIS
         const int sx = config.GetStartX(), ex = config.GetEndX();
         const int sy = config.GetStartY(), ey = config.GetEndY();
         if (ex > sx && ey > sy) {
-            hipLaunchKernelGGL(hip_detail::Kernel2D<TArgs...>,
-                               dim3(hip_detail::Blocks(ex - sx, 64), hip_detail::Blocks(ey - sy, 4)), dim3(64, 4), 0,
-                               GetHipStream(), sx, ex, sy, ey, args...);
+            const unsigned tx = hip_detail::Blocks(ex - sx, 64), ty = hip_detail::Blocks(ey - sy, 4 * hip_detail::ROWS_2D);
+            hipLaunchKernelGGL(hip_detail::Kernel2D<TArgs...>, dim3(tx * ty), dim3(64, 4), 0, GetHipStream(), sx, ex, sy,
+                               ey, (int)tx, args...);
             hip_detail::CheckLaunch("Execute2D");
         }
         hip_detail::ExchangeAndSwap(config);
@@ -156,9 +186,10 @@ namespace APPFIS
         const int sy = config.GetStartY(), ey = config.GetEndY();
         const int sz = config.GetStartZ(), ez = config.GetEndZ();
         if (ex > sx && ey > sy && ez > sz) {
-            hipLaunchKernelGGL(hip_detail::Kernel3D<TArgs...>,
-                               dim3(hip_detail::Blocks(ex - sx, 64), hip_detail::Blocks(ey - sy, 4), (unsigned)(ez - sz)),
-                               dim3(64, 4, 1), 0, GetHipStream(), sx, ex, sy, ey, sz, ez, args...);
+            const unsigned tx = hip_detail::Blocks(ex - sx, 64), ty = hip_detail::Blocks(ey - sy, 4),
+                           tz = hip_detail::Blocks(ez - sz, hip_detail::PLANES_3D);
+            hipLaunchKernelGGL(hip_detail::Kernel3D<TArgs...>, dim3(tx * ty * tz), dim3(64, 4, 1), 0, GetHipStream(), sx,
+                               ex, sy, ey, sz, ez, (int)tx, (int)ty, args...);
             hip_detail::CheckLaunch("Execute3D");
         }
         hip_detail::ExchangeAndSwap(config);

@@ -38,14 +38,18 @@ int main(int argc, char* argv[])
     at.ghostLayer = 4;
     APPFIS::Initialize(argc, argv, APPFIS::DIM_3D, APPFIS::DIM_3D, at);
     APPFIS::Grid<float> ga(APPFIS::DIM_3D, nx, ny, nz, APPFIS::DUAL), gb(APPFIS::DIM_3D, nx, ny, nz, APPFIS::DUAL);
+    // a grid of the same shape for one untimed call, so the hand kernel's one-time
+    // autotuning for this shape stays outside the timed region
+    APPFIS::Grid<float> gw(APPFIS::DIM_3D, nx, ny, nz, APPFIS::DUAL);
     APPFIS::Config<float> config(ga);
     config.GridsToUpdate(ga);
-    APPFIS::InitializeExecution(ga, gb);
+    APPFIS::InitializeExecution(ga, gb, gw);
     const int dims[3] = {nx + 8, ny + 8, nz + 8};
     for (auto* g : {&ga, &gb})
         APPFIS::detail::Check(appfis_fill_uniform(APPFIS::GetContext(), g->GetSubGridsGPUData(), APPFIS_F32, 3, dims,
                                                   4, 1004, 0.0, 1.0), "fill");
     const double cf[5] = {0.5, 0.125, 0.0625, 0.125 / 3, 0.03125};
+    APPFIS::kernels::FDTD3d(gw, cf, 2);
     APPFIS::Timer timer;
     timer.Start("GENERIC");
     for (int s = 0; s < steps; s++)
@@ -55,7 +59,7 @@ int main(int argc, char* argv[])
     timer.Start("KERNEL");
     APPFIS::kernels::FDTD3d(gb, cf, steps);
     timer.Stop("KERNEL");
-    APPFIS::FinalizeExecution(ga, gb);
+    APPFIS::FinalizeExecution(ga, gb, gw);
     const size_t n = (size_t)dims[0] * dims[1] * dims[2];
     const bool same = memcmp(ga.GetSubGrid()->GetData(), gb.GetSubGrid()->GetData(), n * sizeof(float)) == 0;
     const long long pts = (long long)nx * ny * nz;

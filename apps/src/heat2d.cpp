@@ -38,6 +38,10 @@ int main(int argc, char* argv[])
     APPFIS::Grid<double> ga(APPFIS::DIM_2D, nx, ny, APPFIS::DUAL), gb(APPFIS::DIM_2D, nx, ny, APPFIS::DUAL);
     APPFIS::Grid<double> gc(APPFIS::DIM_2D, nx, ny, APPFIS::DUAL), gd(APPFIS::DIM_2D, nx / 2 + 3, ny + 5, APPFIS::DUAL),
         ge(APPFIS::DIM_2D, 37, 29, APPFIS::DUAL);
+    // a grid of the same shape for one untimed call, so the hand kernel's one-time
+    // autotuning for this shape stays outside the timed region
+    APPFIS::Grid<double> gw(APPFIS::DIM_2D, nx, ny, APPFIS::DUAL);
+    gw.SetBoundary(0.0);
     ga.SetBoundary(0.0);
     gb.SetBoundary(0.0);
     gc.SetBoundary(0.0);
@@ -45,7 +49,7 @@ int main(int argc, char* argv[])
     ge.SetBoundary(0.0);
     APPFIS::Config<double> config(ga);
     config.GridsToUpdate(ga);
-    APPFIS::InitializeExecution(ga, gb, gc, gd, ge);
+    APPFIS::InitializeExecution(ga, gb, gc, gd, ge, gw);
     const int dims[2] = {nx + 2, ny + 2};
     APPFIS::detail::Check(appfis_fill_uniform(APPFIS::GetContext(), gc.GetSubGridsGPUData(), APPFIS_F64, 2, dims, 1,
                                               1001, 0.0, 1.0), "fill");
@@ -53,6 +57,7 @@ int main(int argc, char* argv[])
                                               1001, 0.0, 1.0), "fill");
     APPFIS::detail::Check(appfis_fill_uniform(APPFIS::GetContext(), gb.GetSubGridsGPUData(), APPFIS_F64, 2, dims, 1,
                                               1001, 0.0, 1.0), "fill");
+    APPFIS::kernels::Heat2D(gw, 0.1, 2, APPFIS_CHOSEN);
     APPFIS::Timer timer;
     timer.Start("GENERIC");
     for (int it = 0; it < iters; it++) APPFIS::Execute2D(config, 0.1, ga.GetSubGrid(), ga.GetSubGridNext());
@@ -67,7 +72,7 @@ int main(int argc, char* argv[])
     double sa = 0, sb = 0;
     APPFIS::Execute2DReduce(config, sa, APPFIS::SUM, ga);
     APPFIS::Execute2DReduce(config, sb, APPFIS::SUM, gb);
-    APPFIS::FinalizeExecution(ga, gb, gc, gd, ge);
+    APPFIS::FinalizeExecution(ga, gb, gc, gd, ge, gw);
     const size_t n = (size_t)(nx + 2) * (ny + 2);
     const bool same = memcmp(ga.GetSubGrid()->GetData(), gb.GetSubGrid()->GetData(), n * sizeof(double)) == 0 &&
                       memcmp(ga.GetSubGrid()->GetData(), gc.GetSubGrid()->GetData(), n * sizeof(double)) == 0;
