@@ -17,7 +17,10 @@ roofline of its dominant kernel; `--headline-only` skips them.
 Roofline: achieved = algorithmic bytes per launch of the dominant kernel / its average
 launch time (timed region / launches); peak = 8 TB/s HBM.  `traffic` is the HBM bytes
 per launch measured with rocprofv3 --pmc (FETCH_SIZE + WRITE_SIZE, calibrated on a copy
-kernel), read from profiles/traffic.json when present.
+kernel), read from profiles/traffic.json when present.  `valu` is the vector-ALU issue
+rate: the wave instructions rocprofv3 counted per dispatch (SQ_INSTS_VALU, same file)
+over the launch time measured here, against 256 CUs x 64 lanes x 2.4 GHz; the
+VALU-bound kernels (bit-packed GoL, flood flux) are judged by it.
 
 cpu_baseline: the oracle (OpenMP C restatement of the reference CPU path) timed on this
 host on a bounded sample of the headline workload.
