@@ -44,6 +44,7 @@ HBM_PEAK_GBS = 8000.0
 # single-issue rate, not a hard ceiling.
 VALU_PEAK_TOPS = 256 * 64 * 2.4e9 / 1e12
 FDTD_COEFF = [0.5, 0.125, 0.0625, 0.125 / 3, 0.03125]
+HIMENO_DC_OUTER = 16  # the library's default rebase interval of the Himeno defect correction
 
 
 def _traffic(kernels):
@@ -399,12 +400,13 @@ def bench_himeno(ctx, hip, nx=257, ny=257, nz=513, iters=200):
         if bpl:
             r["roofline"] = roofline(float(bpl) * pts, ms / iters, f"k_himeno<{'float' if dt == hip.F32 else 'double'}, 0>")
         else:
-            # defect correction, per sweep on average over one outer cycle of K = 8: seven
-            # FP32 inner sweeps (11 coefficient copies, r, q read, q written: 56 B/LUP) and
-            # one FP64 rebase (11 coefficients, wrk1, B read, q read 4 B, B / r / q written
-            # 8 + 4 + 4 B: 124 B/LUP)
-            r["roofline"] = roofline((7 * 56.0 + 124.0) / 8 * pts, ms / iters,
-                                     {"k_himeno<float, 1>": 7 / 8, "k_himeno_rebase": 1 / 8})
+            # defect correction, per sweep on average over one outer cycle of K sweeps
+            # (the library's default K = 16): K-1 FP32 inner sweeps (11 coefficient copies,
+            # r, q read, q written: 56 B/LUP) and one FP64 rebase (11 coefficients, wrk1, B
+            # read, q read 4 B, B / r / q written 8 + 4 + 4 B: 124 B/LUP)
+            K = HIMENO_DC_OUTER
+            r["roofline"] = roofline(((K - 1) * 56.0 + 124.0) / K * pts, ms / iters,
+                                     {"k_himeno<float, 1>": (K - 1) / K, "k_himeno_rebase": 1 / K})
         out[name] = r
         for a in arrs:
             a.free()

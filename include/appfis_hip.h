@@ -204,7 +204,7 @@ int appfis_fdtd3d_candidates(int dtype, char* names, int len);
  * Field in p[iters % 2] at exit.  gosa_host (iters doubles) receives each sweep's
  * gosa.  variant PLAIN: dtype F32 or F64, reference operation order.
  * variant CHOSEN (defect correction): dtype F64 inputs; FP32 inner sweeps on the
- * correction, FP64 outer residual every `outer` sweeps (<=0: tuned).  */
+ * correction, FP64 outer residual every `outer` sweeps (<=0: every 16).  */
 int appfis_himeno(appfis_ctx* ctx, int dtype, void* const* arrays, int nx, int ny, int nz,
                   double omega, int iters, int variant, int outer, double* gosa_host);
 
