@@ -337,10 +337,128 @@ __global__ __launch_bounds__(64 * NWAVES) void k_step_tb1(const uint32_t* __rest
     }
 }
 
+// Full-row waves: each wave holds K rows of the whole packed width, NW = 64 * WPL words,
+// lane l holding words WPL*l .. WPL*l + WPL - 1 of each row.  A word's horizontal
+// neighbours are then the lane's own registers, except at the lane's two ends, which
+// come from the adjacent lanes by DPP wave rotates (wave_ror / wave_rol), and the
+// rotation is the periodic wrap of the row: no LDS for the horizontal direction, and
+// 16-byte loads and stores.  The block's NWAVES waves stack bands of K rows.  Each
+// generation a wave publishes the horizontal sums (both bit-planes) of its first and
+// last rows into LDS slots double-buffered by generation parity; after the one barrier
+// it reads the sums of the rows just above and below its band (zero outside the
+// block's window).  A wave still reading generation g's slots is never overwritten:
+// the next writes to them come after the barrier of generation g+1, which every wave
+// reaches only after finishing g.  G generations per launch, OUT = NWAVES*K - 2G
+// output rows per block.
+template <int WPL>
+struct WordVec;
+template <> struct WordVec<1> {
+    using V = uint32_t;
+    __device__ static void get(const V& v, uint32_t (&w)[1]) { w[0] = v; }
+    __device__ static V put(const uint32_t (&w)[1]) { return w[0]; }
+};
+template <> struct WordVec<2> {
+    using V = uint2;
+    __device__ static void get(const V& v, uint32_t (&w)[2]) { w[0] = v.x; w[1] = v.y; }
+    __device__ static V put(const uint32_t (&w)[2]) { return make_uint2(w[0], w[1]); }
+};
+template <> struct WordVec<4> {
+    using V = uint4;
+    __device__ static void get(const V& v, uint32_t (&w)[4]) { w[0] = v.x; w[1] = v.y; w[2] = v.z; w[3] = v.w; }
+    __device__ static V put(const uint32_t (&w)[4]) { return make_uint4(w[0], w[1], w[2], w[3]); }
+};
+
+template <int WPL, int K, int G, int NWAVES>
+__global__ __launch_bounds__(64 * NWAVES) void k_step_tw(const uint32_t* __restrict__ P, uint32_t* __restrict__ Q,
+                                                          int NW, int ny, int WX, int OUT, int gens)
+{
+    using VW = WordVec<WPL>;
+    using V = typename VW::V;
+    (void)WX;
+    // [parity][wave][first / last row][bit-plane][word of the row]
+    __shared__ uint32_t hs[2][NWAVES][2][2][64 * WPL];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int band = (int)xcd_tile(blockIdx.x, gridDim.x);  // XCD-contiguous band order
+    const int y0 = band * OUT - G + wv * K;  // global row of register row 0
+    int yw = y0 % ny;
+    if (yw < 0) yw += ny;
+    uint32_t R[K][WPL];
+#pragma unroll
+    for (int k = 0; k < K; k++) {
+        int y = yw + k;  // periodic rows (the window may wrap more than once when ny < K)
+        while (y >= ny) y -= ny;
+        VW::get(reinterpret_cast<const V*>(P + (size_t)y * NW)[lane], R[k]);
+    }
+    for (int g = 0; g < gens; g++) {
+        const int b = g & 1;
+        uint32_t h0[K + 2][WPL], h1[K + 2][WPL];
+#pragma unroll
+        for (int k = 0; k < K; k++) {
+            // the words left of the lane's first and right of its last (periodic row; a
+            // rotation has a source for every lane)
+            const uint32_t lft = (uint32_t)__builtin_amdgcn_mov_dpp((int)R[k][WPL - 1], 0x13C, 0xF, 0xF, true);
+            const uint32_t rgt = (uint32_t)__builtin_amdgcn_mov_dpp((int)R[k][0], 0x134, 0xF, 0xF, true);
+#pragma unroll
+            for (int w = 0; w < WPL; w++) {
+                const uint32_t c = R[k][w];
+                const uint32_t l = w == 0 ? lft : R[k][w - 1];
+                const uint32_t r = w == WPL - 1 ? rgt : R[k][w + 1];
+                const uint32_t Wc = __builtin_amdgcn_alignbit(c, l, 31);
+                const uint32_t Ec = __builtin_amdgcn_alignbit(r, c, 1);
+                h0[k + 1][w] = xor3(Wc, c, Ec);
+                h1[k + 1][w] = maj(Wc, c, Ec);
+            }
+        }
+        reinterpret_cast<V*>(hs[b][wv][0][0])[lane] = VW::put(h0[1]);
+        reinterpret_cast<V*>(hs[b][wv][0][1])[lane] = VW::put(h1[1]);
+        reinterpret_cast<V*>(hs[b][wv][1][0])[lane] = VW::put(h0[K]);
+        reinterpret_cast<V*>(hs[b][wv][1][1])[lane] = VW::put(h1[K]);
+        __syncthreads();
+        {
+            // rows -1 and K (zero outside the block's window): whole-vector reads from the
+            // neighbouring waves' slots, a wave-uniform select at the block edge
+            const int up = wv > 0 ? wv - 1 : wv, dn = wv < NWAVES - 1 ? wv + 1 : wv;
+            uint32_t a0[WPL], a1[WPL], z0[WPL], z1[WPL];
+            VW::get(reinterpret_cast<const V*>(hs[b][up][1][0])[lane], a0);
+            VW::get(reinterpret_cast<const V*>(hs[b][up][1][1])[lane], a1);
+            VW::get(reinterpret_cast<const V*>(hs[b][dn][0][0])[lane], z0);
+            VW::get(reinterpret_cast<const V*>(hs[b][dn][0][1])[lane], z1);
+#pragma unroll
+            for (int w = 0; w < WPL; w++) {
+                h0[0][w] = wv > 0 ? a0[w] : 0u;
+                h1[0][w] = wv > 0 ? a1[w] : 0u;
+                h0[K + 1][w] = wv < NWAVES - 1 ? z0[w] : 0u;
+                h1[K + 1][w] = wv < NWAVES - 1 ? z1[w] : 0u;
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < K; k++)
+#pragma unroll
+            for (int w = 0; w < WPL; w++) {
+                const uint32_t s0 = xor3(h0[k][w], h0[k + 1][w], h0[k + 2][w]);
+                const uint32_t c0 = maj(h0[k][w], h0[k + 1][w], h0[k + 2][w]);
+                const uint32_t v1 = xor3(h1[k][w], h1[k + 1][w], h1[k + 2][w]);
+                const uint32_t c1 = maj(h1[k][w], h1[k + 1][w], h1[k + 2][w]);
+                const uint32_t s1 = v1 ^ c0;
+                const uint32_t s2 = c1 ^ (v1 & c0);
+                R[k][w] = (s0 & s1 & ~s2) | (R[k][w] & s2 & ~(s0 | s1));
+            }
+    }
+    // output rows: window rows G .. G+OUT-1 of the block, global rows band*OUT + r
+    const int ybase = band * OUT - G + wv * K;
+#pragma unroll
+    for (int k = 0; k < K; k++) {
+        const int r = wv * K + k - G;
+        const int y = ybase + k;
+        if (r >= 0 && r < OUT && y < ny) reinterpret_cast<V*>(Q + (size_t)y * NW)[lane] = VW::put(R[k]);
+    }
+}
+
 struct TbVariant {
     const char* name;
     void (*kernel)(const uint32_t*, uint32_t*, int, int, int, int, int);
     int k, g, nwaves;
+    int wpl;  // full-row form: words per lane (the wave spans the row); 0: WX waves per row
 };
 
 static const TbVariant kTb[] = {
@@ -356,6 +474,10 @@ static const TbVariant kTb[] = {
     // 4K - 2G = 32 output rows per block like k10 g4 (256 blocks fill 256 CUs at 8192
     // rows), with more generations per launch to spread the launch's fixed load / store
     {"tb1 k11 g6 w16", k_step_tb1<11, 6, 16>, 11, 6, 16}, {"tb1 k12 g8 w16", k_step_tb1<12, 8, 16>, 12, 8, 16},
+    // full-row waves (nx = 2048 * WPL), also 32 output rows per block at 8192 rows
+    {"tw4 k3 g8 w16", k_step_tw<4, 3, 8, 16>, 3, 8, 16, 4}, {"tw4 k4 g8 w12", k_step_tw<4, 4, 8, 12>, 4, 8, 12, 4},
+    {"tw4 k6 g8 w8", k_step_tw<4, 6, 8, 8>, 6, 8, 8, 4},    {"tw2 k3 g8 w16", k_step_tw<2, 3, 8, 16>, 3, 8, 16, 2},
+    {"tw1 k3 g8 w16", k_step_tw<1, 3, 8, 16>, 3, 8, 16, 1},
 };
 
 }  // namespace golbits
@@ -376,8 +498,16 @@ int run_step(appfis_ctx* ctx, uint32_t* const P[2], int* cur, int NW, int ny, in
     return APPFIS_OK;
 }
 
+// rows of a block's window: the full-row form stacks NWAVES bands; the other forms
+// spread each row over WX = NW / 64 waves
+int tb_rows(const golbits::TbVariant& v, int NW)
+{
+    return v.wpl > 0 ? v.nwaves * v.k : (v.nwaves / (NW / 64)) * v.k;
+}
+
 bool tb_fits(const golbits::TbVariant& v, int NW)
 {
+    if (v.wpl > 0) return NW == 64 * v.wpl && v.nwaves * v.k > 2 * v.g;
     if (NW % 64) return false;
     const int WX = NW / 64;
     return WX <= v.nwaves && v.nwaves % WX == 0 && (v.nwaves / WX) * v.k > 2 * v.g;
@@ -386,8 +516,8 @@ bool tb_fits(const golbits::TbVariant& v, int NW)
 // `gens` generations from P[*cur] with temporal-blocking variant v (G per launch)
 int run_tb(appfis_ctx* ctx, const golbits::TbVariant& v, uint32_t* const P[2], int* cur, int NW, int ny, int gens)
 {
-    const int WX = NW / 64, WY = v.nwaves / WX;
-    const int OUT = WY * v.k - 2 * v.g;
+    const int WX = NW / 64;
+    const int OUT = tb_rows(v, NW) - 2 * v.g;
     const unsigned blocks = ceil_div((size_t)ny, (size_t)OUT);
     for (int done = 0; done < gens;) {
         const int n = gens - done < v.g ? gens - done : v.g;
@@ -416,8 +546,8 @@ int gol_bits_run(appfis_ctx* ctx, uint8_t* b0, uint8_t* b1, int nx, int ny, int 
     if (force >= 0) {
         APPFIS_REQUIRE(force < ntb, "gol: candidate %d out of range (%d candidates)", force, ntb);
         APPFIS_REQUIRE(tb_fits(golbits::kTb[force], NW),
-                       "gol: candidate %s needs nx a multiple of 2048 that its block of waves spans",
-                       golbits::kTb[force].name);
+                       "gol: candidate %s needs nx that its block of waves spans (nx = %d)",
+                       golbits::kTb[force].name, nx);
     }
     const size_t words = (size_t)NW * ny;
     const size_t half = (words * 4 + 255) / 256 * 256;
