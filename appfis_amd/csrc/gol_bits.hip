@@ -35,6 +35,22 @@ __device__ __forceinline__ uint32_t maj(uint32_t a, uint32_t b, uint32_t c)
     return __builtin_amdgcn_bitop3_b32(a, b, c, 0xE8);  // at least two of a, b, c
 }
 
+// B3/S23 from the adders of the three rows' 2-bit horizontal sums: with s0 = xor3 and
+// c0 = maj of the low bit-planes and v1 = xor3, c1 = maj of the high ones, the 3x3 sum
+// is S = s0 + 2(c0 + v1) + 4c1, so
+//   S == 3  <=>  s0 & P,   P = (c0 ^ v1) & ~c1
+//   S == 4  <=>  ~s0 & Q,  Q = ~(c0 ^ v1) & (c0 ^ c1)
+// and alive' = (s0 & P) | (alive & ~s0 & Q): four three-input operations (truth-table
+// index a*4 + b*2 + c for bitop3(a, b, c)), checked against S == 3 | (alive & S == 4)
+// for every triple of 2-bit sums.
+__device__ __forceinline__ uint32_t life_rule(uint32_t s0, uint32_t c0, uint32_t v1, uint32_t c1, uint32_t alive)
+{
+    const uint32_t p = __builtin_amdgcn_bitop3_b32(c0, v1, c1, 0x14);
+    const uint32_t q = __builtin_amdgcn_bitop3_b32(c0, v1, c1, 0x42);
+    const uint32_t t = __builtin_amdgcn_bitop3_b32(alive, s0, q, 0x20);
+    return __builtin_amdgcn_bitop3_b32(s0, p, t, 0xEA);
+}
+
 // cells x = 32w .. 32w+31 of interior row y (uint8, 0/1) -> one word
 __global__ __launch_bounds__(256) void k_pack(const uint8_t* __restrict__ u, uint32_t* __restrict__ P, int nx,
                                               int ny, int NW)
@@ -98,17 +114,12 @@ __global__ __launch_bounds__(256) void k_step(const uint32_t* __restrict__ P, ui
     for (int k = 0; k < K; k++) {
         const int y = y0 + k;
         if (y >= ny) break;
-        // S = h[k] + h[k+1] + h[k+2]: bits s0 (1), s1 (2), s2 (4); the 8s bit is implied
+        // S = h[k] + h[k+1] + h[k+2] through the carry-save adders of the two bit-planes
         const uint32_t s0 = xor3(h0[k], h0[k + 1], h0[k + 2]);
         const uint32_t c0 = maj(h0[k], h0[k + 1], h0[k + 2]);
         const uint32_t u1 = xor3(h1[k], h1[k + 1], h1[k + 2]);
         const uint32_t c1 = maj(h1[k], h1[k + 1], h1[k + 2]);
-        const uint32_t s1 = u1 ^ c0;
-        const uint32_t s2 = c1 ^ (u1 & c0);
-        // S <= 9, so S == 3 <=> s0 s1 ~s2 and S == 4 <=> ~s0 ~s1 s2
-        const uint32_t eq3 = s0 & s1 & ~s2;
-        const uint32_t eq4 = s2 & ~(s0 | s1);
-        Q[(size_t)y * NW + w] = eq3 | (cen[k + 1] & eq4);
+        Q[(size_t)y * NW + w] = life_rule(s0, c0, u1, c1, cen[k + 1]);
     }
 }
 
@@ -220,9 +231,7 @@ __global__ __launch_bounds__(64 * NWAVES) void k_step_tb(const uint32_t* __restr
             const uint32_t c0 = maj(u0, h0[k], d0);
             const uint32_t v1 = xor3(u1, h1[k], d1);
             const uint32_t c1 = maj(u1, h1[k], d1);
-            const uint32_t s1 = v1 ^ c0;
-            const uint32_t s2 = c1 ^ (v1 & c0);
-            R[k] = (s0 & s1 & ~s2) | (R[k] & s2 & ~(s0 | s1));
+            R[k] = life_rule(s0, c0, v1, c1, R[k]);
         }
     }
     // output rows: window rows G .. G+OUT-1 of the block, global rows band*OUT + r
@@ -323,9 +332,7 @@ __global__ __launch_bounds__(64 * NWAVES) void k_step_tb1(const uint32_t* __rest
             const uint32_t c0 = maj(h0[k], h0[k + 1], h0[k + 2]);
             const uint32_t v1 = xor3(h1[k], h1[k + 1], h1[k + 2]);
             const uint32_t c1 = maj(h1[k], h1[k + 1], h1[k + 2]);
-            const uint32_t s1 = v1 ^ c0;
-            const uint32_t s2 = c1 ^ (v1 & c0);
-            R[k] = (s0 & s1 & ~s2) | (R[k] & s2 & ~(s0 | s1));
+            R[k] = life_rule(s0, c0, v1, c1, R[k]);
         }
     }
     const int ybase = band * OUT - G + wy * K;
@@ -439,9 +446,7 @@ __global__ __launch_bounds__(64 * NWAVES) void k_step_tw(const uint32_t* __restr
                 const uint32_t c0 = maj(h0[k][w], h0[k + 1][w], h0[k + 2][w]);
                 const uint32_t v1 = xor3(h1[k][w], h1[k + 1][w], h1[k + 2][w]);
                 const uint32_t c1 = maj(h1[k][w], h1[k + 1][w], h1[k + 2][w]);
-                const uint32_t s1 = v1 ^ c0;
-                const uint32_t s2 = c1 ^ (v1 & c0);
-                R[k][w] = (s0 & s1 & ~s2) | (R[k][w] & s2 & ~(s0 | s1));
+                R[k][w] = life_rule(s0, c0, v1, c1, R[k][w]);
             }
     }
     // output rows: window rows G .. G+OUT-1 of the block, global rows band*OUT + r

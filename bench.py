@@ -137,16 +137,25 @@ def bench_gol(ctx, hip, steps, warmup, variant, nx=8192, ny=8192):
         g = int(tname.split(" g", 1)[1].split()[0]) if tname else 1
         res["tuned"] = tname
         res["autotune"] = [l for l in ctx.tune_report().splitlines() if l.startswith("autotune kernel 4 ")]
-        if tname:
-            k, gg, w = (int(t[1:]) for t in tname.split()[1:4])
-            kname = f"k_step_{tname.split()[0]}<{k}, {gg}, {w}>"
-        else:
-            kname = "k_step<8>"
+        kname = gol_kernel_name(tname)
         # (the blocked kernel is bound by integer VALU issue, not HBM: see roofline.valu)
         res["roofline"] = roofline(2.0 * nx * ny / 8, launch_ms * g, kname)
     d0.free()
     d1.free()
     return res
+
+
+def gol_kernel_name(tname):
+    """Kernel symbol of a bit-packed GoL candidate: "tb k12 g8 w16" -> k_step_tb<12, 8, 16>,
+    "tw4 k4 g8 w12" -> k_step_tw<4, 4, 8, 12> (words per lane first); no candidate (widths
+    the blocked kernels do not take) -> the one-generation k_step<8>."""
+    if not tname:
+        return "k_step<8>"
+    form, *rest = tname.split()
+    k, g, w = (int(t[1:]) for t in rest[:3])
+    if form.startswith("tw"):
+        return f"k_step_tw<{int(form[2:])}, {k}, {g}, {w}>"
+    return f"k_step_{form}<{k}, {g}, {w}>"
 
 
 def cpu_baseline_gol(nx=8192, ny=8192, target_s=12.0):

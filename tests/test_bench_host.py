@@ -74,3 +74,11 @@ def test_valu_lookup(tmp_path, monkeypatch):
     assert bench._valu({"a": 1, "c": 1}, 0.1) is None
     r = bench.roofline(1e9, 1.0, {"a": 1})
     assert r["valu"]["achieved"] == 0.06 and r["traffic"] == 1.0
+
+
+def test_gol_kernel_names_match_the_profiled_symbols():
+    names = json.load(open(os.path.join(ROOT, "profiles", "traffic.json")))
+    for tname, want in (("tw4 k4 g8 w12", "k_step_tw<4, 4, 8, 12>"), ("tb1 k12 g8 w16", "k_step_tb1<12, 8, 16>"),
+                        ("tb k10 g4 w16", "k_step_tb<10, 4, 16>"), (None, "k_step<8>")):
+        assert bench.gol_kernel_name(tname) == want
+        assert want in names, want
