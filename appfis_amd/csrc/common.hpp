@@ -140,6 +140,42 @@ __device__ __forceinline__ T wave_total(T v)
     return r;
 }
 
+// Lane shifts across the wave (DPP wave_shr:1 / wave_shl:1): from_left gives lane l the
+// value of lane l-1, from_right the value of lane l+1 (lanes 0 / 63 keep their own).
+__device__ __forceinline__ int from_left(int v) { return __builtin_amdgcn_update_dpp(v, v, 0x138, 0xF, 0xF, false); }
+__device__ __forceinline__ int from_right(int v) { return __builtin_amdgcn_update_dpp(v, v, 0x130, 0xF, 0xF, false); }
+template <bool LEFT>
+__device__ __forceinline__ float lane_shift(float v)
+{
+    const int b = __float_as_int(v);
+    return __int_as_float(LEFT ? from_left(b) : from_right(b));
+}
+template <bool LEFT>
+__device__ __forceinline__ double lane_shift(double v)
+{
+    const unsigned long long b = (unsigned long long)__double_as_longlong(v);
+    const int lo = (int)(unsigned)b, hi = (int)(unsigned)(b >> 32);
+    const unsigned l2 = (unsigned)(LEFT ? from_left(lo) : from_right(lo));
+    const unsigned h2 = (unsigned)(LEFT ? from_left(hi) : from_right(hi));
+    return __longlong_as_double((long long)(((unsigned long long)h2 << 32) | l2));
+}
+
+// As lane_shift, but lanes 0 / 63 (no source lane) receive zero: one DPP move per 32-bit
+// word, with no copy of the old value.  For values whose edge lanes are discarded.
+template <bool LEFT>
+__device__ __forceinline__ float lane_shift_z(float v)
+{
+    return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), LEFT ? 0x138 : 0x130, 0xF, 0xF, true));
+}
+template <bool LEFT>
+__device__ __forceinline__ double lane_shift_z(double v)
+{
+    const unsigned long long b = (unsigned long long)__double_as_longlong(v);
+    const unsigned l2 = (unsigned)__builtin_amdgcn_mov_dpp((int)(unsigned)b, LEFT ? 0x138 : 0x130, 0xF, 0xF, true);
+    const unsigned h2 = (unsigned)__builtin_amdgcn_mov_dpp((int)(unsigned)(b >> 32), LEFT ? 0x138 : 0x130, 0xF, 0xF, true);
+    return __longlong_as_double((long long)(((unsigned long long)h2 << 32) | l2));
+}
+
 // splitmix64 (Steele, Lea, Flood 2014): r(seed, n) = mix(seed + (n+1)*golden)
 __host__ __device__ inline uint64_t mix64(uint64_t z)
 {

@@ -293,24 +293,6 @@ __device__ inline int find_scen(const Scen<R>* sc, int nscen, int strip)
     return lo;
 }
 
-// Lane shifts across the wave (DPP wave_shr:1 / wave_shl:1): from_left gives lane l the
-// value of lane l-1, from_right the value of lane l+1 (lanes 0 / 63 keep their own).
-__device__ __forceinline__ int from_left(int v) { return __builtin_amdgcn_update_dpp(v, v, 0x138, 0xF, 0xF, false); }
-__device__ __forceinline__ int from_right(int v) { return __builtin_amdgcn_update_dpp(v, v, 0x130, 0xF, 0xF, false); }
-template <bool LEFT> __device__ __forceinline__ float lane_shift(float v)
-{
-    const int b = __float_as_int(v);
-    return __int_as_float(LEFT ? from_left(b) : from_right(b));
-}
-template <bool LEFT> __device__ __forceinline__ double lane_shift(double v)
-{
-    const unsigned long long b = (unsigned long long)__double_as_longlong(v);
-    const int lo = (int)(unsigned)b, hi = (int)(unsigned)(b >> 32);
-    const unsigned l2 = (unsigned)(LEFT ? from_left(lo) : from_right(lo));
-    const unsigned h2 = (unsigned)(LEFT ? from_left(hi) : from_right(hi));
-    return __longlong_as_double((long long)(((unsigned long long)h2 << 32) | l2));
-}
-
 // One cell's state and the per-cell values every edge solve of it uses.
 template <typename R>
 struct CellV {

@@ -215,6 +215,7 @@ int gol_bits_run(appfis_ctx* ctx, uint8_t* b0, uint8_t* b1, int nx, int ny, int 
 extern "C" int appfis_gol(appfis_ctx* ctx, uint8_t* buf0, uint8_t* buf1, int nx, int ny, int gens, int variant,
                           int tsteps)
 {
+    (void)tsteps;  // reserved: the blocked kernels' depth comes from the tuned candidate
     APPFIS_REQUIRE(ctx && buf0 && buf1, "NULL argument");
     APPFIS_REQUIRE(buf0 != buf1, "the two buffers must differ");
     APPFIS_REQUIRE(nx >= 2 && ny >= 2, "periodic GoL needs an interior of at least 2x2 (got %dx%d)", nx, ny);

@@ -467,18 +467,18 @@ struct TbVariant {
 };
 
 static const TbVariant kTb[] = {
-    {"tb k16 g8 w16", k_step_tb<16, 8, 16>, 16, 8, 16}, {"tb k10 g4 w16", k_step_tb<10, 4, 16>, 10, 4, 16},
-    {"tb k20 g4 w8", k_step_tb<20, 4, 8>, 20, 4, 8},    {"tb k12 g4 w8", k_step_tb<12, 4, 8>, 12, 4, 8},
-    {"tb k24 g8 w16", k_step_tb<24, 8, 16>, 24, 8, 16}, {"tb k8 g4 w16", k_step_tb<8, 4, 16>, 8, 4, 16},
-    {"tb k12 g4 w16", k_step_tb<12, 4, 16>, 12, 4, 16}, {"tb k32 g8 w8", k_step_tb<32, 8, 8>, 32, 8, 8},
-    {"tb k16 g4 w8", k_step_tb<16, 4, 8>, 16, 4, 8},    {"tb k11 g5 w16", k_step_tb<11, 5, 16>, 11, 5, 16},
-    {"tb k13 g6 w16", k_step_tb<13, 6, 16>, 13, 6, 16},
-    {"tb1 k10 g4 w16", k_step_tb1<10, 4, 16>, 10, 4, 16}, {"tb1 k11 g5 w16", k_step_tb1<11, 5, 16>, 11, 5, 16},
-    {"tb1 k13 g6 w16", k_step_tb1<13, 6, 16>, 13, 6, 16}, {"tb1 k16 g8 w16", k_step_tb1<16, 8, 16>, 16, 8, 16},
-    {"tb1 k20 g4 w8", k_step_tb1<20, 4, 8>, 20, 4, 8},    {"tb1 k12 g4 w16", k_step_tb1<12, 4, 16>, 12, 4, 16},
+    {"tb k16 g8 w16", k_step_tb<16, 8, 16>, 16, 8, 16, 0}, {"tb k10 g4 w16", k_step_tb<10, 4, 16>, 10, 4, 16, 0},
+    {"tb k20 g4 w8", k_step_tb<20, 4, 8>, 20, 4, 8, 0},    {"tb k12 g4 w8", k_step_tb<12, 4, 8>, 12, 4, 8, 0},
+    {"tb k24 g8 w16", k_step_tb<24, 8, 16>, 24, 8, 16, 0}, {"tb k8 g4 w16", k_step_tb<8, 4, 16>, 8, 4, 16, 0},
+    {"tb k12 g4 w16", k_step_tb<12, 4, 16>, 12, 4, 16, 0}, {"tb k32 g8 w8", k_step_tb<32, 8, 8>, 32, 8, 8, 0},
+    {"tb k16 g4 w8", k_step_tb<16, 4, 8>, 16, 4, 8, 0},    {"tb k11 g5 w16", k_step_tb<11, 5, 16>, 11, 5, 16, 0},
+    {"tb k13 g6 w16", k_step_tb<13, 6, 16>, 13, 6, 16, 0},
+    {"tb1 k10 g4 w16", k_step_tb1<10, 4, 16>, 10, 4, 16, 0}, {"tb1 k11 g5 w16", k_step_tb1<11, 5, 16>, 11, 5, 16, 0},
+    {"tb1 k13 g6 w16", k_step_tb1<13, 6, 16>, 13, 6, 16, 0}, {"tb1 k16 g8 w16", k_step_tb1<16, 8, 16>, 16, 8, 16, 0},
+    {"tb1 k20 g4 w8", k_step_tb1<20, 4, 8>, 20, 4, 8, 0},    {"tb1 k12 g4 w16", k_step_tb1<12, 4, 16>, 12, 4, 16, 0},
     // 4K - 2G = 32 output rows per block like k10 g4 (256 blocks fill 256 CUs at 8192
     // rows), with more generations per launch to spread the launch's fixed load / store
-    {"tb1 k11 g6 w16", k_step_tb1<11, 6, 16>, 11, 6, 16}, {"tb1 k12 g8 w16", k_step_tb1<12, 8, 16>, 12, 8, 16},
+    {"tb1 k11 g6 w16", k_step_tb1<11, 6, 16>, 11, 6, 16, 0}, {"tb1 k12 g8 w16", k_step_tb1<12, 8, 16>, 12, 8, 16, 0},
     // full-row waves (nx = 2048 * WPL), also 32 output rows per block at 8192 rows
     {"tw4 k3 g8 w16", k_step_tw<4, 3, 8, 16>, 3, 8, 16, 4}, {"tw4 k4 g8 w12", k_step_tw<4, 4, 8, 12>, 4, 8, 12, 4},
     {"tw4 k6 g8 w8", k_step_tw<4, 6, 8, 8>, 6, 8, 8, 4},    {"tw2 k3 g8 w16", k_step_tw<2, 3, 8, 16>, 3, 8, 16, 2},

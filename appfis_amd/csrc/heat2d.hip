@@ -5,12 +5,13 @@
 //   u' = u + c*((((uN + uS) + uE) + uW) - 4u)      one rounding per operation, no FMA
 //
 // PLAIN : one sweep per launch, one output per thread, x along the wave.
-// CHOSEN: temporal blocking.  A block loads an (TX+2T) x (TY+2T) window of the field
-//         into LDS and advances it T sweeps in place (each sweep: compute into
-//         registers, barrier, write back, barrier), shrinking the computed region by one
-//         cell per sweep, then writes the TX x TY centre.  Dirichlet ghost cells are
-//         loaded and never updated, exactly as in the one-sweep form, so every interior
-//         value is bit-identical to PLAIN.  HBM/MALL traffic per sweep drops by ~T.
+// CHOSEN: temporal blocking.  A block loads a window of the field with a T-cell halo
+//         and advances it T sweeps on chip, then writes the window's centre; the valid
+//         region shrinks by one cell per sweep.  Two window forms are autotuned: in LDS
+//         (k_heat2d_tb) and in registers with DPP neighbours (k_heat2d_rt).  Dirichlet
+//         ghost cells are loaded and never updated, exactly as in the one-sweep form, so
+//         every interior value is bit-identical to PLAIN.  HBM/MALL traffic per sweep
+//         drops by ~T.
 #include "common.hpp"
 
 #include <vector>
@@ -135,21 +136,104 @@ struct HeatGrid {
     int nx, ny, tiles_x, tile0;
 };
 
-template <typename T, int TX, int TY, int TS, int NT>
-__global__ __launch_bounds__(NT) void k_heat2d_tb_ens(const HeatGrid<T>* __restrict__ tab, int ngrid, int par, T c,
-                                                      int steps)
+// The grid that owns tile `tile` of an ensemble launch: binary search over tile0 (the
+// same for every thread of the block).
+template <typename T>
+__device__ inline int grid_of_tile(const HeatGrid<T>* __restrict__ tab, int ngrid, int tile)
 {
-    const int tile = (int)xcd_tile(blockIdx.x, gridDim.x);
-    int lo = 0, hi = ngrid - 1;  // the grid of this tile: binary search over tile0 (uniform)
+    int lo = 0, hi = ngrid - 1;
     while (lo < hi) {
         const int mid = (lo + hi + 1) >> 1;
         if (tab[mid].tile0 <= tile) lo = mid;
         else hi = mid - 1;
     }
-    const HeatGrid<T>& G = tab[lo];
+    return lo;
+}
+
+// Register tiles: a block of NW waves holds a window of 64 columns x NW*K rows, lane l
+// owning column l of K consecutive rows (wave w: window rows w*K .. w*K+K-1) in
+// registers.  A sweep takes the east / west neighbours from lanes l+1 / l-1 by DPP and
+// the north / south ones from the lane's own registers; only each wave's first and last
+// rows go through LDS, in two slots alternating with the sweep's parity, so a sweep has
+// one barrier.  Cells at the window's edge read zero neighbours (the DPP shift at lanes
+// 0 / 63, the rows above the first wave and below the last), which spoils one more ring
+// per sweep; after TS sweeps the centre (64-2TS) x (NW*K-2TS) is exact.  Ghost cells keep
+// their loaded values, so every interior value equals the one-sweep form's.
+template <typename T, int K, int NW, int TS>
+__device__ inline void rt_tile(const T* __restrict__ u, T* __restrict__ v, int nx, int ny, T c, int steps, int tbx,
+                               int tby)
+{
+    constexpr int TX = 64 - 2 * TS, TY = NW * K - 2 * TS;
+    static_assert(TX > 0 && TY > 0, "the window must be wider and taller than its halo");
+    __shared__ T edge[2][NW][2][64];
+    const int W = nx + 2, H = ny + 2;
+    const int lane = (int)(threadIdx.x & 63), wv = (int)(threadIdx.x >> 6);
+    const int gx = tbx * TX + 1 - TS + lane;     // padded column of this lane
+    const int gy0 = tby * TY + 1 - TS + wv * K;  // padded row of R[0]
+    const bool col_in = gx >= 1 && gx <= nx;
+    T R[K];
+#pragma unroll
+    for (int k = 0; k < K; k++) {
+        const int gy = gy0 + k;
+        R[k] = (gx >= 0 && gx < W && gy >= 0 && gy < H) ? u[(size_t)gx + (size_t)W * (size_t)gy] : (T)0;
+    }
+    // one sweep; its edge rows use LDS slot b (sweeps alternate slots)
+    auto sweep = [&](const int b) {
+        edge[b][wv][0][lane] = R[0];
+        edge[b][wv][1][lane] = R[K - 1];
+        __syncthreads();
+        const T up = wv > 0 ? edge[b][wv - 1][1][lane] : (T)0;
+        const T dn = wv < NW - 1 ? edge[b][wv + 1][0][lane] : (T)0;
+        T above = up;  // the row above row k, before this sweep
+#pragma unroll
+        for (int k = 0; k < K; k++) {
+            const T uc = R[k];
+            T sum = above + (k == K - 1 ? dn : R[k + 1]);
+            sum = sum + lane_shift_z<false>(uc);  // east (lane l+1)
+            sum = sum + lane_shift_z<true>(uc);   // west (lane l-1)
+            sum = sum - (T)4 * uc;
+            const int gy = gy0 + k;
+            above = uc;
+            R[k] = (col_in && gy >= 1 && gy <= ny) ? uc + c * sum : uc;
+        }
+    };
+    // two sweeps per trip, so that the registers of consecutive sweeps are renamed
+    // rather than copied
+    for (int s = 0; s + 1 < steps; s += 2) {
+        sweep(0);
+        sweep(1);
+    }
+    if (steps & 1) sweep(0);
+    // the centre: lanes TS .. TS+TX-1, window rows TS .. TS+TY-1 (interior cells only)
+    if (lane >= TS && lane < TS + TX && col_in) {
+#pragma unroll
+        for (int k = 0; k < K; k++) {
+            const int r = wv * K + k, gy = gy0 + k;
+            if (r >= TS && r < TS + TY && gy >= 1 && gy <= ny) v[(size_t)gx + (size_t)W * (size_t)gy] = R[k];
+        }
+    }
+}
+
+template <typename T, int K, int NW, int TS>
+__global__ __launch_bounds__(64 * NW) void k_heat2d_rt(const T* __restrict__ u, T* __restrict__ v, int nx, int ny,
+                                                       T c, int steps)
+{
+    constexpr int TX = 64 - 2 * TS;
+    const int tiles_x = (nx + TX - 1) / TX;
+    const int t = (int)xcd_tile(blockIdx.x, gridDim.x);
+    const int tby = t / tiles_x;
+    rt_tile<T, K, NW, TS>(u, v, nx, ny, c, steps, t - tby * tiles_x, tby);
+}
+
+template <typename T, int K, int NW, int TS>
+__global__ __launch_bounds__(64 * NW) void k_heat2d_rt_ens(const HeatGrid<T>* __restrict__ tab, int ngrid, int par,
+                                                           T c, int steps)
+{
+    const int tile = (int)xcd_tile(blockIdx.x, gridDim.x);
+    const HeatGrid<T>& G = tab[grid_of_tile(tab, ngrid, tile)];
     const int t = tile - G.tile0;
     const int tby = t / G.tiles_x;
-    tb_tile<T, TX, TY, TS, NT>(G.b[par], G.b[par ^ 1], G.nx, G.ny, c, steps, t - tby * G.tiles_x, tby);
+    rt_tile<T, K, NW, TS>(G.b[par], G.b[par ^ 1], G.nx, G.ny, c, steps, t - tby * G.tiles_x, tby);
 }
 
 constexpr int PLAIN_RY = 4;
@@ -203,6 +287,24 @@ int run_tb_v(appfis_ctx* ctx, void* b0, void* b1, int nx, int ny, double c, int 
     return run_tb<T, TX, TY, TS, NT>(ctx, (T*)b0, (T*)b1, nx, ny, (T)c, iters, depth);
 }
 
+template <typename T, int K, int NW, int TS>
+int run_rt(appfis_ctx* ctx, void* b0, void* b1, int nx, int ny, double c, int iters, int depth)
+{
+    constexpr int TX = 64 - 2 * TS, TY = NW * K - 2 * TS;
+    T* buf[2] = {(T*)b0, (T*)b1};
+    const size_t tiles = (size_t)ceil_div(nx, TX) * ceil_div(ny, TY);
+    APPFIS_REQUIRE(tiles < (1u << 31), "heat2d: too many tiles");
+    std::vector<int> counts((size_t)iters);
+    int n = plan_launches(iters, depth < TS ? depth : TS, counts.data(), iters);
+    if (n <= 0) return invalid("heat2d: cannot split %d sweeps", iters);
+    for (int l = 0; l < n; l++) {
+        hipLaunchKernelGGL((k_heat2d_rt<T, K, NW, TS>), dim3((unsigned)tiles), dim3(64 * NW), 0, ctx->stream,
+                           buf[l & 1], buf[(l + 1) & 1], nx, ny, (T)c, counts[l]);
+        APPFIS_LAUNCH_CHECK();
+    }
+    return APPFIS_OK;
+}
+
 struct HeatVariant {
     const char* name;
     int (*fn)(appfis_ctx*, void*, void*, int, int, double, int, int);
@@ -222,6 +324,15 @@ static const HeatVariant kHeatF64[] = {
     {"tb 64x64 ts12 nt1024", run_tb_v<double, 64, 64, 12, 1024>, 12},
     {"tb 64x32 ts8 nt512", run_tb_v<double, 64, 32, 8, 512>, 8},
     {"tb 64x32 ts16 nt256", run_tb_v<double, 64, 32, 16>, 16},
+    // register tiles, named by output tile: 64 - 2TS columns x NW*K - 2TS rows (at 1024^2
+    // each fills at most 256 blocks, one per CU)
+    {"rt 48x96 ts8 k7 w16", run_rt<double, 7, 16, 8>, 8},
+    {"rt 48x96 ts8 k14 w8", run_rt<double, 14, 8, 8>, 8},
+    {"rt 56x88 ts4 k6 w16", run_rt<double, 6, 16, 4>, 4},
+    {"rt 40x120 ts12 k9 w16", run_rt<double, 9, 16, 12>, 12},
+    {"rt 32x128 ts16 k10 w16", run_rt<double, 10, 16, 16>, 16},
+    // two blocks per CU at 1024^2 (one computes while the other waits at its barrier)
+    {"rt 48x48 ts8 k4 w16", run_rt<double, 4, 16, 8>, 8},
 };
 static const HeatVariant kHeatF32[] = {
     {"tb 64x32 ts8 nt256", run_tb_v<float, 64, 32, 8>, 8},
@@ -231,6 +342,12 @@ static const HeatVariant kHeatF32[] = {
     {"tb 64x64 ts8 nt1024", run_tb_v<float, 64, 64, 8, 1024>, 8},
     {"tb 64x64 ts16 nt1024", run_tb_v<float, 64, 64, 16, 1024>, 16},
     {"tb 128x64 ts16 nt1024", run_tb_v<float, 128, 64, 16, 1024>, 16},
+    {"rt 48x96 ts8 k7 w16", run_rt<float, 7, 16, 8>, 8},
+    {"rt 48x96 ts8 k14 w8", run_rt<float, 14, 8, 8>, 8},
+    {"rt 56x88 ts4 k6 w16", run_rt<float, 6, 16, 4>, 4},
+    {"rt 40x120 ts12 k9 w16", run_rt<float, 9, 16, 12>, 12},
+    {"rt 32x128 ts16 k10 w16", run_rt<float, 10, 16, 16>, 16},
+    {"rt 48x48 ts8 k4 w16", run_rt<float, 4, 16, 8>, 8},
 };
 
 }  // namespace
@@ -296,9 +413,13 @@ extern "C" int appfis_heat2d(appfis_ctx* ctx, int dtype, void* buf0, void* buf1,
     return invalid("heat2d: unknown variant %d", variant);
 }
 
+template <typename T>
+using ens_kernel_t = void (*)(const HeatGrid<T>*, int, int, T, int);
+
+// Ensemble launches with output tiles TX x TY, at most TS sweeps per launch, NT threads.
 template <typename T, int TX, int TY, int TS, int NT>
-int run_ensemble(appfis_ctx* ctx, int ngrid, const int* nx, const int* ny, void* const* buf0, void* const* buf1,
-                 double c, int iters)
+int run_ensemble(appfis_ctx* ctx, ens_kernel_t<T> kern, int ngrid, const int* nx, const int* ny, void* const* buf0,
+                 void* const* buf1, double c, int iters)
 {
     std::vector<HeatGrid<T>> hg((size_t)ngrid);
     long long tiles = 0;
@@ -327,8 +448,7 @@ int run_ensemble(appfis_ctx* ctx, int ngrid, const int* nx, const int* ny, void*
     // the whole launch chain is captured once and replayed as one graph
     APPFIS_HIP_CHECK(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
     for (int l = 0; l < n; l++)
-        hipLaunchKernelGGL((k_heat2d_tb_ens<T, TX, TY, TS, NT>), dim3((unsigned)tiles), dim3(NT), 0, ctx->stream, tab,
-                           ngrid, l & 1, (T)c, counts[l]);
+        hipLaunchKernelGGL(kern, dim3((unsigned)tiles), dim3(NT), 0, ctx->stream, tab, ngrid, l & 1, (T)c, counts[l]);
     hipGraph_t graph = nullptr;
     hipError_t e = hipStreamEndCapture(ctx->stream, &graph);
     if (e != hipSuccess) return hip_fail(e, "heat2d ensemble capture", __FILE__, __LINE__);
@@ -354,8 +474,13 @@ extern "C" int appfis_heat2d_ensemble(appfis_ctx* ctx, int dtype, int ngrid, con
     APPFIS_REQUIRE(ngrid >= 1, "heat2d ensemble: need at least one grid");
     APPFIS_REQUIRE(iters >= 0, "iters must be >= 0");
     if (iters == 0) return APPFIS_OK;
-    if (dtype == APPFIS_F64) return run_ensemble<double, 64, 64, 12, 1024>(ctx, ngrid, nx, ny, buf0, buf1, c, iters);
-    if (dtype == APPFIS_F32) return run_ensemble<float, 64, 64, 8, 1024>(ctx, ngrid, nx, ny, buf0, buf1, c, iters);
+    // register tiles, 48 x 96 output cells, 8 sweeps per launch, 16 waves
+    if (dtype == APPFIS_F64)
+        return run_ensemble<double, 48, 96, 8, 1024>(ctx, k_heat2d_rt_ens<double, 7, 16, 8>, ngrid, nx, ny, buf0, buf1,
+                                                     c, iters);
+    if (dtype == APPFIS_F32)
+        return run_ensemble<float, 48, 96, 8, 1024>(ctx, k_heat2d_rt_ens<float, 7, 16, 8>, ngrid, nx, ny, buf0, buf1, c,
+                                                    iters);
     return invalid("heat2d ensemble: dtype must be F64 or F32");
 }
 

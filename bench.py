@@ -145,6 +145,17 @@ def bench_gol(ctx, hip, steps, warmup, variant, nx=8192, ny=8192):
     return res
 
 
+def heat_kernel_name(tname, dtype="double"):
+    """Kernel symbol of a Heat2D temporal-blocking candidate: "tb 64x64 ts12 nt1024" ->
+    k_heat2d_tb<double, 64, 64, 12, 1024> (output tile, sweeps, threads); "rt 48x96 ts8 k7
+    w16" -> k_heat2d_rt<double, 7, 16, 8> (rows per wave, waves, sweeps)."""
+    form, tile, ts, a, *rest = tname.split()
+    if form == "rt":
+        return f"k_heat2d_rt<{dtype}, {int(a[1:])}, {int(rest[0][1:])}, {int(ts[2:])}>"
+    tx, ty = tile.split("x")
+    return f"k_heat2d_tb<{dtype}, {tx}, {ty}, {int(ts[2:])}, {int(a[2:])}>"
+
+
 def gol_kernel_name(tname):
     """Kernel symbol of a bit-packed GoL candidate: "tb k12 g8 w16" -> k_step_tb<12, 8, 16>,
     "tw4 k4 g8 w12" -> k_step_tw<4, 4, 8, 12> (words per lane first); no candidate (widths
@@ -209,20 +220,16 @@ def bench_heat(ctx, hip, n=1024, iters=500):
         if variant == hip.PLAIN:
             launches = iters
         else:
-            tname = _tuned(ctx, 1, hip.F64)  # e.g. "tb 64x64 ts8 nt1024"
-            parts = tname.split()
-            ts, nt = int(parts[2][2:]), int(parts[3][2:])
+            tname = _tuned(ctx, 1, hip.F64)  # e.g. "tb 64x64 ts8 nt1024", "rt 48x96 ts8 k7 w16"
+            ts = int(tname.split()[2][2:])
             launches = len(_plan(iters, ts))
         per_launch_sweeps = iters / launches
-        if variant == hip.PLAIN:
-            kname = "k_heat2d_plain<double, 4>"
-        else:
-            tx, ty = parts[1].split("x")
-            kname = f"k_heat2d_tb<double, {tx}, {ty}, {ts}, {nt}>"
+        kname = "k_heat2d_plain<double, 4>" if variant == hip.PLAIN else heat_kernel_name(tname)
         r = {"ms": round(ms, 4), "glups": round(n * n * iters / (ms * 1e-3) / 1e9, 2), "checksum": chk,
              "roofline": roofline(16.0 * n * n * per_launch_sweeps, ms / launches, kname)}
         if variant == hip.CHOSEN:
             r["tuned"] = tname
+            r["autotune"] = [l for l in ctx.tune_report().splitlines() if l.startswith("autotune kernel 1 ")]
         out[name] = r
     # mixed precision: the same sweeps in FP32 (temporal blocking), error against FP64
     ctx.fill_uniform(a, 1, 1001)
@@ -312,10 +319,9 @@ def bench_heat_ensemble(ctx, hip, ngrid=64, iters=500):
         res[name] = [(a, b)[iters % 2].download().sum() for a, b, _, _ in grids[:2]]
         r = {"ms": round(ms, 3), "glups": round(cells * iters / (ms * 1e-3) / 1e9, 2)}
         if name == "ensemble":
-            # the FP64 ensemble runs 64 x 64 tiles, 12 sweeps per launch, 1024 threads
-            launches = len(_plan(iters, 12))
-            r["roofline"] = roofline(16.0 * cells * iters / launches, ms / launches,
-                                     "k_heat2d_tb_ens<double, 64, 64, 12, 1024>")
+            # the FP64 ensemble runs 48 x 96 register tiles (7 rows x 16 waves), 8 sweeps per launch
+            launches = len(_plan(iters, 8))
+            r["roofline"] = roofline(16.0 * cells * iters / launches, ms / launches, "k_heat2d_rt_ens<double, 7, 16, 8>")
         out[name] = r
     assert res["ensemble"] == res["plain_one_at_a_time"] == res["chosen_one_at_a_time"]
     out["ensemble"]["speedup_vs_plain"] = round(out["plain_one_at_a_time"]["ms"] / out["ensemble"]["ms"], 3)

@@ -82,3 +82,10 @@ def test_gol_kernel_names_match_the_profiled_symbols():
                         ("tb k10 g4 w16", "k_step_tb<10, 4, 16>"), (None, "k_step<8>")):
         assert bench.gol_kernel_name(tname) == want
         assert want in names, want
+
+
+def test_heat_kernel_names():
+    assert bench.heat_kernel_name("tb 64x64 ts12 nt1024") == "k_heat2d_tb<double, 64, 64, 12, 1024>"
+    assert bench.heat_kernel_name("rt 48x96 ts8 k7 w16") == "k_heat2d_rt<double, 7, 16, 8>"
+    assert bench._short("void (anonymous namespace)::k_heat2d_rt<double, 7, 16, 8>(double const*, double*, int, "
+                        "int, double, int)") == "k_heat2d_rt<double, 7, 16, 8>"

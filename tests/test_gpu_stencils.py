@@ -43,9 +43,10 @@ def test_heat2d_chosen_depths(ctx, tsteps):
 
 
 @pytest.mark.parametrize("dt", [hip.F64, hip.F32])
-def test_heat2d_every_candidate_bit_exact(ctx, dt):
+@pytest.mark.parametrize("nx,ny,iters", [(200, 150, 37), (5, 300, 19), (1030, 9, 21)])
+def test_heat2d_every_candidate_bit_exact(ctx, dt, nx, ny, iters):
     # every temporal-blocking candidate the autotuner may pick, forced one at a time
-    nx, ny, iters = 200, 150, 37
+    # (partial tiles in both axes; a grid narrower and one shorter than any tile)
     a, d0, d1 = _heat_pair(ctx, nx, ny, dt)
     ref = oracle.heat2d(a.copy(), np.zeros_like(a), nx, ny, 0.1, iters)
     names = hip.heat2d_candidates(dt)
