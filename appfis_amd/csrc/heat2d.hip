@@ -150,51 +150,75 @@ __device__ inline int grid_of_tile(const HeatGrid<T>* __restrict__ tab, int ngri
     return lo;
 }
 
-// Register tiles: a block of NW waves holds a window of 64 columns x NW*K rows, lane l
-// owning column l of K consecutive rows (wave w: window rows w*K .. w*K+K-1) in
-// registers.  A sweep takes the east / west neighbours from lanes l+1 / l-1 by DPP and
-// the north / south ones from the lane's own registers; only each wave's first and last
-// rows go through LDS, in two slots alternating with the sweep's parity, so a sweep has
-// one barrier.  Cells at the window's edge read zero neighbours (the DPP shift at lanes
-// 0 / 63, the rows above the first wave and below the last), which spoils one more ring
-// per sweep; after TS sweeps the centre (64-2TS) x (NW*K-2TS) is exact.  Ghost cells keep
+// Register tiles: a block of NW waves holds a window of 64*WPL columns x NW*K rows, lane
+// l owning columns WPL*l .. WPL*l+WPL-1 of K consecutive rows (wave w: window rows w*K ..
+// w*K+K-1) in registers.  A sweep takes a cell's east / west neighbours from the lane's
+// own registers, or by DPP from lanes l+1 / l-1 for its outer columns, and the north /
+// south ones from the lane's own registers; only each wave's first and last rows go
+// through LDS, in two slots alternating with the sweep's parity, so a sweep has one
+// barrier.  Cells at the window's edge read zero neighbours (the DPP shift at lanes 0 /
+// 63, the rows above the first wave and below the last), which spoils one more ring per
+// sweep; after TS sweeps the centre (64*WPL-2TS) x (NW*K-2TS) is exact.  Ghost cells keep
 // their loaded values, so every interior value equals the one-sweep form's.
-template <typename T, int K, int NW, int TS>
+template <typename T, int K, int NW, int TS, int WPL>
 __device__ inline void rt_tile(const T* __restrict__ u, T* __restrict__ v, int nx, int ny, T c, int steps, int tbx,
                                int tby)
 {
-    constexpr int TX = 64 - 2 * TS, TY = NW * K - 2 * TS;
+    constexpr int TX = 64 * WPL - 2 * TS, TY = NW * K - 2 * TS;
     static_assert(TX > 0 && TY > 0, "the window must be wider and taller than its halo");
-    __shared__ T edge[2][NW][2][64];
+    __shared__ T edge[2][NW][2][WPL][64];
     const int W = nx + 2, H = ny + 2;
     const int lane = (int)(threadIdx.x & 63), wv = (int)(threadIdx.x >> 6);
-    const int gx = tbx * TX + 1 - TS + lane;     // padded column of this lane
-    const int gy0 = tby * TY + 1 - TS + wv * K;  // padded row of R[0]
-    const bool col_in = gx >= 1 && gx <= nx;
-    T R[K];
+    const int gx0 = tbx * TX + 1 - TS + WPL * lane;  // padded column of R[.][0]
+    const int gy0 = tby * TY + 1 - TS + wv * K;      // padded row of R[0][.]
+    bool col_in[WPL];
+#pragma unroll
+    for (int w = 0; w < WPL; w++) col_in[w] = gx0 + w >= 1 && gx0 + w <= nx;
+    T R[K][WPL];
 #pragma unroll
     for (int k = 0; k < K; k++) {
         const int gy = gy0 + k;
-        R[k] = (gx >= 0 && gx < W && gy >= 0 && gy < H) ? u[(size_t)gx + (size_t)W * (size_t)gy] : (T)0;
+#pragma unroll
+        for (int w = 0; w < WPL; w++) {
+            const int gx = gx0 + w;
+            R[k][w] = (gx >= 0 && gx < W && gy >= 0 && gy < H) ? u[(size_t)gx + (size_t)W * (size_t)gy] : (T)0;
+        }
     }
     // one sweep; its edge rows use LDS slot b (sweeps alternate slots)
     auto sweep = [&](const int b) {
-        edge[b][wv][0][lane] = R[0];
-        edge[b][wv][1][lane] = R[K - 1];
+#pragma unroll
+        for (int w = 0; w < WPL; w++) {
+            edge[b][wv][0][w][lane] = R[0][w];
+            edge[b][wv][1][w][lane] = R[K - 1][w];
+        }
         __syncthreads();
-        const T up = wv > 0 ? edge[b][wv - 1][1][lane] : (T)0;
-        const T dn = wv < NW - 1 ? edge[b][wv + 1][0][lane] : (T)0;
-        T above = up;  // the row above row k, before this sweep
+        T above[WPL], dn[WPL];  // the row above row k before this sweep; the row below the wave
+#pragma unroll
+        for (int w = 0; w < WPL; w++) {
+            above[w] = wv > 0 ? edge[b][wv - 1][1][w][lane] : (T)0;
+            dn[w] = wv < NW - 1 ? edge[b][wv + 1][0][w][lane] : (T)0;
+        }
 #pragma unroll
         for (int k = 0; k < K; k++) {
-            const T uc = R[k];
-            T sum = above + (k == K - 1 ? dn : R[k + 1]);
-            sum = sum + lane_shift_z<false>(uc);  // east (lane l+1)
-            sum = sum + lane_shift_z<true>(uc);   // west (lane l-1)
-            sum = sum - (T)4 * uc;
+            const T east = lane_shift_z<false>(R[k][0]);       // lane l+1's first column
+            const T west = lane_shift_z<true>(R[k][WPL - 1]);  // lane l-1's last column
             const int gy = gy0 + k;
-            above = uc;
-            R[k] = (col_in && gy >= 1 && gy <= ny) ? uc + c * sum : uc;
+            const bool row_in = gy >= 1 && gy <= ny;
+            T nv[WPL];
+#pragma unroll
+            for (int w = 0; w < WPL; w++) {
+                const T uc = R[k][w];
+                T sum = above[w] + (k == K - 1 ? dn[w] : R[k + 1][w]);
+                sum = sum + (w == WPL - 1 ? east : R[k][w + 1]);
+                sum = sum + (w == 0 ? west : R[k][w - 1]);
+                sum = sum - (T)4 * uc;
+                nv[w] = (col_in[w] && row_in) ? uc + c * sum : uc;
+            }
+#pragma unroll
+            for (int w = 0; w < WPL; w++) {
+                above[w] = R[k][w];
+                R[k][w] = nv[w];
+            }
         }
     };
     // two sweeps per trip, so that the registers of consecutive sweeps are renamed
@@ -204,28 +228,31 @@ __device__ inline void rt_tile(const T* __restrict__ u, T* __restrict__ v, int n
         sweep(1);
     }
     if (steps & 1) sweep(0);
-    // the centre: lanes TS .. TS+TX-1, window rows TS .. TS+TY-1 (interior cells only)
-    if (lane >= TS && lane < TS + TX && col_in) {
+    // the centre: window columns TS .. TS+TX-1, rows TS .. TS+TY-1 (interior cells only)
 #pragma unroll
-        for (int k = 0; k < K; k++) {
-            const int r = wv * K + k, gy = gy0 + k;
-            if (r >= TS && r < TS + TY && gy >= 1 && gy <= ny) v[(size_t)gx + (size_t)W * (size_t)gy] = R[k];
+    for (int k = 0; k < K; k++) {
+        const int r = wv * K + k, gy = gy0 + k;
+        if (r < TS || r >= TS + TY || gy < 1 || gy > ny) continue;
+#pragma unroll
+        for (int w = 0; w < WPL; w++) {
+            const int lx = WPL * lane + w;
+            if (lx >= TS && lx < TS + TX && col_in[w]) v[(size_t)(gx0 + w) + (size_t)W * (size_t)gy] = R[k][w];
         }
     }
 }
 
-template <typename T, int K, int NW, int TS>
+template <typename T, int K, int NW, int TS, int WPL>
 __global__ __launch_bounds__(64 * NW) void k_heat2d_rt(const T* __restrict__ u, T* __restrict__ v, int nx, int ny,
                                                        T c, int steps)
 {
-    constexpr int TX = 64 - 2 * TS;
+    constexpr int TX = 64 * WPL - 2 * TS;
     const int tiles_x = (nx + TX - 1) / TX;
     const int t = (int)xcd_tile(blockIdx.x, gridDim.x);
     const int tby = t / tiles_x;
-    rt_tile<T, K, NW, TS>(u, v, nx, ny, c, steps, t - tby * tiles_x, tby);
+    rt_tile<T, K, NW, TS, WPL>(u, v, nx, ny, c, steps, t - tby * tiles_x, tby);
 }
 
-template <typename T, int K, int NW, int TS>
+template <typename T, int K, int NW, int TS, int WPL>
 __global__ __launch_bounds__(64 * NW) void k_heat2d_rt_ens(const HeatGrid<T>* __restrict__ tab, int ngrid, int par,
                                                            T c, int steps)
 {
@@ -233,7 +260,7 @@ __global__ __launch_bounds__(64 * NW) void k_heat2d_rt_ens(const HeatGrid<T>* __
     const HeatGrid<T>& G = tab[grid_of_tile(tab, ngrid, tile)];
     const int t = tile - G.tile0;
     const int tby = t / G.tiles_x;
-    rt_tile<T, K, NW, TS>(G.b[par], G.b[par ^ 1], G.nx, G.ny, c, steps, t - tby * G.tiles_x, tby);
+    rt_tile<T, K, NW, TS, WPL>(G.b[par], G.b[par ^ 1], G.nx, G.ny, c, steps, t - tby * G.tiles_x, tby);
 }
 
 constexpr int PLAIN_RY = 4;
@@ -287,10 +314,10 @@ int run_tb_v(appfis_ctx* ctx, void* b0, void* b1, int nx, int ny, double c, int 
     return run_tb<T, TX, TY, TS, NT>(ctx, (T*)b0, (T*)b1, nx, ny, (T)c, iters, depth);
 }
 
-template <typename T, int K, int NW, int TS>
+template <typename T, int K, int NW, int TS, int WPL = 1>
 int run_rt(appfis_ctx* ctx, void* b0, void* b1, int nx, int ny, double c, int iters, int depth)
 {
-    constexpr int TX = 64 - 2 * TS, TY = NW * K - 2 * TS;
+    constexpr int TX = 64 * WPL - 2 * TS, TY = NW * K - 2 * TS;
     T* buf[2] = {(T*)b0, (T*)b1};
     const size_t tiles = (size_t)ceil_div(nx, TX) * ceil_div(ny, TY);
     APPFIS_REQUIRE(tiles < (1u << 31), "heat2d: too many tiles");
@@ -298,7 +325,7 @@ int run_rt(appfis_ctx* ctx, void* b0, void* b1, int nx, int ny, double c, int it
     int n = plan_launches(iters, depth < TS ? depth : TS, counts.data(), iters);
     if (n <= 0) return invalid("heat2d: cannot split %d sweeps", iters);
     for (int l = 0; l < n; l++) {
-        hipLaunchKernelGGL((k_heat2d_rt<T, K, NW, TS>), dim3((unsigned)tiles), dim3(64 * NW), 0, ctx->stream,
+        hipLaunchKernelGGL((k_heat2d_rt<T, K, NW, TS, WPL>), dim3((unsigned)tiles), dim3(64 * NW), 0, ctx->stream,
                            buf[l & 1], buf[(l + 1) & 1], nx, ny, (T)c, counts[l]);
         APPFIS_LAUNCH_CHECK();
     }
@@ -333,6 +360,9 @@ static const HeatVariant kHeatF64[] = {
     {"rt 32x128 ts16 k10 w16", run_rt<double, 10, 16, 16>, 16},
     // two blocks per CU at 1024^2 (one computes while the other waits at its barrier)
     {"rt 48x48 ts8 k4 w16", run_rt<double, 4, 16, 8>, 8},
+    // two columns per lane: one horizontal neighbour of each cell is in the lane's
+    // registers (at 1024^2 it loses to one column per lane: 0.064 against 0.060 ms)
+    {"rt2 112x48 ts8 k4 w16", run_rt<double, 4, 16, 8, 2>, 8},
 };
 static const HeatVariant kHeatF32[] = {
     {"tb 64x32 ts8 nt256", run_tb_v<float, 64, 32, 8>, 8},
@@ -348,6 +378,7 @@ static const HeatVariant kHeatF32[] = {
     {"rt 40x120 ts12 k9 w16", run_rt<float, 9, 16, 12>, 12},
     {"rt 32x128 ts16 k10 w16", run_rt<float, 10, 16, 16>, 16},
     {"rt 48x48 ts8 k4 w16", run_rt<float, 4, 16, 8>, 8},
+    {"rt2 112x48 ts8 k4 w16", run_rt<float, 4, 16, 8, 2>, 8},
 };
 
 }  // namespace
@@ -476,10 +507,10 @@ extern "C" int appfis_heat2d_ensemble(appfis_ctx* ctx, int dtype, int ngrid, con
     if (iters == 0) return APPFIS_OK;
     // register tiles, 48 x 96 output cells, 8 sweeps per launch, 16 waves
     if (dtype == APPFIS_F64)
-        return run_ensemble<double, 48, 96, 8, 1024>(ctx, k_heat2d_rt_ens<double, 7, 16, 8>, ngrid, nx, ny, buf0, buf1,
+        return run_ensemble<double, 48, 96, 8, 1024>(ctx, k_heat2d_rt_ens<double, 7, 16, 8, 1>, ngrid, nx, ny, buf0, buf1,
                                                      c, iters);
     if (dtype == APPFIS_F32)
-        return run_ensemble<float, 48, 96, 8, 1024>(ctx, k_heat2d_rt_ens<float, 7, 16, 8>, ngrid, nx, ny, buf0, buf1, c,
+        return run_ensemble<float, 48, 96, 8, 1024>(ctx, k_heat2d_rt_ens<float, 7, 16, 8, 1>, ngrid, nx, ny, buf0, buf1, c,
                                                     iters);
     return invalid("heat2d ensemble: dtype must be F64 or F32");
 }

@@ -148,10 +148,12 @@ def bench_gol(ctx, hip, steps, warmup, variant, nx=8192, ny=8192):
 def heat_kernel_name(tname, dtype="double"):
     """Kernel symbol of a Heat2D temporal-blocking candidate: "tb 64x64 ts12 nt1024" ->
     k_heat2d_tb<double, 64, 64, 12, 1024> (output tile, sweeps, threads); "rt 48x96 ts8 k7
-    w16" -> k_heat2d_rt<double, 7, 16, 8> (rows per wave, waves, sweeps)."""
+    w16" -> k_heat2d_rt<double, 7, 16, 8, 1> (rows per wave, waves, sweeps, columns per
+    lane: "rt2" holds two)."""
     form, tile, ts, a, *rest = tname.split()
-    if form == "rt":
-        return f"k_heat2d_rt<{dtype}, {int(a[1:])}, {int(rest[0][1:])}, {int(ts[2:])}>"
+    if form.startswith("rt"):
+        wpl = int(form[2:] or 1)
+        return f"k_heat2d_rt<{dtype}, {int(a[1:])}, {int(rest[0][1:])}, {int(ts[2:])}, {wpl}>"
     tx, ty = tile.split("x")
     return f"k_heat2d_tb<{dtype}, {tx}, {ty}, {int(ts[2:])}, {int(a[2:])}>"
 
@@ -321,7 +323,7 @@ def bench_heat_ensemble(ctx, hip, ngrid=64, iters=500):
         if name == "ensemble":
             # the FP64 ensemble runs 48 x 96 register tiles (7 rows x 16 waves), 8 sweeps per launch
             launches = len(_plan(iters, 8))
-            r["roofline"] = roofline(16.0 * cells * iters / launches, ms / launches, "k_heat2d_rt_ens<double, 7, 16, 8>")
+            r["roofline"] = roofline(16.0 * cells * iters / launches, ms / launches, "k_heat2d_rt_ens<double, 7, 16, 8, 1>")
         out[name] = r
     assert res["ensemble"] == res["plain_one_at_a_time"] == res["chosen_one_at_a_time"]
     out["ensemble"]["speedup_vs_plain"] = round(out["plain_one_at_a_time"]["ms"] / out["ensemble"]["ms"], 3)

@@ -86,6 +86,7 @@ def test_gol_kernel_names_match_the_profiled_symbols():
 
 def test_heat_kernel_names():
     assert bench.heat_kernel_name("tb 64x64 ts12 nt1024") == "k_heat2d_tb<double, 64, 64, 12, 1024>"
-    assert bench.heat_kernel_name("rt 48x96 ts8 k7 w16") == "k_heat2d_rt<double, 7, 16, 8>"
-    assert bench._short("void (anonymous namespace)::k_heat2d_rt<double, 7, 16, 8>(double const*, double*, int, "
-                        "int, double, int)") == "k_heat2d_rt<double, 7, 16, 8>"
+    assert bench.heat_kernel_name("rt 48x96 ts8 k7 w16") == "k_heat2d_rt<double, 7, 16, 8, 1>"
+    assert bench.heat_kernel_name("rt2 112x48 ts8 k8 w8") == "k_heat2d_rt<double, 8, 8, 8, 2>"
+    assert bench._short("void (anonymous namespace)::k_heat2d_rt<double, 7, 16, 8, 1>(double const*, double*, int, "
+                        "int, double, int)") == "k_heat2d_rt<double, 7, 16, 8, 1>"
