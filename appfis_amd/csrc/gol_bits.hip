@@ -422,21 +422,14 @@ __global__ __launch_bounds__(64 * NWAVES) void k_step_tw(const uint32_t* __restr
         reinterpret_cast<V*>(hs[b][wv][1][1])[lane] = VW::put(h1[K]);
         __syncthreads();
         {
-            // rows -1 and K (zero outside the block's window): whole-vector reads from the
-            // neighbouring waves' slots, a wave-uniform select at the block edge
+            // rows -1 and K: whole-vector reads from the neighbouring waves' slots.  The
+            // block's first and last waves read their own slots instead: rows outside the
+            // window only reach the halo rows, which are never stored.
             const int up = wv > 0 ? wv - 1 : wv, dn = wv < NWAVES - 1 ? wv + 1 : wv;
-            uint32_t a0[WPL], a1[WPL], z0[WPL], z1[WPL];
-            VW::get(reinterpret_cast<const V*>(hs[b][up][1][0])[lane], a0);
-            VW::get(reinterpret_cast<const V*>(hs[b][up][1][1])[lane], a1);
-            VW::get(reinterpret_cast<const V*>(hs[b][dn][0][0])[lane], z0);
-            VW::get(reinterpret_cast<const V*>(hs[b][dn][0][1])[lane], z1);
-#pragma unroll
-            for (int w = 0; w < WPL; w++) {
-                h0[0][w] = wv > 0 ? a0[w] : 0u;
-                h1[0][w] = wv > 0 ? a1[w] : 0u;
-                h0[K + 1][w] = wv < NWAVES - 1 ? z0[w] : 0u;
-                h1[K + 1][w] = wv < NWAVES - 1 ? z1[w] : 0u;
-            }
+            VW::get(reinterpret_cast<const V*>(hs[b][up][1][0])[lane], h0[0]);
+            VW::get(reinterpret_cast<const V*>(hs[b][up][1][1])[lane], h1[0]);
+            VW::get(reinterpret_cast<const V*>(hs[b][dn][0][0])[lane], h0[K + 1]);
+            VW::get(reinterpret_cast<const V*>(hs[b][dn][0][1])[lane], h1[K + 1]);
         }
 #pragma unroll
         for (int k = 0; k < K; k++)
