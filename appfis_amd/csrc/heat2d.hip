@@ -156,10 +156,11 @@ __device__ inline int grid_of_tile(const HeatGrid<T>* __restrict__ tab, int ngri
 // own registers, or by DPP from lanes l+1 / l-1 for its outer columns, and the north /
 // south ones from the lane's own registers; only each wave's first and last rows go
 // through LDS, in two slots alternating with the sweep's parity, so a sweep has one
-// barrier.  Cells at the window's edge read zero neighbours (the DPP shift at lanes 0 /
-// 63, the rows above the first wave and below the last), which spoils one more ring per
-// sweep; after TS sweeps the centre (64*WPL-2TS) x (NW*K-2TS) is exact.  Ghost cells keep
-// their loaded values, so every interior value equals the one-sweep form's.
+// barrier.  Cells at the window's edge read meaningless neighbours (zero from the DPP
+// shift at lanes 0 / 63, the wave's own edge rows above the first wave and below the
+// last), which spoils one more ring per sweep; after TS sweeps the centre
+// (64*WPL-2TS) x (NW*K-2TS) is exact.  Ghost cells keep their loaded values, so every
+// interior value equals the one-sweep form's.
 template <typename T, int K, int NW, int TS, int WPL>
 __device__ inline void rt_tile(const T* __restrict__ u, T* __restrict__ v, int nx, int ny, T c, int steps, int tbx,
                                int tby)
@@ -192,11 +193,15 @@ __device__ inline void rt_tile(const T* __restrict__ u, T* __restrict__ v, int n
             edge[b][wv][1][w][lane] = R[K - 1][w];
         }
         __syncthreads();
-        T above[WPL], dn[WPL];  // the row above row k before this sweep; the row below the wave
+        // the row above row k before this sweep, and the row below the wave.  The block's
+        // first and last waves read their own slots instead: rows outside the window only
+        // reach the halo ring, which is never stored.
+        const int wu = wv > 0 ? wv - 1 : wv, wd = wv < NW - 1 ? wv + 1 : wv;
+        T above[WPL], dn[WPL];
 #pragma unroll
         for (int w = 0; w < WPL; w++) {
-            above[w] = wv > 0 ? edge[b][wv - 1][1][w][lane] : (T)0;
-            dn[w] = wv < NW - 1 ? edge[b][wv + 1][0][w][lane] : (T)0;
+            above[w] = edge[b][wu][1][w][lane];
+            dn[w] = edge[b][wd][0][w][lane];
         }
 #pragma unroll
         for (int k = 0; k < K; k++) {
