@@ -176,6 +176,18 @@ __device__ __forceinline__ double lane_shift_z(double v)
     return __longlong_as_double((long long)(((unsigned long long)h2 << 32) | l2));
 }
 
+// Global memory.  A pointer that a kernel reads from a table in memory (rather than from
+// its arguments) is generic to the compiler, which then emits flat loads and stores; flat
+// accesses count against both the vector-memory and the LDS/scalar wait counters, so
+// every wait for one of them waits for all.  to_global states the address space.
+template <typename T>
+using gmem = T __attribute__((address_space(1)));
+template <typename T>
+__device__ __forceinline__ gmem<T>* to_global(T* p)
+{
+    return (gmem<T>*)p;
+}
+
 // splitmix64 (Steele, Lea, Flood 2014): r(seed, n) = mix(seed + (n+1)*golden)
 __host__ __device__ inline uint64_t mix64(uint64_t z)
 {

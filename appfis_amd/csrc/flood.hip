@@ -299,18 +299,17 @@ struct CellV {
     R h, qx, qy, z, n2, sq, u, v;
 };
 
+// Unconditional loads (the compiler can then count them in flight): callers clamp the
+// offset into the array, and the values of lanes or rows outside it are never used.
 template <typename R>
-__device__ inline void cell_load(CellV<R>& c, const R* h, const R* qx, const R* qy, const R* z, const R* n2, bool in,
-                                 size_t off)
+__device__ inline void cell_load(CellV<R>& c, const gmem<R>* h, const gmem<R>* qx, const gmem<R>* qy, const gmem<R>* z,
+                                 const gmem<R>* n2, size_t off)
 {
-    c.h = c.qx = c.qy = c.z = c.n2 = (R)0.0;
-    if (in) {
-        c.h = h[off];
-        c.qx = qx[off];
-        c.qy = qy[off];
-        c.z = z[off];
-        c.n2 = n2[off];
-    }
+    c.h = h[off];
+    c.qx = qx[off];
+    c.qy = qy[off];
+    c.z = z[off];
+    c.n2 = n2[off];
 }
 
 template <typename R>
@@ -354,12 +353,15 @@ __global__ __launch_bounds__(BT) void k_flux_march(const Scen<R>* __restrict__ s
     const bool colmem = col <= nx + 1;
     const bool cell = lane >= 1 && lane <= MW && col <= nx;
     const bool xedge = lane >= 1 && colmem;  // edge (col-1, col) with col-1 <= nx
-    const R* h = S.h[par];
-    const R* qx = S.qx[par];
-    const R* qy = S.qy[par];
-    const R* z = S.z;
-    const R* n2 = S.n2;
-    const int o = par ^ 1;
+    const gmem<R>* h = to_global((const R*)S.h[par]);
+    const gmem<R>* qx = to_global((const R*)S.qx[par]);
+    const gmem<R>* qy = to_global((const R*)S.qy[par]);
+    const gmem<R>* z = to_global(S.z);
+    const gmem<R>* n2 = to_global(S.n2);
+    const int o = par ^ 1;  // the buffer written
+    gmem<R>* hout = to_global(S.h[o]);
+    gmem<R>* qxout = to_global(S.qx[o]);
+    gmem<R>* qyout = to_global(S.qy[o]);
     const R dt = (R)S.st->dt;
     const R dx = (R)P.dx, hextra = (R)P.hextra, rain = (R)P.rain;
     const R G = (R)9.81, EPS12 = (R)1e-12;
@@ -367,15 +369,16 @@ __global__ __launch_bounds__(BT) void k_flux_march(const Scen<R>* __restrict__ s
     const RhsScale<R> scale(dt, dx);
 
     CellV<R> prev, cur, nxt;
-    cell_load(prev, h, qx, qy, z, n2, colmem, (size_t)col + W * (size_t)(j0 - 1));
-    cell_load(cur, h, qx, qy, z, n2, colmem, (size_t)col + W * (size_t)j0);
+    const size_t colc = (size_t)(colmem ? col : nx + 1);  // lanes past the array load column nx+1
+    cell_load(prev, h, qx, qy, z, n2, colc + W * (size_t)(j0 - 1));
+    cell_load(cur, h, qx, qy, z, n2, colc + W * (size_t)j0);
     cell_derive(prev, hextra);
     // rhs*0 / rhs*1 of the previous row's cell (h, qx, qy)
     R p0[3] = {(R)0, (R)0, (R)0}, p1[3] = {(R)0, (R)0, (R)0};
     for (int r = 0; r <= rend; r++) {
         const int j = j0 + r;
         // prefetch row j+1 while row j is solved
-        cell_load(nxt, h, qx, qy, z, n2, colmem && r < rend, (size_t)col + W * (size_t)(j + 1));
+        cell_load(nxt, h, qx, qy, z, n2, colc + W * (size_t)min(j + 1, ny + 1));  // unused at r == rend
         cell_derive(cur, hextra);
         R c0[3] = {(R)0, (R)0, (R)0}, c1[3] = {(R)0, (R)0, (R)0};
         if (r < MR) {  // uniform: row j is a strip row
@@ -447,9 +450,9 @@ __global__ __launch_bounds__(BT) void k_flux_march(const Scen<R>* __restrict__ s
                     qyij = my;
                 }
             }
-            S.h[o][c] = hij + rain * dt;
-            S.qx[o][c] = qxij;
-            S.qy[o][c] = qyij;
+            hout[c] = hij + rain * dt;
+            qxout[c] = qxij;
+            qyout[c] = qyij;
         }
         prev = cur;
         cur = nxt;
@@ -515,13 +518,13 @@ __global__ __launch_bounds__(BT) void k_fix_march(const Scen<R>* __restrict__ sc
     const bool colmem = col <= nx + 1;
     const bool cell = lane >= 1 && lane <= MW && col <= nx;
     const int nr = min(MR, ny + 1 - j0);  // cell rows of the strip (j0 <= ny)
-    const R* h = S.h[par];
-    R* qx = S.qx[par];
-    R* qy = S.qy[par];
-    const R* z = S.z;
+    const gmem<R>* h = to_global((const R*)S.h[par]);
+    gmem<R>* qx = to_global(S.qx[par]);
+    gmem<R>* qy = to_global(S.qy[par]);
+    const gmem<R>* z = to_global(S.z);
     const R G = (R)9.81, EPS12 = (R)1e-12;
     const R hextra = (R)P.hextra, dx = (R)P.dx, cn = (R)P.cn;
-    auto ld = [&](const R* a, int row) { return colmem ? a[(size_t)col + W * (size_t)row] : (R)0.0; };
+    auto ld = [&](const gmem<R>* a, int row) { return colmem ? a[(size_t)col + W * (size_t)row] : (R)0.0; };
     R cfl = RealLim<R>::max();
     double vs = 0.0, vc = 0.0;
     R hm = ld(h, j0 - 1), hc = ld(h, j0), zm = (R)0.0, zc = (R)0.0;
