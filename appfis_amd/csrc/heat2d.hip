@@ -168,6 +168,8 @@ __device__ inline void rt_tile(const T* __restrict__ u, T* __restrict__ v, int n
     constexpr int TX = 64 * WPL - 2 * TS, TY = NW * K - 2 * TS;
     static_assert(TX > 0 && TY > 0, "the window must be wider and taller than its halo");
     __shared__ T edge[2][NW][2][WPL][64];
+    const gmem<T>* ug = to_global(u);  // the ensemble reads u and v from its grid table
+    gmem<T>* vg = to_global(v);
     const int W = nx + 2, H = ny + 2;
     const int lane = (int)(threadIdx.x & 63), wv = (int)(threadIdx.x >> 6);
     const int gx0 = tbx * TX + 1 - TS + WPL * lane;  // padded column of R[.][0]
@@ -182,7 +184,7 @@ __device__ inline void rt_tile(const T* __restrict__ u, T* __restrict__ v, int n
 #pragma unroll
         for (int w = 0; w < WPL; w++) {
             const int gx = gx0 + w;
-            R[k][w] = (gx >= 0 && gx < W && gy >= 0 && gy < H) ? u[(size_t)gx + (size_t)W * (size_t)gy] : (T)0;
+            R[k][w] = (gx >= 0 && gx < W && gy >= 0 && gy < H) ? ug[(size_t)gx + (size_t)W * (size_t)gy] : (T)0;
         }
     }
     // one sweep; its edge rows use LDS slot b (sweeps alternate slots)
@@ -241,7 +243,7 @@ __device__ inline void rt_tile(const T* __restrict__ u, T* __restrict__ v, int n
 #pragma unroll
         for (int w = 0; w < WPL; w++) {
             const int lx = WPL * lane + w;
-            if (lx >= TS && lx < TS + TX && col_in[w]) v[(size_t)(gx0 + w) + (size_t)W * (size_t)gy] = R[k][w];
+            if (lx >= TS && lx < TS + TX && col_in[w]) vg[(size_t)(gx0 + w) + (size_t)W * (size_t)gy] = R[k][w];
         }
     }
 }
