@@ -76,12 +76,22 @@ namespace APPFIS
             return *this;
         }
 
-        EXEC_SPACE void Set(int i, T val) { _data[i] = val; }
-        EXEC_SPACE void Set(int i, int j, T val) { _data[Get1DIndex(i, j)] = val; }
-        EXEC_SPACE void Set(int i, int j, int k, T val) { _data[Get1DIndex(i, j, k)] = val; }
-        EXEC_SPACE T Get(int i) { return _data[i]; }
-        EXEC_SPACE T Get(int i, int j) { return _data[Get1DIndex(i, j)]; }
-        EXEC_SPACE T Get(int i, int j, int k) { return _data[Get1DIndex(i, j, k)]; }
+        // Host and device forms of the element access.  On the device the elements are
+        // a device allocation (global memory) and the extents are this object's inline
+        // copy; saying so lets the compiler emit global loads and stores, where a pointer
+        // read from memory would otherwise give flat ones.
+        __host__ void Set(int i, T val) { _data[i] = val; }
+        __host__ void Set(int i, int j, T val) { _data[Get1DIndex(i, j)] = val; }
+        __host__ void Set(int i, int j, int k, T val) { _data[Get1DIndex(i, j, k)] = val; }
+        __host__ T Get(int i) { return _data[i]; }
+        __host__ T Get(int i, int j) { return _data[Get1DIndex(i, j)]; }
+        __host__ T Get(int i, int j, int k) { return _data[Get1DIndex(i, j, k)]; }
+        __device__ void Set(int i, T val) { Elems()[i] = val; }
+        __device__ void Set(int i, int j, T val) { Elems()[DevIndex(i, j)] = val; }
+        __device__ void Set(int i, int j, int k, T val) { Elems()[DevIndex(i, j, k)] = val; }
+        __device__ T Get(int i) { return Elems()[i]; }
+        __device__ T Get(int i, int j) { return Elems()[DevIndex(i, j)]; }
+        __device__ T Get(int i, int j, int k) { return Elems()[DevIndex(i, j, k)]; }
 
         int GetDimCount() const { return _nDim; }
         int GetSize() const { return _size; }
@@ -251,6 +261,13 @@ namespace APPFIS
 
         EXEC_SPACE int Get1DIndex(int i, int j) { return i + _dims[X] * j; }
         EXEC_SPACE int Get1DIndex(int i, int j, int k) { return i + _dims[X] * j + _dims[X] * _dims[Y] * k; }
+        using GlobalT = T __attribute__((address_space(1)));
+        __device__ GlobalT* Elems() const { return (GlobalT*)_data; }
+        __device__ int DevIndex(int i, int j) const { return i + _dimsStore[X] * j; }
+        __device__ int DevIndex(int i, int j, int k) const
+        {
+            return i + _dimsStore[X] * j + _dimsStore[X] * _dimsStore[Y] * k;
+        }
     };
 }  // namespace APPFIS
 
