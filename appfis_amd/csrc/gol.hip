@@ -75,7 +75,7 @@ __device__ __forceinline__ uint8_t rule(int n, uint8_t alive) { return (uint8_t)
 
 // New value of cell (x, y) of the padded buffer, ghost cells mapped to the wrapped
 // interior cell; neighbours read from the current buffer (valid ring).
-__device__ uint8_t cell_new(const uint8_t* __restrict__ u, int W, int H, int x, int y)
+__device__ __noinline__ uint8_t cell_new(const uint8_t* __restrict__ u, int W, int H, int x, int y)
 {
     const int nx = W - 2, ny = H - 2;
     int xw = x == 0 ? nx : (x == W - 1 ? 1 : x);
