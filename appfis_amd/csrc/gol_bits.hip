@@ -474,7 +474,7 @@ static const TbVariant kTb[] = {
     {"tb1 k11 g6 w16", k_step_tb1<11, 6, 16>, 11, 6, 16, 0}, {"tb1 k12 g8 w16", k_step_tb1<12, 8, 16>, 12, 8, 16, 0},
     // full-row waves (nx = 2048 * WPL), also 32 output rows per block at 8192 rows
     {"tw4 k3 g8 w16", k_step_tw<4, 3, 8, 16>, 3, 8, 16, 4}, {"tw4 k4 g8 w12", k_step_tw<4, 4, 8, 12>, 4, 8, 12, 4},
-    {"tw4 k6 g8 w8", k_step_tw<4, 6, 8, 8>, 6, 8, 8, 4},    {"tw2 k3 g8 w16", k_step_tw<2, 3, 8, 16>, 3, 8, 16, 2},
+    {"tw2 k3 g8 w16", k_step_tw<2, 3, 8, 16>, 3, 8, 16, 2},
     {"tw1 k3 g8 w16", k_step_tw<1, 3, 8, 16>, 3, 8, 16, 1},
 };
 
@@ -570,7 +570,7 @@ int gol_bits_run(appfis_ctx* ctx, uint8_t* b0, uint8_t* b1, int nx, int ny, int 
                              int tmp = 0;
                              return run_tb(ctx, golbits::kTb[cand[i]], P, &tmp, NW, ny, 48);
                          },
-                         2, &c);
+                         4, &c);
         if (st) return st;
         pick = cand[c];
     }
