@@ -20,8 +20,8 @@
 //   FP64 rebase sweep forms P = B + q, computes ss(P) exactly in FP64 (gosa from it),
 //   and restarts with B = P, r = (float)ss(P), q = (float)(omega*ss(P)).  In between,
 //   FP32 sweeps advance q with ss = r + L q using FP32 copies of the coefficients, so
-//   the cancellation ss << p never happens in FP32.  The FP32 coefficient copies are
-//   made once per call, inside the call.
+//   the cancellation ss << p never happens in FP32.  The FP32 coefficient copy is
+//   made once per call, inside the call, blocked by 64 points (Coef::inter).
 #include "common.hpp"
 
 #include <vector>
@@ -47,6 +47,10 @@ struct Coef {
     const void* c2;
     const void* bnd;
     const void* wrk1;  // PLAIN: wrk1;  inner sweep: r (FP32 residual of the base)
+    // inner sweep, MODE 2: the 11 static coefficients in blocks of 64 points: coefficient
+    // a (a0 a1 a2 a3 b0 b1 b2 c0 c1 c2 bnd) of point p at [(p / 64) * 704 + a * 64 + p % 64],
+    // so a wave's 11 coefficient loads fall in one 2.75 KB region instead of 11 arrays
+    const void* inter = nullptr;
 };
 
 // Block -> (x, y, z) tile of a 1-D launch: tiles x fastest, then y, then z chunks, and
@@ -160,9 +164,18 @@ __global__ __launch_bounds__(TX) void k_himeno(const T* __restrict__ p, T* __res
     };
     auto coef_load = [&](int k, T(&c)[RY][NC - 1]) {
 #pragma unroll
-        for (int r = 0; r < RY; r++)
+        for (int r = 0; r < RY; r++) {
+            if constexpr (MODE == 2) {
+                const unsigned pt = (unsigned)(own[r] + sz * k);
+                const float* ci = (const float*)cf.inter + (size_t)(pt >> 6) * 704 + (pt & 63);
 #pragma unroll
-            for (int a = 0; a < NC - 1; a++) c[r][a] = rin[r] ? arr[a][own[r] + sz * k] : (T)0;
+                for (int a = 0; a < NC - 2; a++) c[r][a] = rin[r] ? ci[a * 64] : (T)0;
+                c[r][NC - 2] = rin[r] ? arr[NC - 2][own[r] + sz * k] : (T)0;
+            } else {
+#pragma unroll
+                for (int a = 0; a < NC - 1; a++) c[r][a] = rin[r] ? arr[a][own[r] + sz * k] : (T)0;
+            }
+        }
     };
     {
         T v[PPT];
@@ -290,11 +303,20 @@ __global__ __launch_bounds__(TX) void k_himeno_rebase(const double* __restrict__
     block_sum_last(g, partials, ticket, gosa_out, red);
 }
 
-__global__ void k_f64_to_f32(const double* __restrict__ in, float* __restrict__ out, size_t n)
+// The FP32 blocked coefficients of the inner sweep (see Coef::inter) from the FP64
+// arrays a0..a3, b0..b2, c0..c2, bnd of cf; out holds ceil(n / 64) * 704 floats
+__global__ void k_block_coefs(Coef cf, float* __restrict__ out, size_t n)
 {
+    const double* a[11] = {(const double*)cf.a0, (const double*)cf.a1, (const double*)cf.a2, (const double*)cf.a3,
+                           (const double*)cf.b0, (const double*)cf.b1, (const double*)cf.b2, (const double*)cf.c0,
+                           (const double*)cf.c1, (const double*)cf.c2, (const double*)cf.bnd};
     size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    size_t stride = (size_t)gridDim.x * blockDim.x;
-    for (; t < n; t += stride) out[t] = (float)in[t];
+    const size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (; t < n; t += stride) {
+        float* o = out + (t >> 6) * 704 + (t & 63);
+#pragma unroll
+        for (int c = 0; c < 11; c++) o[c * 64] = (float)a[c][t];
+    }
 }
 
 // P = B + q on the interior (final field of the defect-correction form)
@@ -375,33 +397,33 @@ extern "C" int appfis_himeno(appfis_ctx* ctx, int dtype, void* const* arrays, in
         // config, against 7.5e-7 at 8 and 2.6e-6 at 24; 1.08x the speed of 8)
         const int K = outer > 0 ? outer : 16;
         const size_t n = (size_t)nx * ny * nz;
-        // workspace (kept by the context across calls): 11 FP32 coefficient copies, r,
-        // q x2 (FP32) and B x2 (FP64), each 256-byte aligned
+        // workspace (kept by the context across calls): the blocked FP32
+        // coefficients (44 bytes per point), r, q x2 (FP32) and B x2 (FP64), each
+        // 256-byte aligned
         const size_t n4 = ((n * 4 + 255) / 256) * 256, n8 = ((n * 8 + 255) / 256) * 256;
-        st = ensure_work(ctx, 14 * n4 + 2 * n8);
+        const size_t n48 = (((n + 63) / 64) * 704 * 4 + 255) / 256 * 256;  // blocked coefficients
+        st = ensure_work(ctx, n48 + 3 * n4 + 2 * n8);
         if (st) return st;
         char* w = (char*)ctx->work;
-        float* c32[11];
-        for (int a = 0; a < 11; a++) c32[a] = (float*)(w + (size_t)a * n4);
-        float* r32 = (float*)(w + 11 * n4);
-        float* q[2] = {(float*)(w + 12 * n4), (float*)(w + 13 * n4)};
-        double* B[2] = {(double*)(w + 14 * n4), (double*)(w + 14 * n4 + n8)};
+        float* ci = (float*)w;
+        float* r32 = (float*)(w + n48);
+        float* q[2] = {(float*)(w + n48 + n4), (float*)(w + n48 + 2 * n4)};
+        double* B[2] = {(double*)(w + n48 + 3 * n4), (double*)(w + n48 + 3 * n4 + n8)};
         const unsigned cb = (unsigned)std::min<size_t>(ceil_div(n, 256), 65536);
-        for (int a = 0; a < 11; a++) {
-            hipLaunchKernelGGL(k_f64_to_f32, dim3(cb), dim3(256), 0, ctx->stream, (const double*)arrays[2 + a], c32[a],
-                               n);
-            APPFIS_LAUNCH_CHECK();
-        }
+        hipLaunchKernelGGL(k_block_coefs, dim3(cb), dim3(256), 0, ctx->stream, cf, ci, n);
+        APPFIS_LAUNCH_CHECK();
         // start: B = p (whole array, shell included), q = 0
         APPFIS_HIP_CHECK(hipMemcpyAsync(B[0], arrays[0], n * 8, hipMemcpyDeviceToDevice, ctx->stream));
         APPFIS_HIP_CHECK(hipMemcpyAsync(B[1], arrays[0], n * 8, hipMemcpyDeviceToDevice, ctx->stream));
         APPFIS_HIP_CHECK(hipMemsetAsync(q[0], 0, n * 4, ctx->stream));
         APPFIS_HIP_CHECK(hipMemsetAsync(q[1], 0, n * 4, ctx->stream));
-        Coef c32s{c32[0], c32[1], c32[2], c32[3], c32[4], c32[5], c32[6], c32[7], c32[8], c32[9], c32[10], r32};
+        Coef c32s{};
+        c32s.wrk1 = r32;
+        c32s.inter = ci;
         // z-chunk of the inner FP32 sweep, timed between the correction buffers (q[1] is
         // rewritten by the first rebase; q[0] stays 0)
         const int zc = tune_zc(1, [&](int c) {
-            hipLaunchKernelGGL((k_himeno<float, 1>), grid_of(nx, ny, nz, kZc[c]), dim3(TX), 0, ctx->stream, q[0],
+            hipLaunchKernelGGL((k_himeno<float, 2>), grid_of(nx, ny, nz, kZc[c]), dim3(TX), 0, ctx->stream, q[0],
                                q[1], c32s, nx, ny, nz, kZc[c], (float)omega, partials, ticket, gosa_dev);
             APPFIS_LAUNCH_CHECK();
             return APPFIS_OK;
@@ -415,7 +437,7 @@ extern "C" int appfis_himeno(appfis_ctx* ctx, int dtype, void* const* arrays, in
                                    q[qi ^ 1], cf, nx, ny, nz, zc, omega, partials, ticket, gosa_dev + it);
                 bi ^= 1;
             } else {
-                hipLaunchKernelGGL((k_himeno<float, 1>), grid, dim3(TX), 0, ctx->stream, q[qi], q[qi ^ 1], c32s, nx,
+                hipLaunchKernelGGL((k_himeno<float, 2>), grid, dim3(TX), 0, ctx->stream, q[qi], q[qi ^ 1], c32s, nx,
                                    ny, nz, zc, (float)omega, partials, ticket, gosa_dev + it);
             }
             APPFIS_LAUNCH_CHECK();

@@ -418,12 +418,13 @@ def bench_himeno(ctx, hip, nx=257, ny=257, nz=513, iters=200):
             r["roofline"] = roofline(float(bpl) * pts, ms / iters, f"k_himeno<{'float' if dt == hip.F32 else 'double'}, 0>")
         else:
             # defect correction, per sweep on average over one outer cycle of K sweeps
-            # (the library's default K = 16): K-1 FP32 inner sweeps (11 coefficient copies,
-            # r, q read, q written: 56 B/LUP) and one FP64 rebase (11 coefficients, wrk1, B
-            # read, q read 4 B, B / r / q written 8 + 4 + 4 B: 124 B/LUP)
+            # (the library's default K = 16): K-1 FP32 inner sweeps (11 coefficients, read
+            # from a copy blocked by 64 points, r, q read, q written: 56 B/LUP) and one FP64
+            # rebase (11 coefficients, wrk1, B read, q read 4 B, B / r / q written
+            # 8 + 4 + 4 B: 124 B/LUP)
             K = HIMENO_DC_OUTER
             r["roofline"] = roofline(((K - 1) * 56.0 + 124.0) / K * pts, ms / iters,
-                                     {"k_himeno<float, 1>": (K - 1) / K, "k_himeno_rebase": 1 / K})
+                                     {"k_himeno<float, 2>": (K - 1) / K, "k_himeno_rebase": 1 / K})
         out[name] = r
         for a in arrs:
             a.free()
