@@ -46,12 +46,15 @@ struct Coef {
     const void* c1;
     const void* c2;
     const void* bnd;
-    const void* wrk1;  // PLAIN: wrk1;  inner sweep: r (FP32 residual of the base)
-    // inner sweep, MODE 2: the 11 static coefficients in blocks of 64 points: coefficient
-    // a (a0 a1 a2 a3 b0 b1 b2 c0 c1 c2 bnd) of point p at [(p / 64) * 704 + a * 64 + p % 64],
-    // so a wave's 11 coefficient loads fall in one 2.75 KB region instead of 11 arrays
+    const void* wrk1;  // PLAIN: wrk1 (the inner sweep reads r from `inter`)
+    // inner sweep, MODE 2: the 11 static coefficients and r in blocks of 64 points: value
+    // a (a0 a1 a2 a3 b0 b1 b2 c0 c1 c2 bnd r) of point p at blk_idx(p, a), so a wave's 12
+    // coefficient loads fall in one 3 KB region instead of 12 arrays
     const void* inter = nullptr;
 };
+
+constexpr int BLK = 12 * 64;  // floats per block of 64 points
+__host__ __device__ inline size_t blk_idx(size_t p, int a) { return (p >> 6) * BLK + (size_t)a * 64 + (p & 63); }
 
 // Block -> (x, y, z) tile of a 1-D launch: tiles x fastest, then y, then z chunks, and
 // every XCD takes a contiguous run, so y-neighbour tiles (which share p rows) share an L2.
@@ -166,11 +169,9 @@ __global__ __launch_bounds__(TX) void k_himeno(const T* __restrict__ p, T* __res
 #pragma unroll
         for (int r = 0; r < RY; r++) {
             if constexpr (MODE == 2) {
-                const unsigned pt = (unsigned)(own[r] + sz * k);
-                const float* ci = (const float*)cf.inter + (size_t)(pt >> 6) * 704 + (pt & 63);
+                const float* ci = (const float*)cf.inter + blk_idx((size_t)(own[r] + sz * k), 0);
 #pragma unroll
-                for (int a = 0; a < NC - 2; a++) c[r][a] = rin[r] ? ci[a * 64] : (T)0;
-                c[r][NC - 2] = rin[r] ? arr[NC - 2][own[r] + sz * k] : (T)0;
+                for (int a = 0; a < NC - 1; a++) c[r][a] = rin[r] ? ci[a * 64] : (T)0;
             } else {
 #pragma unroll
                 for (int a = 0; a < NC - 1; a++) c[r][a] = rin[r] ? arr[a][own[r] + sz * k] : (T)0;
@@ -293,7 +294,7 @@ __global__ __launch_bounds__(TX) void k_himeno_rebase(const double* __restrict__
                             + c2[o] * ring[sm][ly][lx] + w1[o];
                 const double ss = (s0 * a3[o] - pc) * bnd[o];
                 Bn[o] = pc;
-                r[o] = (float)ss;
+                r[blk_idx(o, 11)] = (float)ss;  // the r slot of the blocked coefficients
                 qn[o] = (float)(omega * ss);
                 g += ss * ss;
             }
@@ -304,7 +305,8 @@ __global__ __launch_bounds__(TX) void k_himeno_rebase(const double* __restrict__
 }
 
 // The FP32 blocked coefficients of the inner sweep (see Coef::inter) from the FP64
-// arrays a0..a3, b0..b2, c0..c2, bnd of cf; out holds ceil(n / 64) * 704 floats
+// arrays a0..a3, b0..b2, c0..c2, bnd of cf (slot 11, r, is written by the rebase sweep);
+// out holds ceil(n / 64) * BLK floats
 __global__ void k_block_coefs(Coef cf, float* __restrict__ out, size_t n)
 {
     const double* a[11] = {(const double*)cf.a0, (const double*)cf.a1, (const double*)cf.a2, (const double*)cf.a3,
@@ -313,7 +315,7 @@ __global__ void k_block_coefs(Coef cf, float* __restrict__ out, size_t n)
     size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     const size_t stride = (size_t)gridDim.x * blockDim.x;
     for (; t < n; t += stride) {
-        float* o = out + (t >> 6) * 704 + (t & 63);
+        float* o = out + blk_idx(t, 0);
 #pragma unroll
         for (int c = 0; c < 11; c++) o[c * 64] = (float)a[c][t];
     }
@@ -398,17 +400,16 @@ extern "C" int appfis_himeno(appfis_ctx* ctx, int dtype, void* const* arrays, in
         const int K = outer > 0 ? outer : 16;
         const size_t n = (size_t)nx * ny * nz;
         // workspace (kept by the context across calls): the blocked FP32
-        // coefficients (44 bytes per point), r, q x2 (FP32) and B x2 (FP64), each
+        // coefficients and r (48 bytes per point), q x2 (FP32) and B x2 (FP64), each
         // 256-byte aligned
         const size_t n4 = ((n * 4 + 255) / 256) * 256, n8 = ((n * 8 + 255) / 256) * 256;
-        const size_t n48 = (((n + 63) / 64) * 704 * 4 + 255) / 256 * 256;  // blocked coefficients
-        st = ensure_work(ctx, n48 + 3 * n4 + 2 * n8);
+        const size_t n48 = (((n + 63) / 64) * BLK * 4 + 255) / 256 * 256;  // blocked coefficients and r
+        st = ensure_work(ctx, n48 + 2 * n4 + 2 * n8);
         if (st) return st;
         char* w = (char*)ctx->work;
         float* ci = (float*)w;
-        float* r32 = (float*)(w + n48);
-        float* q[2] = {(float*)(w + n48 + n4), (float*)(w + n48 + 2 * n4)};
-        double* B[2] = {(double*)(w + n48 + 3 * n4), (double*)(w + n48 + 3 * n4 + n8)};
+        float* q[2] = {(float*)(w + n48), (float*)(w + n48 + n4)};
+        double* B[2] = {(double*)(w + n48 + 2 * n4), (double*)(w + n48 + 2 * n4 + n8)};
         const unsigned cb = (unsigned)std::min<size_t>(ceil_div(n, 256), 65536);
         hipLaunchKernelGGL(k_block_coefs, dim3(cb), dim3(256), 0, ctx->stream, cf, ci, n);
         APPFIS_LAUNCH_CHECK();
@@ -418,7 +419,6 @@ extern "C" int appfis_himeno(appfis_ctx* ctx, int dtype, void* const* arrays, in
         APPFIS_HIP_CHECK(hipMemsetAsync(q[0], 0, n * 4, ctx->stream));
         APPFIS_HIP_CHECK(hipMemsetAsync(q[1], 0, n * 4, ctx->stream));
         Coef c32s{};
-        c32s.wrk1 = r32;
         c32s.inter = ci;
         // z-chunk of the inner FP32 sweep, timed between the correction buffers (q[1] is
         // rewritten by the first rebase; q[0] stays 0)
@@ -433,7 +433,7 @@ extern "C" int appfis_himeno(appfis_ctx* ctx, int dtype, void* const* arrays, in
         int bi = 0, qi = 0;
         for (int it = 0; it < iters; it++) {
             if (it % K == 0) {
-                hipLaunchKernelGGL(k_himeno_rebase, grid, dim3(TX), 0, ctx->stream, B[bi], q[qi], B[bi ^ 1], r32,
+                hipLaunchKernelGGL(k_himeno_rebase, grid, dim3(TX), 0, ctx->stream, B[bi], q[qi], B[bi ^ 1], ci,
                                    q[qi ^ 1], cf, nx, ny, nz, zc, omega, partials, ticket, gosa_dev + it);
                 bi ^= 1;
             } else {
