@@ -2,7 +2,7 @@
  * Env.hpp -- macros, enums and the execution flag of the APPFIS API
  * (same names and meaning as the reference Appfis/Env.hpp:16-32).
  *
- * This build targets one backend: MI355X through HIP.  Compile with -DAPPFIS_HIP
+ * The API has one backend here: MI355X through HIP.  Compile with -DAPPFIS_HIP
  * (hipcc, gfx950).  The reference apps select their APPFIS variant with
  * `#if defined(APPFIS_CUDA) || defined(APPFIS_CPU)`, so they are built with
  * -DAPPFIS_HIP -DAPPFIS_CPU: APPFIS_CPU only picks the *_appfis.hpp sources there,

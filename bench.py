@@ -668,8 +668,8 @@ def main():
     global CPU_SAMPLES
     CPU_SAMPLES = not args.no_cpu
     if args.gpus != 1:
-        print(f"bench.py measures one GPU: this build has no multi-GPU path (asked for --gpus {args.gpus}); "
-              "DESIGN.md describes how the stencils would shard.", file=sys.stderr)
+        print(f"bench.py measures one GPU: there is no multi-GPU path (asked for --gpus {args.gpus}); "
+              "the stencils would shard by slabs along the slowest axis, one rank per GPU.", file=sys.stderr)
         sys.exit(2)
 
     from appfis_amd import hip

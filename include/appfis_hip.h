@@ -35,7 +35,7 @@ extern "C" {
 #define APPFIS_OK 0
 #define APPFIS_ERR_INVALID 1     /* bad argument (shape, dtype, face, null pointer) */
 #define APPFIS_ERR_HIP 2         /* a HIP runtime call failed */
-#define APPFIS_ERR_UNSUPPORTED 3 /* valid request that this build does not implement */
+#define APPFIS_ERR_UNSUPPORTED 3 /* valid request that the library does not implement */
 #define APPFIS_ERR_NOMEM 4       /* device allocation failed */
 
 /* element types */
