@@ -124,6 +124,33 @@ namespace APPFIS
             detail_k::SwapParity(p, iters);
             return gosa;
         }
+
+        // Triton shallow-water flood step chain (Roe flux, implicit Manning friction, wet /
+        // dry fix; triton_appfis.hpp:316-386 of the reference) on SINGLE grids h, qx, qy
+        // (updated in place), dem z (wall height in the ghost ring) and n2 (Manning n^2),
+        // ghost 1, with optional hydrograph point sources.  Runs while the simulated time
+        // is below params.end_time (> 0), at most max_steps steps; returns the steps run.
+        template <typename T>
+        int Flood(Grid<T>& h, Grid<T>& qx, Grid<T>& qy, Grid<T>& z, Grid<T>& n2, const appfis_flood_params& params,
+                  const appfis_flood_sources* sources, int max_steps, int variant = APPFIS_CHOSEN)
+        {
+            Grid<T>* gs[5] = {&h, &qx, &qy, &z, &n2};
+            for (Grid<T>* g : gs)
+                if (!EXECUTION_FLAG || !g->HasDeviceMirror()) {
+                    std::cerr << "APPFIS::kernels::Flood needs its grids inside InitializeExecution" << std::endl;
+                    exit(EXIT_FAILURE);
+                }
+            Array<T>* a = h.GetSubGridCPU();
+            const int nx = a->GetDim(X) - 2, ny = a->GetDim(Y) - 2;
+            int done = 0;
+            APPFIS::detail::Check(appfis_flood_run_src(GetContext(), APPFIS::detail::DtypeOf<T>(),
+                                                       h.GetSubGridsGPUData(), qx.GetSubGridsGPUData(),
+                                                       qy.GetSubGridsGPUData(), z.GetSubGridsGPUData(),
+                                                       n2.GetSubGridsGPUData(), nx, ny, &params, sources, max_steps,
+                                                       variant, nullptr, nullptr, &done),
+                                  "appfis_flood_run_src");
+            return done;
+        }
     }  // namespace kernels
 }  // namespace APPFIS
 

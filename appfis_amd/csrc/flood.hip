@@ -32,6 +32,7 @@
 // compensated FP64 sum).  No FMA contraction: the operation order is the functors'.
 #include "common.hpp"
 
+#include <algorithm>
 #include <cfloat>
 #include <vector>
 
@@ -55,6 +56,7 @@ struct ScenState {
     double simtime;  // simulated time (s)
     int print_id;
     int step;        // steps completed in this call
+    int src_low;     // hydrograph row where the next interval search starts
 };
 
 template <typename R>
@@ -73,12 +75,28 @@ struct Scen {
     double* rec_time;  // per-step records (nsteps each) or null
     double* rec_vol;
     double* rec_dt;
+    // point sources: nsrc padded cell indices, a hydrograph of src_rows times (s) and
+    // src_rows x nsrc flows (m^3/s, row-major)
+    int nsrc, src_rows;
+    const int* src_cell;
+    const double* src_time;
+    const double* src_flow;
 };
 
 struct Params {
     double dx, cn, hextra, dt_fallback, print_interval, rain;
     int nsteps_rec;  // length of the record arrays
+    int fixed_dt;    // time_increment_fixed: every step takes dt_fallback
+    double end_time; // > 0: a step starting at simtime >= end_time does nothing
 };
+
+// The reference loops while simtime < sim_duration (triton_appfis.hpp:316): with an end
+// time, every kernel of a step that would start at or past it returns at once (the test
+// reads a value the previous launch wrote, the same for every thread of the launch).
+__device__ __forceinline__ bool finished(const ScenState* st, const Params& P)
+{
+    return P.end_time > 0.0 && st->simtime >= P.end_time;
+}
 
 // Division and square root: correctly rounded for R = double (the reference's
 // arithmetic: the FP64 form reproduces the oracle); for the FP32 bulk form the hardware
@@ -344,6 +362,7 @@ __global__ __launch_bounds__(BT) void k_flux_march(const Scen<R>* __restrict__ s
     if (strip >= nstrips) return;
     const int s = find_scen(sc, nscen, strip);
     const Scen<R>& S = sc[s];
+    if (finished(S.st, P)) return;
     const int t = strip - S.strip0;
     const int ty = t / S.strips_x, tx = t - ty * S.strips_x;
     const int nx = S.nx, ny = S.ny;
@@ -464,6 +483,50 @@ __global__ __launch_bounds__(BT) void k_flux_march(const Scen<R>* __restrict__ s
     }
 }
 
+// Point sources of the step (triton_appfis.hpp:358-377 on the host, then the functor of
+// 1115-1141), one block per scenario, after K1 wrote the updated depth and before K2:
+// thread 0 advances the hydrograph interval of the step's start time from the row the
+// previous step stopped at; thread k then interpolates source k's flow linearly in time
+// and adds flow*dt/dx^2 to its cell (the cells are distinct).
+template <typename R>
+__global__ __launch_bounds__(64) void k_sources(const Scen<R>* __restrict__ sc, int par_out, Params P)
+{
+    const Scen<R>& S = sc[blockIdx.x];
+    if (S.nsrc == 0 || finished(S.st, P)) return;
+    __shared__ int lohi[2];
+    ScenState* st = S.st;
+    const double simtime = st->simtime, gdt = st->dt;
+    if (threadIdx.x == 0) {
+        int lo = st->src_low, hi = S.src_rows - 1;
+        bool found = false;
+        for (int i = lo; i < hi; i++)
+            if (S.src_time[i + 1] > simtime) {
+                lo = i;
+                hi = i + 1;
+                found = true;
+                break;
+            }
+        if (!found) lo = hi;
+        st->src_low = lo;
+        lohi[0] = lo;
+        lohi[1] = hi;
+    }
+    __syncthreads();
+    const int lo = lohi[0], hi = lohi[1];
+    R* h = S.h[par_out];
+    for (int k = threadIdx.x; k < S.nsrc; k += blockDim.x) {
+        const double fl = S.src_flow[(size_t)lo * S.nsrc + k], fh = S.src_flow[(size_t)hi * S.nsrc + k];
+        const double tl = S.src_time[lo], th = S.src_time[hi];
+        double flow;
+        if (lo == hi) flow = fl;
+        else flow = fl + (((fh - fl) * (simtime - tl)) / (th - tl));
+        const int c = S.src_cell[k];
+        double hij = (double)h[c];
+        hij += (flow * gdt) / (P.dx * P.dx);
+        h[c] = (R)hij;
+    }
+}
+
 __device__ inline void two_sum(double& s, double& cmp, double x)
 {
     double t = __dadd_rn(s, x);
@@ -509,6 +572,7 @@ __global__ __launch_bounds__(BT) void k_fix_march(const Scen<R>* __restrict__ sc
     if (strip >= nstrips) return;
     const int s = find_scen(sc, nscen, strip);
     const Scen<R>& S = sc[s];
+    if (MODE == 1 && finished(S.st, P)) return;
     const int t = strip - S.strip0;
     const int ty = t / S.strips_x, tx = t - ty * S.strips_x;
     const int nx = S.nx, ny = S.ny;
@@ -590,6 +654,7 @@ __global__ __launch_bounds__(BT) void k_finalize(const Scen<R>* __restrict__ sc,
     __shared__ double smin[BT / 64];
     __shared__ double ssum[BT / 64][2];
     const Scen<R>& S = sc[blockIdx.x];
+    if (MODE == 1 && finished(S.st, P)) return;  // before thread 0 updates simtime below
     double m = (double)RealLim<R>::max();
     double as = 0.0, ac = 0.0;
     for (int p = threadIdx.x; p < S.nstrips; p += BT) {
@@ -633,10 +698,14 @@ __global__ __launch_bounds__(BT) void k_finalize(const Scen<R>* __restrict__ sc,
         st->simtime = simtime;
         st->print_id = pid;
     }
-    // next dt (triton_appfis.hpp:322-335): the minimum is an R value held exactly in m
-    double gdt = (double)(R)m;
-    if (gdt >= (double)RealLim<R>::max() - 1.0) gdt = P.dt_fallback;
-    if (PI > 0.0 && simtime + gdt > PI * (pid + 1)) gdt = PI * (pid + 1) - simtime;
+    // next dt (triton_appfis.hpp:322-335): the minimum is an R value held exactly in m;
+    // with time_increment_fixed the reference skips that block and keeps time_step
+    double gdt = P.dt_fallback;
+    if (!P.fixed_dt) {
+        gdt = (double)(R)m;
+        if (gdt >= (double)RealLim<R>::max() - 1.0) gdt = P.dt_fallback;
+        if (PI > 0.0 && simtime + gdt > PI * (pid + 1)) gdt = PI * (pid + 1) - simtime;
+    }
     st->dt = (double)(R)gdt;
 }
 
@@ -648,13 +717,28 @@ __global__ void k_init_state(ScenState* st, int n, double start_time)
     st[s].simtime = start_time;
     st[s].print_id = 0;
     st[s].step = 0;
+    st[s].src_low = 0;
 }
 
 template <typename R>
 int run_flood(appfis_ctx* ctx, int nscen, const int* nx, const int* ny, void* const* h, void* const* qx,
               void* const* qy, const void* const* z, const void* const* n2, const appfis_flood_params* prm, int steps,
-              bool graph, double* time_host, double* volume_host, double* dt_host)
+              bool graph, double* time_host, double* volume_host, double* dt_host,
+              const appfis_flood_sources* src = nullptr, int* steps_done = nullptr)
 {
+    const int nsrc = src ? src->count : 0;
+    if (nsrc > 0) {
+        APPFIS_REQUIRE(nscen == 1, "point sources are supported for one scenario per call");
+        APPFIS_REQUIRE(src->cell && src->time && src->flow && src->rows >= 1, "sources: NULL table or no rows");
+        std::vector<int> seen(src->cell, src->cell + nsrc);
+        std::sort(seen.begin(), seen.end());
+        for (int k = 0; k < nsrc; k++) {
+            const int c = src->cell[k], W = nx[0] + 2, i = c % W, j = c / W;
+            APPFIS_REQUIRE(c >= 0 && i >= 1 && i <= nx[0] && j >= 1 && j <= ny[0],
+                           "source %d: cell %d is not an interior cell", k, c);
+            APPFIS_REQUIRE(k == 0 || seen[k] != seen[k - 1], "sources: cell %d appears twice", seen[k]);
+        }
+    }
     for (int s = 0; s < nscen; s++) {
         APPFIS_REQUIRE(nx[s] >= 1 && ny[s] >= 1, "scenario %d: empty interior", s);
         APPFIS_REQUIRE((double)(nx[s] + 2) * (ny[s] + 2) < 2147483647.0, "scenario %d exceeds the int index range", s);
@@ -676,13 +760,18 @@ int run_flood(appfis_ctx* ctx, int nscen, const int* nx, const int* ny, void* co
         field_bytes += ((size_t)(nx[s] + 2) * (ny[s] + 2) * sizeof(R) + 255) / 256 * 256;
     }
     APPFIS_REQUIRE(total_strips < (1LL << 31), "too many strips");
-    const int rec = steps > 0 ? steps : 1;
+    // per-step records only when the caller asked for them
+    const bool want_rec = time_host || volume_host || dt_host;
+    const int rec = want_rec && steps > 0 ? steps : 1;
     // workspace: second buffers (h, qx, qy) | states | partials | records | descriptor table
     const size_t st_bytes = ((size_t)nscen * sizeof(ScenState) + 255) / 256 * 256;
     const size_t part_bytes = ((size_t)total_strips * 24 + 255) / 256 * 256;
     const size_t rec_bytes = ((size_t)nscen * rec * 8 + 255) / 256 * 256;
     const size_t tab_bytes = ((size_t)nscen * sizeof(Scen<R>) + 255) / 256 * 256;
-    int stt = ensure_work(ctx, 3 * field_bytes + st_bytes + part_bytes + 3 * rec_bytes + tab_bytes);
+    const size_t src_rows = nsrc > 0 ? (size_t)src->rows : 0;
+    const size_t src_bytes = ((size_t)nsrc * 4 + 255) / 256 * 256 + (src_rows * 8 + 255) / 256 * 256 +
+                             (src_rows * nsrc * 8 + 255) / 256 * 256;
+    int stt = ensure_work(ctx, 3 * field_bytes + st_bytes + part_bytes + 3 * rec_bytes + tab_bytes + src_bytes);
     if (stt) return stt;
     char* w = (char*)ctx->work;
     size_t off = 0;
@@ -710,16 +799,55 @@ int run_flood(appfis_ctx* ctx, int nscen, const int* nx, const int* ny, void* co
     double* rv = (double*)(base + st_bytes + part_bytes + rec_bytes);
     double* rd = (double*)(base + st_bytes + part_bytes + 2 * rec_bytes);
     Scen<R>* tab = (Scen<R>*)(base + st_bytes + part_bytes + 3 * rec_bytes);
+    char* sbase = base + st_bytes + part_bytes + 3 * rec_bytes + tab_bytes;
+    int* d_cell = (int*)sbase;
+    double* d_time = (double*)(sbase + ((size_t)nsrc * 4 + 255) / 256 * 256);
+    double* d_flow = (double*)((char*)d_time + (src_rows * 8 + 255) / 256 * 256);
     for (int s = 0; s < nscen; s++) {
         hs[s].st = st + s;
         hs[s].partials = partials + 3 * (size_t)hs[s].strip0;
         hs[s].rec_time = rt + (size_t)s * rec;
         hs[s].rec_vol = rv + (size_t)s * rec;
         hs[s].rec_dt = rd + (size_t)s * rec;
+        hs[s].nsrc = 0;
+        hs[s].src_rows = 0;
+        hs[s].src_cell = nullptr;
+        hs[s].src_time = nullptr;
+        hs[s].src_flow = nullptr;
+    }
+    if (nsrc > 0) {
+        hs[0].nsrc = nsrc;
+        hs[0].src_rows = src->rows;
+        hs[0].src_cell = d_cell;
+        hs[0].src_time = d_time;
+        hs[0].src_flow = d_flow;
+        APPFIS_HIP_CHECK(hipMemcpyAsync(d_cell, src->cell, (size_t)nsrc * 4, hipMemcpyHostToDevice, ctx->stream));
+        APPFIS_HIP_CHECK(hipMemcpyAsync(d_time, src->time, src_rows * 8, hipMemcpyHostToDevice, ctx->stream));
+        APPFIS_HIP_CHECK(
+            hipMemcpyAsync(d_flow, src->flow, src_rows * nsrc * 8, hipMemcpyHostToDevice, ctx->stream));
     }
     APPFIS_HIP_CHECK(hipMemcpyAsync(tab, hs.data(), (size_t)nscen * sizeof(Scen<R>), hipMemcpyHostToDevice, ctx->stream));
     APPFIS_HIP_CHECK(hipStreamSynchronize(ctx->stream));  // the host table is a local
-    Params P{prm->dx, prm->courant, prm->hextra, prm->dt_fallback, prm->print_interval, prm->rain_rate, rec};
+    Params P{prm->dx,        prm->courant,   prm->hextra, prm->dt_fallback,
+             prm->print_interval, prm->rain_rate, want_rec ? rec : 0, prm->time_increment_fixed ? 1 : 0,
+             prm->end_time};
+    // with an end time, the host looks at the scenarios' simulated time every
+    // CHECK_EVERY steps and stops issuing steps once all of them have reached it (steps
+    // issued past that point do nothing on the device)
+    const bool until = prm->end_time > 0.0;
+    constexpr int CHECK_EVERY = 10 * CHUNK;
+    std::vector<ScenState> hst((size_t)nscen);
+    auto read_states = [&]() -> int {
+        APPFIS_HIP_CHECK(
+            hipMemcpyAsync(hst.data(), st, (size_t)nscen * sizeof(ScenState), hipMemcpyDeviceToHost, ctx->stream));
+        APPFIS_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+        return APPFIS_OK;
+    };
+    auto all_finished = [&]() {
+        for (const ScenState& x : hst)
+            if (!(x.simtime >= prm->end_time)) return false;
+        return true;
+    };
     hipLaunchKernelGGL(k_init_state, dim3(ceil_div(nscen, 64)), dim3(64), 0, ctx->stream, st, nscen, prm->start_time);
     APPFIS_LAUNCH_CHECK();
     const dim3 grid((unsigned)ceil_div(total_strips, MWAVES));  // MWAVES strips per block
@@ -732,6 +860,10 @@ int run_flood(appfis_ctx* ctx, int nscen, const int* nx, const int* ny, void* co
     auto step_pair = [&](int par) -> int {
         hipLaunchKernelGGL((k_flux_march<R>), grid, dim3(BT), 0, ctx->stream, tab, nscen, nstrips, par, P);
         APPFIS_LAUNCH_CHECK();
+        if (nsrc > 0) {
+            hipLaunchKernelGGL((k_sources<R>), dim3(nscen), dim3(64), 0, ctx->stream, tab, par ^ 1, P);
+            APPFIS_LAUNCH_CHECK();
+        }
         hipLaunchKernelGGL((k_fix_march<R, 1>), grid, dim3(BT), 0, ctx->stream, tab, nscen, nstrips, par ^ 1, P);
         APPFIS_LAUNCH_CHECK();
         hipLaunchKernelGGL((k_finalize<R, 1>), dim3(nscen), dim3(BT), 0, ctx->stream, tab, P);
@@ -753,31 +885,46 @@ int run_flood(appfis_ctx* ctx, int nscen, const int* nx, const int* ny, void* co
         if (e != hipSuccess) return hip_fail(e, "flood graph instantiate", __FILE__, __LINE__);
         if (e2 != hipSuccess) return hip_fail(e2, "flood graph destroy", __FILE__, __LINE__);
         const int reps = steps / CHUNK;
-        for (int r = 0; r < reps; r++) {
+        bool stop = false;
+        for (int r = 0; r < reps && !stop; r++) {
             e = hipGraphLaunch(ge, ctx->stream);
-            if (e != hipSuccess) {
-                (void)hipGraphExecDestroy(ge);
-                return hip_fail(e, "flood graph launch", __FILE__, __LINE__);
+            int rst = e == hipSuccess ? APPFIS_OK : hip_fail(e, "flood graph launch", __FILE__, __LINE__);
+            if (rst == APPFIS_OK && until && (r + 1) * CHUNK % CHECK_EVERY == 0) {
+                rst = read_states();
+                stop = rst == APPFIS_OK && all_finished();
             }
+            if (rst != APPFIS_OK) {
+                (void)hipGraphExecDestroy(ge);
+                return rst;
+            }
+            done = (r + 1) * CHUNK;
         }
         APPFIS_HIP_CHECK(hipStreamSynchronize(ctx->stream));
         APPFIS_HIP_CHECK(hipGraphExecDestroy(ge));
-        done = reps * CHUNK;
+        if (stop) done = steps;
     }
     for (int k = done; k < steps; k++) {
         int cst = step_pair(k & 1);
         if (cst) return cst;
-    }
-    // result in buffer steps % 2: bring it home when that is the work copy
-    if (steps & 1) {
-        for (int s = 0; s < nscen; s++) {
-            const size_t fb = (size_t)(nx[s] + 2) * (ny[s] + 2) * sizeof(R);
-            APPFIS_HIP_CHECK(hipMemcpyAsync(h[s], hs[s].h[1], fb, hipMemcpyDeviceToDevice, ctx->stream));
-            APPFIS_HIP_CHECK(hipMemcpyAsync(qx[s], hs[s].qx[1], fb, hipMemcpyDeviceToDevice, ctx->stream));
-            APPFIS_HIP_CHECK(hipMemcpyAsync(qy[s], hs[s].qy[1], fb, hipMemcpyDeviceToDevice, ctx->stream));
+        if (until && (k + 1) % CHECK_EVERY == 0) {
+            cst = read_states();
+            if (cst) return cst;
+            if (all_finished()) break;
         }
     }
-    if (steps > 0) {
+    // each scenario's result is in buffer (steps it ran) % 2: bring it home when that is
+    // the work copy
+    int st_read = read_states();
+    if (st_read) return st_read;
+    for (int s = 0; s < nscen; s++) {
+        if (!(hst[s].step & 1)) continue;
+        const size_t fb = (size_t)(nx[s] + 2) * (ny[s] + 2) * sizeof(R);
+        APPFIS_HIP_CHECK(hipMemcpyAsync(h[s], hs[s].h[1], fb, hipMemcpyDeviceToDevice, ctx->stream));
+        APPFIS_HIP_CHECK(hipMemcpyAsync(qx[s], hs[s].qx[1], fb, hipMemcpyDeviceToDevice, ctx->stream));
+        APPFIS_HIP_CHECK(hipMemcpyAsync(qy[s], hs[s].qy[1], fb, hipMemcpyDeviceToDevice, ctx->stream));
+    }
+    if (steps_done) *steps_done = hst[0].step;
+    if (want_rec && steps > 0) {
         const size_t rb = (size_t)nscen * rec * 8;
         if (time_host) APPFIS_HIP_CHECK(hipMemcpyAsync(time_host, rt, rb, hipMemcpyDeviceToHost, ctx->stream));
         if (volume_host) APPFIS_HIP_CHECK(hipMemcpyAsync(volume_host, rv, rb, hipMemcpyDeviceToHost, ctx->stream));
@@ -831,6 +978,29 @@ int appfis_flood_run(appfis_ctx* ctx, int dtype, void* h, void* qx, void* qy, co
     if (dtype == APPFIS_F32)
         return run_flood<float>(ctx, 1, &nx, &ny, hh, qq, rr, zz, nn, params, steps, graph, time_host, volume_host,
                                 nullptr);
+    return invalid("flood: dtype must be F64 or F32");
+}
+
+int appfis_flood_run_src(appfis_ctx* ctx, int dtype, void* h, void* qx, void* qy, const void* z, const void* n2,
+                         int nx, int ny, const appfis_flood_params* params, const appfis_flood_sources* sources,
+                         int steps, int variant, double* time_host, double* volume_host, int* steps_done)
+{
+    APPFIS_REQUIRE(ctx && params, "NULL argument");
+    APPFIS_REQUIRE(steps >= 0, "steps must be >= 0");
+    APPFIS_REQUIRE(variant == APPFIS_PLAIN || variant == APPFIS_CHOSEN, "flood: unknown variant %d", variant);
+    APPFIS_REQUIRE(!sources || sources->count >= 0, "sources: negative count");
+    void* hh[1] = {h};
+    void* qq[1] = {qx};
+    void* rr[1] = {qy};
+    const void* zz[1] = {z};
+    const void* nn[1] = {n2};
+    const bool graph = variant == APPFIS_CHOSEN;
+    if (dtype == APPFIS_F64)
+        return run_flood<double>(ctx, 1, &nx, &ny, hh, qq, rr, zz, nn, params, steps, graph, time_host, volume_host,
+                                 nullptr, sources, steps_done);
+    if (dtype == APPFIS_F32)
+        return run_flood<float>(ctx, 1, &nx, &ny, hh, qq, rr, zz, nn, params, steps, graph, time_host, volume_host,
+                                nullptr, sources, steps_done);
     return invalid("flood: dtype must be F64 or F32");
 }
 

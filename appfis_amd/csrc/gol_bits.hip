@@ -1,9 +1,9 @@
 // GameOfLife CHOSEN form: the same B3/S23 generations on a bit-packed copy of the field.
 //
 // The call packs the interior of the uint8 field into 32-bit words (bit b of word w of
-// row y = cell (32w + b, y), interior coordinates), advances the packed field one
-// generation per launch, and unpacks the last two generations back into the two uint8
-// buffers, ring exchanged: on return both buffers hold exactly what the PLAIN form
+// row y = cell (32w + b, y), interior coordinates), advances the packed field (many
+// generations per launch), and its last launch computes the last generation while
+// unpacking the last two back into the two uint8 buffers, rings exchanged: on return both buffers hold exactly what the PLAIN form
 // leaves (generation `gens` in buf[gens % 2], generation gens-1 in the other).  The
 // packed layout needs nx % 32 == 0 so that the periodic wrap falls on word boundaries;
 // appfis_gol runs the byte kernel for other widths.  Cells must hold 0 or 1.
@@ -72,19 +72,6 @@ __global__ __launch_bounds__(256) void k_pack(const uint8_t* __restrict__ u, uin
     P[t] = word;
 }
 
-// one word -> cells x = 32w .. 32w+31 of interior row y
-__global__ __launch_bounds__(256) void k_unpack(const uint32_t* __restrict__ P, uint8_t* __restrict__ u, int nx,
-                                                int ny, int NW)
-{
-    const size_t t = (size_t)blockIdx.x * 256 + threadIdx.x;
-    if (t >= (size_t)NW * ny) return;
-    const int y = (int)(t / NW), w = (int)(t - (size_t)y * NW);
-    uint8_t* row = u + (size_t)(y + 1) * (nx + 2) + 1 + 32 * w;
-    const uint32_t word = P[t];
-#pragma unroll
-    for (int b = 0; b < 32; b++) row[b] = (uint8_t)((word >> b) & 1u);
-}
-
 template <int K>
 __global__ __launch_bounds__(256) void k_step(const uint32_t* __restrict__ P, uint32_t* __restrict__ Q, int NW, int ny,
                                               unsigned nitems)
@@ -124,6 +111,73 @@ __global__ __launch_bounds__(256) void k_step(const uint32_t* __restrict__ P, ui
 }
 
 constexpr int KROWS = 8;
+
+// One padded uint8 row from a packed word: the 32 cells of word w, and the ring cell
+// that the periodic exchange (X first) copies from it -- ghost column nx+1 holds cell 0,
+// ghost column 0 holds cell nx-1.
+__device__ __forceinline__ void put_word(uint8_t* __restrict__ u, size_t W, int prow, int w, int NW, uint32_t word)
+{
+    uint8_t* row = u + (size_t)prow * W;
+#pragma unroll
+    for (int b = 0; b < 32; b++) row[1 + 32 * w + b] = (uint8_t)((word >> b) & 1u);
+    if (w == 0) row[W - 1] = (uint8_t)(word & 1u);
+    if (w == NW - 1) row[0] = (uint8_t)(word >> 31);
+}
+
+// The last generation of a call fused with the unpacking: from the packed generation
+// n-1 in P it computes generation n (as k_step does) and writes generation n into `un`
+// and generation n-1 into `uprev`, both with their periodic rings -- what one k_step,
+// unpacking both generations and exchanging both rings would leave, in one pass.  The exchange's Y step copies
+// padded row ny (ghost columns included) into ghost row 0 and padded row 1 into ghost
+// row ny+1, so the thread of interior row ny-1 also writes row 0 and that of row 0 also
+// writes row ny+1.  Reads only P: `uprev` may be the buffer the call's input came from.
+template <int K>
+__global__ __launch_bounds__(256) void k_step_unpack(const uint32_t* __restrict__ P, uint8_t* __restrict__ un,
+                                                     uint8_t* __restrict__ uprev, int nx, int NW, int ny,
+                                                     unsigned nitems)
+{
+    const unsigned t = xcd_tile(blockIdx.x, gridDim.x) * 256 + threadIdx.x;
+    if (t >= nitems) return;
+    const int band = (int)(t / (unsigned)NW), w = (int)(t - (unsigned)band * NW);
+    const int wl = w == 0 ? NW - 1 : w - 1;
+    const int wr = w == NW - 1 ? 0 : w + 1;
+    const int y0 = band * K;
+    const size_t W = (size_t)nx + 2;
+    uint32_t h0[K + 2], h1[K + 2], cen[K + 2];
+#pragma unroll
+    for (int r = 0; r < K + 2; r++) {
+        int y = y0 - 1 + r;
+        if (y < 0) y += ny;
+        while (y >= ny) y -= ny;
+        const uint32_t* row = P + (size_t)y * NW;
+        const uint32_t c = row[w], l = row[wl], rr = row[wr];
+        const uint32_t Wc = __builtin_amdgcn_alignbit(c, l, 31);
+        const uint32_t Ec = __builtin_amdgcn_alignbit(rr, c, 1);
+        h0[r] = xor3(Wc, c, Ec);
+        h1[r] = maj(Wc, c, Ec);
+        cen[r] = c;
+    }
+#pragma unroll
+    for (int k = 0; k < K; k++) {
+        const int y = y0 + k;
+        if (y >= ny) break;
+        const uint32_t s0 = xor3(h0[k], h0[k + 1], h0[k + 2]);
+        const uint32_t c0 = maj(h0[k], h0[k + 1], h0[k + 2]);
+        const uint32_t u1 = xor3(h1[k], h1[k + 1], h1[k + 2]);
+        const uint32_t c1 = maj(h1[k], h1[k + 1], h1[k + 2]);
+        const uint32_t next = life_rule(s0, c0, u1, c1, cen[k + 1]);
+        put_word(un, W, y + 1, w, NW, next);
+        put_word(uprev, W, y + 1, w, NW, cen[k + 1]);
+        if (y == ny - 1) {
+            put_word(un, W, 0, w, NW, next);
+            put_word(uprev, W, 0, w, NW, cen[k + 1]);
+        }
+        if (y == 0) {
+            put_word(un, W, ny + 1, w, NW, next);
+            put_word(uprev, W, ny + 1, w, NW, cen[k + 1]);
+        }
+    }
+}
 
 // Temporal blocking: G generations per launch.  A block of NWAVES waves spans whole
 // packed rows: NW = 64*WX words, lane l of wave (wx, wy) owns word 64*wx + l, and wave
@@ -578,29 +632,18 @@ int gol_bits_run(appfis_ctx* ctx, uint8_t* b0, uint8_t* b1, int nx, int ny, int 
     hipLaunchKernelGGL(golbits::k_pack, dim3(pblocks), dim3(256), 0, ctx->stream, b0, P[0], nx, ny, NW);
     APPFIS_LAUNCH_CHECK();
     int cur = 0;
-    if (pick >= 0 && gens > 1) {
-        st = run_tb(ctx, golbits::kTb[pick], P, &cur, NW, ny, gens - 1);
-        if (st) return st;
-        st = run_step(ctx, P, &cur, NW, ny, 1);
-    } else {
-        st = run_step(ctx, P, &cur, NW, ny, gens);
-    }
+    if (pick >= 0 && gens > 1) st = run_tb(ctx, golbits::kTb[pick], P, &cur, NW, ny, gens - 1);
+    else st = run_step(ctx, P, &cur, NW, ny, gens - 1);
     if (st) return st;
-    // generation gens is in P[cur], generation gens-1 in P[cur ^ 1]: unpack them into
-    // buf[gens % 2] and the other buffer, rings exchanged
+    // generation gens-1 is in P[cur]: the last generation runs fused with the unpacking
+    // of generations gens (into buf[gens % 2]) and gens-1 (the other buffer), rings
+    // exchanged
     uint8_t* buf[2] = {b0, b1};
-    const int dims[2] = {nx + 2, ny + 2};
-    for (int back = 0; back < 2; back++) {
-        const int g = gens - back;
-        uint8_t* dst = buf[g & 1];
-        if (g > 0) {
-            hipLaunchKernelGGL(golbits::k_unpack, dim3(pblocks), dim3(256), 0, ctx->stream, P[cur ^ back], dst, nx,
-                               ny, NW);
-            APPFIS_LAUNCH_CHECK();
-        }
-        st = appfis_exchange(ctx, dst, APPFIS_U8, 2, dims, 1, 1);
-        if (st) return st;
-    }
+    const int nbands = (ny + golbits::KROWS - 1) / golbits::KROWS;
+    const unsigned nitems = (unsigned)NW * (unsigned)nbands;
+    hipLaunchKernelGGL(golbits::k_step_unpack<golbits::KROWS>, dim3(ceil_div(nitems, 256)), dim3(256), 0,
+                       ctx->stream, P[cur], buf[gens & 1], buf[(gens & 1) ^ 1], nx, NW, ny, nitems);
+    APPFIS_LAUNCH_CHECK();
     return APPFIS_OK;
 }
 

@@ -57,6 +57,18 @@ class FloodParams(C.Structure):
         ("print_interval", C.c_double),
         ("rain_rate", C.c_double),
         ("start_time", C.c_double),
+        ("time_increment_fixed", C.c_int),
+        ("end_time", C.c_double),
+    ]
+
+
+class FloodSources(C.Structure):
+    _fields_ = [
+        ("count", C.c_int),
+        ("cell", C.POINTER(C.c_int)),
+        ("rows", C.c_int),
+        ("time", C.POINTER(C.c_double)),
+        ("flow", C.POINTER(C.c_double)),
     ]
 
 
@@ -123,6 +135,8 @@ def lib():
         "appfis_gol_candidates": ([C.c_char_p, i], i),
         "appfis_himeno": ([vp, i, C.POINTER(vp), i, i, i, d, i, i, i, dp], i),
         "appfis_flood_run": ([vp, i, vp, vp, vp, vp, vp, i, i, C.POINTER(FloodParams), i, i, dp, dp], i),
+        "appfis_flood_run_src": ([vp, i, vp, vp, vp, vp, vp, i, i, C.POINTER(FloodParams), C.POINTER(FloodSources),
+                                  i, i, dp, dp, ip], i),
         "appfis_flood_ensemble_run": ([vp, i, i, ip, ip, C.POINTER(vp), C.POINTER(vp), C.POINTER(vp),
                                        C.POINTER(vp), C.POINTER(vp), C.POINTER(FloodParams), i, dp, dp], i),
         "appfis_flood_dem": ([vp, i, vp, i, i, d, u64, i, d, d], i),
@@ -356,6 +370,25 @@ class Context:
                                      t.ctypes.data_as(C.POINTER(C.c_double)),
                                      v.ctypes.data_as(C.POINTER(C.c_double))), "flood_run")
         return t[:steps], v[:steps]
+
+    def flood_run_src(self, h, qx, qy, z, n2, nx, ny, params: FloodParams, steps, variant=PLAIN, src_cell=(),
+                      src_time=(), src_flow=None):
+        """flood_run with point sources: src_cell padded linear indices, src_time (s) and
+        src_flow a (rows, count) table (m^3/s).  Returns (time, volume) of the steps run
+        (fewer than `steps` when params.end_time stopped the loop)."""
+        cells = np.ascontiguousarray(src_cell, dtype=np.int32)
+        times = np.ascontiguousarray(src_time, dtype=np.float64)
+        flows = np.ascontiguousarray(src_flow if src_flow is not None else np.zeros((0, 0)), dtype=np.float64)
+        src = FloodSources(len(cells), cells.ctypes.data_as(C.POINTER(C.c_int)), len(times),
+                           times.ctypes.data_as(C.POINTER(C.c_double)), flows.ctypes.data_as(C.POINTER(C.c_double)))
+        t = np.zeros(max(steps, 1), dtype=np.float64)
+        v = np.zeros(max(steps, 1), dtype=np.float64)
+        done = C.c_int(0)
+        check(lib().appfis_flood_run_src(self.handle, h.dtype, h.ptr, qx.ptr, qy.ptr, z.ptr, n2.ptr, nx, ny,
+                                         C.byref(params), C.byref(src), steps, variant,
+                                         t.ctypes.data_as(C.POINTER(C.c_double)),
+                                         v.ctypes.data_as(C.POINTER(C.c_double)), C.byref(done)), "flood_run_src")
+        return t[:done.value], v[:done.value]
 
     def flood_ensemble_run(self, scen, params: FloodParams, steps, dtype):
         n = len(scen)

@@ -29,7 +29,7 @@
 extern "C" {
 #endif
 
-#define APPFIS_ABI_VERSION 1
+#define APPFIS_ABI_VERSION 2
 
 /* status codes */
 #define APPFIS_OK 0
@@ -223,11 +223,38 @@ typedef struct appfis_flood_params {
     double print_interval;/* dt is clamped to the next multiple of this (<=0: never) */
     double rain_rate;     /* uniform rainfall (m/s), added as h += rain*dt after the update */
     double start_time;    /* simulated time at entry (s) */
+    int time_increment_fixed; /* != 0: every step takes dt_fallback, no CFL and no print
+                               * clamp (arglist.time_increment_fixed, triton_appfis.hpp:322) */
+    double end_time;      /* > 0: run while simtime < end_time, as the reference's loop
+                           * (triton_appfis.hpp:316), for at most `steps` steps; <= 0: exactly
+                           * `steps` steps */
 } appfis_flood_params;
 
 int appfis_flood_run(appfis_ctx* ctx, int dtype, void* h, void* qx, void* qy, const void* z,
                      const void* n2, int nx, int ny, const appfis_flood_params* params, int steps,
                      int variant, double* time_host, double* volume_host);
+
+/* Point sources of the reference Triton app (hydrograph inflows, Triton/src/inflow.hpp:
+ * 93-136 and the functor of triton_appfis.hpp:1115-1141, applied after the update as at
+ * triton_appfis.hpp:358-377).  Every step, the hydrograph interval holding the step's start
+ * time is searched from the row the previous step stopped at; each source's flow is
+ * interpolated linearly in time (the last row's flow once past it) and flow*dt/dx^2 is
+ * added to the depth of its cell. */
+typedef struct appfis_flood_sources {
+    int count;            /* number of sources (0: none) */
+    const int* cell;      /* count distinct interior cells, linear padded index i + (nx+2)*j */
+    int rows;             /* hydrograph rows (>= 1 when count > 0) */
+    const double* time;   /* rows times (s) */
+    const double* flow;   /* rows x count flows (m^3/s), row-major: flow[r*count + k] */
+} appfis_flood_sources;
+
+/* appfis_flood_run with point sources (sources may be NULL; host tables; one scenario).
+ * steps_done (may be NULL) receives the number of steps run: `steps`, or fewer when
+ * params->end_time stopped the loop. */
+int appfis_flood_run_src(appfis_ctx* ctx, int dtype, void* h, void* qx, void* qy, const void* z,
+                         const void* n2, int nx, int ny, const appfis_flood_params* params,
+                         const appfis_flood_sources* sources, int steps, int variant,
+                         double* time_host, double* volume_host, int* steps_done);
 
 /* Ensemble: nscen independent scenarios of different sizes advanced together, one
  * batched launch per phase over a tile work-list, the step loop captured in a

@@ -62,6 +62,10 @@ def lib():
             "orc_himeno": ([i, C.POINTER(vp), i, i, i, d, i, dp], i),
             "orc_flood_run_f64": ([vp, vp, vp, vp, vp, i, i, d, d, d, d, d, d, d, i, dp, dp, dp], i),
             "orc_flood_run_f32": ([vp, vp, vp, vp, vp, i, i, d, d, d, d, d, d, d, i, dp, dp, dp], i),
+            "orc_flood_run_src_f64": ([vp, vp, vp, vp, vp, i, i, d, d, d, d, d, d, d, i, i, vp, i, vp, vp, i,
+                                       dp, dp, dp], i),
+            "orc_flood_run_src_f32": ([vp, vp, vp, vp, vp, i, i, d, d, d, d, d, d, d, i, i, vp, i, vp, vp, i,
+                                       dp, dp, dp], i),
             "orc_flood_dem_f64": ([vp, i, i, d, u64, i, d, d], i),
             "orc_omp_threads": ([], i),
             "orc_f32_to_f16_bits": ([C.c_float], C.c_uint16),
@@ -213,6 +217,29 @@ def flood_run(h, qx, qy, z, n2, nx, ny, dx, courant, hextra, dt_fallback, print_
     _chk(fn(_ptr(h), _ptr(qx), _ptr(qy), _ptr(z), _ptr(n2), nx, ny, dx, courant, hextra, dt_fallback,
             print_interval, rain, start_time, steps, t.ctypes.data_as(dpp), v.ctypes.data_as(dpp),
             dt.ctypes.data_as(dpp)), "flood_run")
+    return t[:steps], v[:steps], dt[:steps]
+
+
+def flood_run_src(h, qx, qy, z, n2, nx, ny, dx, courant, hextra, dt_fallback, print_interval, rain, start_time,
+                  steps, fixed_dt=False, src_cell=(), src_time=(), src_flow=None):
+    """flood_run with the reference app's point sources and fixed-increment mode:
+    src_cell are padded linear indices, src_time the hydrograph times (s) and src_flow a
+    (rows, nsrc) table of flows (m^3/s)."""
+    t = np.zeros(max(steps, 1))
+    v = np.zeros(max(steps, 1))
+    dt = np.zeros(max(steps, 1))
+    cells = np.ascontiguousarray(src_cell, dtype=np.int32)
+    times = np.ascontiguousarray(src_time, dtype=np.float64)
+    flows = np.ascontiguousarray(src_flow if src_flow is not None else np.zeros((0, 0)), dtype=np.float64)
+    nsrc = len(cells)
+    if nsrc:
+        assert flows.shape == (len(times), nsrc), (flows.shape, len(times), nsrc)
+    fn = lib().orc_flood_run_src_f64 if h.dtype == np.float64 else lib().orc_flood_run_src_f32
+    dpp = C.POINTER(C.c_double)
+    _chk(fn(_ptr(h), _ptr(qx), _ptr(qy), _ptr(z), _ptr(n2), nx, ny, dx, courant, hextra, dt_fallback,
+            print_interval, rain, start_time, int(bool(fixed_dt)), nsrc, _ptr(cells) if nsrc else None,
+            len(times), _ptr(times) if nsrc else None, _ptr(flows) if nsrc else None, steps,
+            t.ctypes.data_as(dpp), v.ctypes.data_as(dpp), dt.ctypes.data_as(dpp)), "flood_run_src")
     return t[:steps], v[:steps], dt[:steps]
 
 

@@ -271,11 +271,44 @@ static void FN(wetdry)(int W, int nx, int ny, REAL hextra, const REAL* h, REAL* 
         }
 }
 
-/* The step loop of triton_appfis.hpp:316-386 for `steps` steps (fixed count, dynamic
- * dt), no point sources; the config's rain is folded into the update pass. */
-int FN(orc_flood_run)(REAL* h, REAL* qx, REAL* qy, const REAL* z, const REAL* n2, int nx, int ny, double dx_,
-                      double cn_, double hextra_, double dt_fallback, double print_interval, double rain_,
-                      double start_time, int steps, double* time_out, double* volume_out, double* dt_out)
+/* Point sources (triton_appfis.hpp:358-377 on the host, functor 1115-1141): find the
+ * hydrograph interval of simtime starting from *idx_low (kept across steps), interpolate
+ * each source's flow linearly in time and add flow*dt/dx^2 to the depth of its cell.
+ * src_flow is rows x nsrc, row-major; src_cell are padded linear indices. */
+static void FN(sources)(REAL* h, int nsrc, const int* src_cell, int rows, const double* src_time,
+                        const double* src_flow, double simtime, double gdt, double dx, int* idx_low)
+{
+    int lo = *idx_low, hi = rows - 1, found = 0;
+    for (int i = lo; i < hi; i++)
+        if (src_time[i + 1] > simtime) {
+            lo = i;
+            hi = i + 1;
+            found = 1;
+            break;
+        }
+    if (!found) lo = hi;
+    *idx_low = lo;
+    for (int s = 0; s < nsrc; s++) {
+        double fl = src_flow[(size_t)lo * nsrc + s], fh = src_flow[(size_t)hi * nsrc + s];
+        double tl = src_time[lo], th = src_time[hi];
+        double flow;
+        if (lo == hi) flow = fl;
+        else flow = fl + (((fh - fl) * (simtime - tl)) / (th - tl));
+        double hij = (double)h[src_cell[s]];
+        hij += (flow * gdt) / (dx * dx);
+        h[src_cell[s]] = (REAL)hij;
+    }
+}
+
+/* The step loop of triton_appfis.hpp:316-386 for `steps` steps (fixed count): dynamic
+ * dt unless time_increment_fixed (then dt_fallback every step, no print clamp, as the
+ * reference skips the CFL block), the config's rain folded into the update pass, then
+ * the point sources (nsrc may be 0), the wet/dry fix and the time bookkeeping. */
+int FN(orc_flood_run_src)(REAL* h, REAL* qx, REAL* qy, const REAL* z, const REAL* n2, int nx, int ny, double dx_,
+                          double cn_, double hextra_, double dt_fallback, double print_interval, double rain_,
+                          double start_time, int time_increment_fixed, int nsrc, const int* src_cell, int rows,
+                          const double* src_time, const double* src_flow, int steps, double* time_out,
+                          double* volume_out, double* dt_out)
 {
     const int W = nx + 2, H = ny + 2;
     const size_t n = (size_t)W * (size_t)H;
@@ -287,22 +320,26 @@ int FN(orc_flood_run)(REAL* h, REAL* qx, REAL* qy, const REAL* z, const REAL* n2
     for (int q = 0; q < 6; q++) r[q] = work + (2 + q) * n;
     const REAL dx = (REAL)dx_, cn = (REAL)cn_, hextra = (REAL)hextra_, rain = (REAL)rain_;
     double simtime = start_time;
-    int print_id = 0;
+    int print_id = 0, idx_low = 0;
     for (int s = 0; s < steps; s++) {
-        FN(cfl)(W, nx, ny, dx, cn, hextra, qx, qy, h, dtv);
-        REAL m = RMAX;
-        for (int j = 1; j <= ny; j++)
-            for (int i = 1; i <= nx; i++)
-                if (AT(dtv, i, j) < m) m = AT(dtv, i, j);
-        double gdt = (double)m;
-        if (gdt >= (double)RMAX - 1.0) gdt = dt_fallback;
-        if (print_interval > 0.0 && simtime + gdt > print_interval * (print_id + 1))
-            gdt = print_interval * (print_id + 1) - simtime;
+        double gdt = dt_fallback;
+        if (!time_increment_fixed) {
+            FN(cfl)(W, nx, ny, dx, cn, hextra, qx, qy, h, dtv);
+            REAL m = RMAX;
+            for (int j = 1; j <= ny; j++)
+                for (int i = 1; i <= nx; i++)
+                    if (AT(dtv, i, j) < m) m = AT(dtv, i, j);
+            gdt = (double)m;
+            if (gdt >= (double)RMAX - 1.0) gdt = dt_fallback;
+            if (print_interval > 0.0 && simtime + gdt > print_interval * (print_id + 1))
+                gdt = print_interval * (print_id + 1) - simtime;
+        }
         const REAL dt = (REAL)gdt;
         FN(reset)(W, H, h, sqrth, r);
         FN(flux_x)(W, nx, ny, dx, dt, hextra, h, qx, qy, z, sqrth, r);
         FN(flux_y)(W, nx, ny, dx, dt, hextra, h, qx, qy, z, sqrth, r);
         FN(update)(W, nx, ny, dt, hextra, rain, h, qx, qy, n2, r);
+        if (nsrc > 0) FN(sources)(h, nsrc, src_cell, rows, src_time, src_flow, simtime, (double)dt, dx_, &idx_low);
         FN(wetdry)(W, nx, ny, hextra, h, qx, qy, z);
         simtime += (double)dt;
         if (print_interval > 0.0 && simtime >= print_interval * (print_id + 1)) print_id++;
@@ -317,6 +354,15 @@ int FN(orc_flood_run)(REAL* h, REAL* qx, REAL* qy, const REAL* z, const REAL* n2
     }
     free(work);
     return 0;
+}
+
+/* No point sources, dynamic dt: the config's workload. */
+int FN(orc_flood_run)(REAL* h, REAL* qx, REAL* qy, const REAL* z, const REAL* n2, int nx, int ny, double dx_,
+                      double cn_, double hextra_, double dt_fallback, double print_interval, double rain_,
+                      double start_time, int steps, double* time_out, double* volume_out, double* dt_out)
+{
+    return FN(orc_flood_run_src)(h, qx, qy, z, n2, nx, ny, dx_, cn_, hextra_, dt_fallback, print_interval, rain_,
+                                 start_time, 0, 0, NULL, 0, NULL, NULL, steps, time_out, volume_out, dt_out);
 }
 
 #undef AT

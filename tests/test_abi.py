@@ -38,7 +38,7 @@ def test_library_loads_and_reports_version():
     from appfis_amd import hip
 
     L = hip.lib()
-    assert L.appfis_abi_version() == 1
+    assert L.appfis_abi_version() == 2
     assert L.appfis_status_string(0) == b"ok"
     assert L.appfis_status_string(1) == b"invalid argument"
 

@@ -390,6 +390,27 @@ def test_himeno_linear_field_is_fixed_point():
     assert np.all(gosa < 1e-28)
 
 
+@pytest.mark.parametrize("axis", [0, 1, 2])
+def test_himeno_axis_orientation(axis):
+    # a_axis multiplies the forward neighbour p(+1) and c_axis the backward one p(-1)
+    # (himeno_appfis.hpp:184-193).  With p = the coordinate along `axis`, a_axis = 2,
+    # c_axis = 0 and the other a, c = 1, a3 = 1/6: s0 = 2(x+1) + 4x = 6x + 2, so
+    # ss = 1/3 and the new p = x + omega/3 everywhere inside; the transposed operands
+    # (a with p(-1), c with p(+1)) would give ss = -1/3, and a wrong axis ss = 0.
+    nx, ny, nz = 7, 8, 9
+    k, j, i = np.meshgrid(np.arange(nz), np.arange(ny), np.arange(nx), indexing="ij")
+    p = [i, j, k][axis].astype(np.float64)
+    a = [1.0, 1.0, 1.0, 1.0 / 6.0]
+    c = [1.0, 1.0, 1.0]
+    a[axis], c[axis] = 2.0, 0.0
+    arrs = _himeno_arrays(nx, ny, nz, p, a=a, c=c)
+    omega = 0.8
+    gosa = oracle.himeno(arrs, nx, ny, nz, omega, 1)
+    inner = (slice(1, -1),) * 3
+    assert np.allclose(arrs[1][inner], p[inner] + omega / 3.0, rtol=0, atol=1e-13)
+    assert gosa[0] == pytest.approx((nx - 2) * (ny - 2) * (nz - 2) / 9.0, rel=1e-12)
+
+
 @pytest.mark.parametrize("term", [0, 1, 2])
 def test_himeno_cross_terms(term):
     # p = product of two coordinates makes exactly one of the three corner terms equal
