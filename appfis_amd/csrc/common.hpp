@@ -73,6 +73,10 @@ inline size_t dtype_size(int dtype)
 int tune_select(appfis_ctx* ctx, const int key[6], int ncand, const char* const* names,
                 const std::function<int(int)>& run_variant, int reps, int* choice);
 
+// Looks up (once per device, with `device` current) the device address of the fixed
+// scratch array behind appfis_reduce_device_noalloc; called by appfis_ctx_create.
+int noalloc_scratch_setup(int device);
+
 // Validates ndim in 1..3, dims > 0, returns padded extents in d[3] (missing axes = 1).
 int check_shape(int ndim, const int* dims, int* d);
 

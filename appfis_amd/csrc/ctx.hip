@@ -216,6 +216,11 @@ int appfis_ctx_create(int device, appfis_ctx** out)
         appfis_ctx_destroy(c);
         return hip_fail(e, "context creation", __FILE__, __LINE__);
     }
+    int st = noalloc_scratch_setup(device);
+    if (st) {
+        appfis_ctx_destroy(c);
+        return st;
+    }
     *out = c;
     return APPFIS_OK;
 }

@@ -17,6 +17,7 @@
 
 #include <cfloat>
 #include <climits>
+#include <mutex>
 
 using namespace appfis;
 
@@ -184,20 +185,33 @@ __global__ __launch_bounds__(RB) void k_reduce(const T* __restrict__ data, Box b
     }
 }
 
-template <typename T, int OP>
-int launch_reduce(appfis_ctx* ctx, const void* data, const Box& b, uint64_t n, void* dev_out)
+// Layout of a reduction's scratch: [0,256) ticket, [256, 256 + MAX_BLOCKS*16) partials
+// (one 16-byte slot per block at most: KSum is the widest accumulator).
+constexpr size_t SCRATCH_PARTIALS = 256;
+constexpr size_t SCRATCH_BYTES = SCRATCH_PARTIALS + (size_t)MAX_BLOCKS * 16;
+static_assert(SCRATCH_BYTES == APPFIS_NOALLOC_REDUCE_SCRATCH_BYTES, "header and kernel disagree on the scratch size");
+
+unsigned reduce_blocks(const appfis_ctx* ctx, uint64_t n)
 {
-    using A = typename Ops<T, OP>::A;
     unsigned blocks = ceil_div(n, RB);
     unsigned cap = (unsigned)(ctx->num_cu > 0 ? ctx->num_cu * 4 : 1024);
     if (cap > MAX_BLOCKS) cap = MAX_BLOCKS;
     if (blocks > cap) blocks = cap;
     if (blocks == 0) blocks = 1;
-    // scratch: [0,256) ticket, [256, 256 + MAX_BLOCKS*16) partials
-    int st = ensure_scratch(ctx, 256 + (size_t)MAX_BLOCKS * 16 + 256);
-    if (st) return st;
-    unsigned* ticket = (unsigned*)ctx->scratch;
-    A* partials = (A*)((char*)ctx->scratch + 256);
+    return blocks;
+}
+
+// scratch: SCRATCH_BYTES of device memory whose ticket word is zero at launch (the last
+// block resets it, so consecutive reductions on one scratch need no clearing)
+template <typename T, int OP>
+int launch_reduce(appfis_ctx* ctx, const void* data, const Box& b, uint64_t n, void* dev_out, void* scratch)
+{
+    using A = typename Ops<T, OP>::A;
+    const unsigned blocks = reduce_blocks(ctx, n);
+    if (SCRATCH_PARTIALS + (size_t)blocks * sizeof(A) > SCRATCH_BYTES)
+        return invalid("reduce: %u partials exceed the %zu-byte scratch", blocks, SCRATCH_BYTES);
+    unsigned* ticket = (unsigned*)scratch;
+    A* partials = (A*)((char*)scratch + SCRATCH_PARTIALS);
     hipLaunchKernelGGL((k_reduce<T, OP>), dim3(blocks), dim3(RB), 0, ctx->stream, (const T*)data, b, n, partials,
                        ticket, dev_out);
     APPFIS_LAUNCH_CHECK();
@@ -205,12 +219,12 @@ int launch_reduce(appfis_ctx* ctx, const void* data, const Box& b, uint64_t n, v
 }
 
 template <typename T>
-int dispatch_op(appfis_ctx* ctx, const void* data, const Box& b, uint64_t n, int op, void* dev_out)
+int dispatch_op(appfis_ctx* ctx, const void* data, const Box& b, uint64_t n, int op, void* dev_out, void* scratch)
 {
     switch (op) {
-    case APPFIS_MAX: return launch_reduce<T, APPFIS_MAX>(ctx, data, b, n, dev_out);
-    case APPFIS_MIN: return launch_reduce<T, APPFIS_MIN>(ctx, data, b, n, dev_out);
-    case APPFIS_SUM: return launch_reduce<T, APPFIS_SUM>(ctx, data, b, n, dev_out);
+    case APPFIS_MAX: return launch_reduce<T, APPFIS_MAX>(ctx, data, b, n, dev_out, scratch);
+    case APPFIS_MIN: return launch_reduce<T, APPFIS_MIN>(ctx, data, b, n, dev_out, scratch);
+    case APPFIS_SUM: return launch_reduce<T, APPFIS_SUM>(ctx, data, b, n, dev_out, scratch);
     default: return invalid("unknown reduction op %d", op);
     }
 }
@@ -240,7 +254,7 @@ int make_box(int ndim, const int* dims, const int* box, Box* b, uint64_t* n)
 }
 
 int reduce_to_device(appfis_ctx* ctx, const void* data, int dtype, int ndim, const int* dims, const int* box, int op,
-                     void* dev_out)
+                     void* dev_out, void* scratch)
 {
     Box b;
     uint64_t n;
@@ -248,12 +262,70 @@ int reduce_to_device(appfis_ctx* ctx, const void* data, int dtype, int ndim, con
     if (st) return st;
     if (n == 0) b.n0 = b.n1 = b.n2 = 0;  // empty box: kernel writes the identity
     switch (dtype) {
-    case APPFIS_U8: return dispatch_op<uint8_t>(ctx, data, b, n, op, dev_out);
-    case APPFIS_I32: return dispatch_op<int32_t>(ctx, data, b, n, op, dev_out);
-    case APPFIS_F32: return dispatch_op<float>(ctx, data, b, n, op, dev_out);
-    case APPFIS_F64: return dispatch_op<double>(ctx, data, b, n, op, dev_out);
+    case APPFIS_U8: return dispatch_op<uint8_t>(ctx, data, b, n, op, dev_out, scratch);
+    case APPFIS_I32: return dispatch_op<int32_t>(ctx, data, b, n, op, dev_out, scratch);
+    case APPFIS_F32: return dispatch_op<float>(ctx, data, b, n, op, dev_out, scratch);
+    case APPFIS_F64: return dispatch_op<double>(ctx, data, b, n, op, dev_out, scratch);
     default: return invalid("reduce: unsupported dtype %d", dtype);
     }
+}
+
+// The context scratch of the allocating entry points: SCRATCH_BYTES, then 256 bytes for
+// the result that the host copies back.
+int ctx_reduce_scratch(appfis_ctx* ctx, void** scratch, void** result)
+{
+    int st = ensure_scratch(ctx, SCRATCH_BYTES + 256);
+    if (st) return st;
+    *scratch = ctx->scratch;
+    if (result) *result = (char*)ctx->scratch + SCRATCH_BYTES;
+    return APPFIS_OK;
+}
+
+}  // namespace
+
+// ---- fixed scratch of appfis_reduce_device_noalloc --------------------------------
+// One copy per GPU, zero at module load and held until the process ends.  Host code
+// never passes the symbol itself to a kernel (on the host it only names the variable):
+// it passes the device address that hipGetSymbolAddress returned for the current GPU.
+static __device__ __align__(256) unsigned char scratch[SCRATCH_BYTES];
+
+namespace {
+
+constexpr int MAX_DEVICES = 64;
+std::mutex g_scratch_mutex;
+void* g_scratch_ptr[MAX_DEVICES] = {};
+
+__global__ void k_clear_ticket(unsigned* ticket) { *ticket = 0u; }
+
+}  // namespace
+
+namespace appfis {
+
+int noalloc_scratch_setup(int device)
+{
+    if (device < 0 || device >= MAX_DEVICES)
+        return invalid("device %d: the fixed reduction scratch supports devices 0..%d", device, MAX_DEVICES - 1);
+    std::lock_guard<std::mutex> lock(g_scratch_mutex);
+    if (g_scratch_ptr[device]) return APPFIS_OK;
+    int cur = -1;
+    APPFIS_HIP_CHECK(hipGetDevice(&cur));
+    if (cur != device) return invalid("noalloc_scratch_setup: device %d is not current (%d is)", device, cur);
+    void* ptr = nullptr;
+    APPFIS_HIP_CHECK(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+    if (!ptr) return invalid("hipGetSymbolAddress returned NULL for the reduction scratch");
+    g_scratch_ptr[device] = ptr;
+    return APPFIS_OK;
+}
+
+}  // namespace appfis
+
+namespace {
+
+void* noalloc_scratch(int device)
+{
+    if (device < 0 || device >= MAX_DEVICES) return nullptr;
+    std::lock_guard<std::mutex> lock(g_scratch_mutex);
+    return g_scratch_ptr[device];
 }
 
 }  // namespace
@@ -264,17 +336,34 @@ int appfis_reduce_device(appfis_ctx* ctx, const void* data, int dtype, int ndim,
                          int op, void* dev_result)
 {
     APPFIS_REQUIRE(ctx && data && dev_result, "NULL argument");
-    return reduce_to_device(ctx, data, dtype, ndim, dims, box, op, dev_result);
+    void* scr = nullptr;
+    int st = ctx_reduce_scratch(ctx, &scr, nullptr);
+    if (st) return st;
+    return reduce_to_device(ctx, data, dtype, ndim, dims, box, op, dev_result, scr);
+}
+
+int appfis_reduce_device_noalloc(appfis_ctx* ctx, const void* data, int dtype, int ndim, const int* dims,
+                                 const int* box, int op, void* dev_result)
+{
+    APPFIS_REQUIRE(ctx && data && dev_result, "NULL argument");
+    void* scr = noalloc_scratch(ctx->device);
+    APPFIS_REQUIRE(scr, "device %d has no fixed reduction scratch (context not created through appfis_ctx_create)",
+                   ctx->device);
+    // the ticket must start at zero: a one-thread kernel clears it (a kernel, not a
+    // memset node, so that the call can be captured into a graph)
+    hipLaunchKernelGGL(k_clear_ticket, dim3(1), dim3(1), 0, ctx->stream, (unsigned*)scr);
+    APPFIS_LAUNCH_CHECK();
+    return reduce_to_device(ctx, data, dtype, ndim, dims, box, op, dev_result, scr);
 }
 
 int appfis_reduce_sum_wide(appfis_ctx* ctx, const void* data, int dtype, int ndim, const int* dims, const int* box,
                            void* result)
 {
     APPFIS_REQUIRE(ctx && data && result, "NULL argument");
-    int st = ensure_scratch(ctx, 256 + (size_t)MAX_BLOCKS * 16 + 256);
+    void *scr = nullptr, *dev_out = nullptr;
+    int st = ctx_reduce_scratch(ctx, &scr, &dev_out);
     if (st) return st;
-    void* dev_out = (char*)ctx->scratch + 256 + (size_t)MAX_BLOCKS * 16;
-    st = reduce_to_device(ctx, data, dtype, ndim, dims, box, APPFIS_SUM, dev_out);
+    st = reduce_to_device(ctx, data, dtype, ndim, dims, box, APPFIS_SUM, dev_out, scr);
     if (st) return st;
     APPFIS_HIP_CHECK(hipMemcpyAsync(result, dev_out, 8, hipMemcpyDeviceToHost, ctx->stream));
     APPFIS_HIP_CHECK(hipStreamSynchronize(ctx->stream));
@@ -285,10 +374,10 @@ int appfis_reduce(appfis_ctx* ctx, const void* data, int dtype, int ndim, const 
                   void* result)
 {
     APPFIS_REQUIRE(ctx && data && result, "NULL argument");
-    int st = ensure_scratch(ctx, 256 + (size_t)MAX_BLOCKS * 16 + 256);
+    void *scr = nullptr, *dev_out = nullptr;
+    int st = ctx_reduce_scratch(ctx, &scr, &dev_out);
     if (st) return st;
-    void* dev_out = (char*)ctx->scratch + 256 + (size_t)MAX_BLOCKS * 16;
-    st = reduce_to_device(ctx, data, dtype, ndim, dims, box, op, dev_out);
+    st = reduce_to_device(ctx, data, dtype, ndim, dims, box, op, dev_out, scr);
     if (st) return st;
     unsigned char wide[8];
     APPFIS_HIP_CHECK(hipMemcpyAsync(wide, dev_out, 8, hipMemcpyDeviceToHost, ctx->stream));

@@ -126,6 +126,7 @@ def lib():
         "appfis_reduce": ([vp, vp, i, i, ip, ip, i, vp], i),
         "appfis_reduce_sum_wide": ([vp, vp, i, i, ip, ip, vp], i),
         "appfis_reduce_device": ([vp, vp, i, i, ip, ip, i, vp], i),
+        "appfis_reduce_device_noalloc": ([vp, vp, i, i, ip, ip, i, vp], i),
         "appfis_heat2d": ([vp, i, vp, vp, i, i, d, i, i, i], i),
         "appfis_gol": ([vp, vp, vp, i, i, i, i, i], i),
         "appfis_fdtd3d": ([vp, i, vp, vp, i, i, i, dp, i, i], i),

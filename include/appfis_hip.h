@@ -144,6 +144,21 @@ int appfis_reduce_sum_wide(appfis_ctx* ctx, const void* data, int dtype, int ndi
 int appfis_reduce_device(appfis_ctx* ctx, const void* data, int dtype, int ndim, const int* dims,
                          const int* box, int op, void* dev_result);
 
+/* appfis_reduce_device_noalloc: no counterpart in the reference.  For callers that
+ * cannot allocate device memory (code under hipGraph capture, tight serving loops,
+ * harnesses that forbid hipMalloc): the same reduction and result as
+ * appfis_reduce_device, but its scratch (a ticket word and at most 2048 per-block
+ * partials, APPFIS_NOALLOC_REDUCE_SCRATCH_BYTES in all) is a fixed device array that
+ * each GPU holds from library load until the process ends, and the call allocates
+ * nothing, synchronises nothing and records no memset (the ticket is cleared by a
+ * kernel), so it can be captured into a graph.  The scratch use does not grow with the
+ * box, so any box that appfis_reduce_device accepts is accepted.  Every call on one GPU
+ * shares that GPU's array: calls that overlap on one GPU (two streams, two contexts,
+ * two host threads) race and give wrong results; serialise them. */
+#define APPFIS_NOALLOC_REDUCE_SCRATCH_BYTES (256 + 2048 * 16)
+int appfis_reduce_device_noalloc(appfis_ctx* ctx, const void* data, int dtype, int ndim, const int* dims,
+                                 const int* box, int op, void* dev_result);
+
 /* ---- HeatDiffusion 2D (config form of the reference functor, HeatDiffusion/src/
  *      heat_appfis.hpp:166-174, driven as in heat_appfis.hpp:86-91) -----------------
  * u' = u + c*(uN + uS + uE + uW - 4u) over the nx*ny interior, ghost 1, double
