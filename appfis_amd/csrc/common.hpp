@@ -47,6 +47,11 @@ void set_error(const char* fmt, ...);
 int hip_fail(hipError_t e, const char* what, const char* file, int line);
 int invalid(const char* fmt, ...);
 
+// APPFIS_ERR_UNSUPPORTED (with a message naming `what`) when ctx's stream is being
+// captured into a graph: for the steps that would allocate, free, synchronise or begin a
+// capture of their own, none of which a capture allows.
+int refuse_if_capturing(appfis_ctx* ctx, const char* what);
+
 // Ensure ctx->scratch holds at least `bytes`; returns status.
 int ensure_scratch(appfis_ctx* ctx, size_t bytes);
 int ensure_work(appfis_ctx* ctx, size_t bytes);
@@ -216,6 +221,12 @@ __host__ __device__ inline uint64_t rng(uint64_t seed, uint64_t n)
     do {                                                                            \
         hipError_t _e = hipGetLastError();                                          \
         if (_e != hipSuccess) return appfis::hip_fail(_e, "kernel launch", __FILE__, __LINE__); \
+    } while (0)
+
+#define APPFIS_NOT_CAPTURING(ctx, what)                                             \
+    do {                                                                            \
+        int _st = appfis::refuse_if_capturing((ctx), (what));                      \
+        if (_st) return _st;                                                        \
     } while (0)
 
 #define APPFIS_REQUIRE(cond, ...)                                                   \

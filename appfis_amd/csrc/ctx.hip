@@ -36,9 +36,21 @@ int invalid(const char* fmt, ...)
     return APPFIS_ERR_INVALID;
 }
 
+int refuse_if_capturing(appfis_ctx* ctx, const char* what)
+{
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    APPFIS_HIP_CHECK(hipStreamIsCapturing(ctx->stream, &cs));
+    if (cs == hipStreamCaptureStatusNone) return APPFIS_OK;
+    set_error("%s cannot run while the context's stream is captured into a graph (it would allocate, free, "
+              "synchronise or capture on its own); call it once before the capture, or use a capture-safe entry point",
+              what);
+    return APPFIS_ERR_UNSUPPORTED;
+}
+
 int ensure_scratch(appfis_ctx* ctx, size_t bytes)
 {
     if (ctx->scratch_bytes >= bytes) return APPFIS_OK;
+    APPFIS_NOT_CAPTURING(ctx, "growing the context scratch");
     if (ctx->scratch) {
         APPFIS_HIP_CHECK(hipStreamSynchronize(ctx->stream));
         APPFIS_HIP_CHECK(hipFree(ctx->scratch));
@@ -55,6 +67,7 @@ int ensure_scratch(appfis_ctx* ctx, size_t bytes)
 int ensure_work(appfis_ctx* ctx, size_t bytes)
 {
     if (ctx->work_bytes >= bytes) return APPFIS_OK;
+    APPFIS_NOT_CAPTURING(ctx, "growing the context workspace");
     if (ctx->work) {
         APPFIS_HIP_CHECK(hipStreamSynchronize(ctx->stream));
         APPFIS_HIP_CHECK(hipFree(ctx->work));
@@ -69,6 +82,7 @@ int ensure_work(appfis_ctx* ctx, size_t bytes)
 int ensure_pinned(appfis_ctx* ctx, size_t bytes)
 {
     if (ctx->pinned_bytes >= bytes) return APPFIS_OK;
+    APPFIS_NOT_CAPTURING(ctx, "growing the pinned staging buffer");
     if (ctx->pinned) {
         APPFIS_HIP_CHECK(hipStreamSynchronize(ctx->stream));
         APPFIS_HIP_CHECK(hipHostFree(ctx->pinned));
@@ -89,6 +103,7 @@ int tune_select(appfis_ctx* ctx, const int key[6], int ncand, const char* const*
             *choice = e.choice;
             return APPFIS_OK;
         }
+    APPFIS_NOT_CAPTURING(ctx, "autotuning (the first CHOSEN call for a shape)");
     std::string rep = "autotune kernel " + std::to_string(key[0]) + " dtype " + std::to_string(key[1]) + " shape " +
                       std::to_string(key[2]) + "x" + std::to_string(key[3]) + "x" + std::to_string(key[4]) + ":";
     int best = 0;
@@ -258,6 +273,7 @@ void* appfis_ctx_stream(appfis_ctx* ctx) { return ctx ? (void*)ctx->stream : nul
 int appfis_sync(appfis_ctx* ctx)
 {
     APPFIS_REQUIRE(ctx, "ctx is NULL");
+    APPFIS_NOT_CAPTURING(ctx, "appfis_sync");
     APPFIS_HIP_CHECK(hipStreamSynchronize(ctx->stream));
     return APPFIS_OK;
 }
@@ -275,6 +291,7 @@ int appfis_malloc(appfis_ctx* ctx, size_t bytes, void** out)
 {
     APPFIS_REQUIRE(ctx && out, "NULL argument");
     *out = nullptr;
+    APPFIS_NOT_CAPTURING(ctx, "appfis_malloc");
     if (bytes == 0) bytes = 1;
     APPFIS_HIP_CHECK(hipSetDevice(ctx->device));
     APPFIS_HIP_CHECK(hipMalloc(out, bytes));
@@ -286,6 +303,7 @@ int appfis_free(appfis_ctx* ctx, void* ptr)
 {
     APPFIS_REQUIRE(ctx, "ctx is NULL");
     if (!ptr) return APPFIS_OK;
+    APPFIS_NOT_CAPTURING(ctx, "appfis_free");
     APPFIS_HIP_CHECK(hipStreamSynchronize(ctx->stream));
     APPFIS_HIP_CHECK(hipFree(ptr));
     return APPFIS_OK;
@@ -294,6 +312,7 @@ int appfis_free(appfis_ctx* ctx, void* ptr)
 int appfis_memcpy_h2d(appfis_ctx* ctx, void* dst, const void* src, size_t bytes)
 {
     APPFIS_REQUIRE(ctx && dst && src, "NULL argument");
+    APPFIS_NOT_CAPTURING(ctx, "appfis_memcpy_h2d");
     APPFIS_HIP_CHECK(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, ctx->stream));
     APPFIS_HIP_CHECK(hipStreamSynchronize(ctx->stream));
     return APPFIS_OK;
@@ -302,6 +321,7 @@ int appfis_memcpy_h2d(appfis_ctx* ctx, void* dst, const void* src, size_t bytes)
 int appfis_memcpy_d2h(appfis_ctx* ctx, void* dst, const void* src, size_t bytes)
 {
     APPFIS_REQUIRE(ctx && dst && src, "NULL argument");
+    APPFIS_NOT_CAPTURING(ctx, "appfis_memcpy_d2h");
     APPFIS_HIP_CHECK(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, ctx->stream));
     APPFIS_HIP_CHECK(hipStreamSynchronize(ctx->stream));
     return APPFIS_OK;
@@ -336,6 +356,9 @@ int appfis_memcpy_d2d(appfis_ctx* ctx, void* dst, const void* src, size_t bytes)
 int appfis_memset(appfis_ctx* ctx, void* dst, int value, size_t bytes)
 {
     APPFIS_REQUIRE(ctx && dst, "NULL argument");
+    // a memset node in a captured graph is not reliable on replay: fill with a kernel
+    // (appfis_fill_value) inside a capture
+    APPFIS_NOT_CAPTURING(ctx, "appfis_memset");
     APPFIS_HIP_CHECK(hipMemsetAsync(dst, value, bytes, ctx->stream));
     return APPFIS_OK;
 }
@@ -350,6 +373,7 @@ int appfis_timer_start(appfis_ctx* ctx)
 int appfis_timer_stop(appfis_ctx* ctx, float* ms)
 {
     APPFIS_REQUIRE(ctx && ms, "NULL argument");
+    APPFIS_NOT_CAPTURING(ctx, "appfis_timer_stop");
     APPFIS_HIP_CHECK(hipEventRecord(ctx->ev_stop, ctx->stream));
     APPFIS_HIP_CHECK(hipEventSynchronize(ctx->ev_stop));
     APPFIS_HIP_CHECK(hipEventElapsedTime(ms, ctx->ev_start, ctx->ev_stop));
@@ -359,6 +383,7 @@ int appfis_timer_stop(appfis_ctx* ctx, float* ms)
 int appfis_graph_begin(appfis_ctx* ctx)
 {
     APPFIS_REQUIRE(ctx, "ctx is NULL");
+    APPFIS_NOT_CAPTURING(ctx, "appfis_graph_begin (a capture is already active)");
     APPFIS_HIP_CHECK(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
     return APPFIS_OK;
 }

@@ -445,6 +445,7 @@ extern "C" int appfis_fdtd3d(appfis_ctx* ctx, int dtype, void* buf0, void* buf1,
     void* s0 = nullptr;
     void* s1 = nullptr;
     if (!cached) {
+        APPFIS_NOT_CAPTURING(ctx, "appfis_fdtd3d CHOSEN's first call for a shape (autotuning)");
         const size_t half = (bytes + 255) / 256 * 256;
         st = ensure_work(ctx, 2 * half);
         if (st) return st;

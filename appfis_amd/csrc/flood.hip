@@ -726,6 +726,7 @@ int run_flood(appfis_ctx* ctx, int nscen, const int* nx, const int* ny, void* co
               bool graph, double* time_host, double* volume_host, double* dt_host,
               const appfis_flood_sources* src = nullptr, int* steps_done = nullptr)
 {
+    APPFIS_NOT_CAPTURING(ctx, "the flood step loop (it synchronises and captures graphs of its own)");
     const int nsrc = src ? src->count : 0;
     if (nsrc > 0) {
         APPFIS_REQUIRE(nscen == 1, "point sources are supported for one scenario per call");
@@ -1024,6 +1025,7 @@ int appfis_flood_dem(appfis_ctx* ctx, int dtype, void* z, int nx, int ny, double
                      double slope, double wall)
 {
     APPFIS_REQUIRE(ctx && z, "NULL argument");
+    APPFIS_NOT_CAPTURING(ctx, "appfis_flood_dem");
     APPFIS_REQUIRE(nx >= 1 && ny >= 1 && nhills >= 0 && nhills <= 4096, "bad DEM arguments");
     std::vector<double> hills((size_t)(nhills > 0 ? nhills : 1) * 4);
     const double L = (double)nx * dx;

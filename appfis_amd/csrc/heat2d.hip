@@ -424,6 +424,7 @@ extern "C" int appfis_heat2d(appfis_ctx* ctx, int dtype, void* buf0, void* buf1,
         void* s0 = nullptr;
         void* s1 = nullptr;
         if (!cached) {
+            APPFIS_NOT_CAPTURING(ctx, "appfis_heat2d CHOSEN's first call for a shape (autotuning)");
             const size_t half = ((size_t)(nx + 2) * (ny + 2) * dtype_size(dtype) + 255) / 256 * 256;
             int st = ensure_work(ctx, 2 * half);
             if (st) return st;
@@ -509,6 +510,7 @@ extern "C" int appfis_heat2d_ensemble(appfis_ctx* ctx, int dtype, int ngrid, con
                                       void* const* buf0, void* const* buf1, double c, int iters)
 {
     APPFIS_REQUIRE(ctx && nx && ny && buf0 && buf1, "NULL argument");
+    APPFIS_NOT_CAPTURING(ctx, "appfis_heat2d_ensemble (it captures and replays a graph of its own)");
     APPFIS_REQUIRE(ngrid >= 1, "heat2d ensemble: need at least one grid");
     APPFIS_REQUIRE(iters >= 0, "iters must be >= 0");
     if (iters == 0) return APPFIS_OK;

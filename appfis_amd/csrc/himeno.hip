@@ -347,6 +347,7 @@ extern "C" int appfis_himeno(appfis_ctx* ctx, int dtype, void* const* arrays, in
                              int iters, int variant, int outer, double* gosa_host)
 {
     APPFIS_REQUIRE(ctx && arrays, "NULL argument");
+    APPFIS_NOT_CAPTURING(ctx, "appfis_himeno (it returns gosa to the host)");
     for (int a = 0; a < 14; a++) APPFIS_REQUIRE(arrays[a], "arrays[%d] is NULL", a);
     APPFIS_REQUIRE(arrays[0] != arrays[1], "p0 and p1 must differ");
     APPFIS_REQUIRE(nx >= 3 && ny >= 3 && nz >= 3, "himeno needs at least 3 points per axis (shell included)");

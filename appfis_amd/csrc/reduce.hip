@@ -360,6 +360,7 @@ int appfis_reduce_sum_wide(appfis_ctx* ctx, const void* data, int dtype, int ndi
                            void* result)
 {
     APPFIS_REQUIRE(ctx && data && result, "NULL argument");
+    APPFIS_NOT_CAPTURING(ctx, "a reduction with a host result (use appfis_reduce_device)");
     void *scr = nullptr, *dev_out = nullptr;
     int st = ctx_reduce_scratch(ctx, &scr, &dev_out);
     if (st) return st;
@@ -374,6 +375,7 @@ int appfis_reduce(appfis_ctx* ctx, const void* data, int dtype, int ndim, const 
                   void* result)
 {
     APPFIS_REQUIRE(ctx && data && result, "NULL argument");
+    APPFIS_NOT_CAPTURING(ctx, "a reduction with a host result (use appfis_reduce_device)");
     void *scr = nullptr, *dev_out = nullptr;
     int st = ctx_reduce_scratch(ctx, &scr, &dev_out);
     if (st) return st;

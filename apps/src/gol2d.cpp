@@ -1,10 +1,17 @@
 // GameOfLife config workload (uint8 cells, periodic, B3/S23, every cell written) on
 // the APPFIS API: user functor through the generic Execute2D path (periodic exchange
 // on the device after every generation) against the hand-written kernel
-// (APPFIS::kernels::GameOfLife).  Prints one JSON line (live counts, GLUP/s, identity).
-//   gol2d [-dimx=8192] [-dimy=8192] [-iteration=1000] [-chosen]
+// (APPFIS::kernels::GameOfLife).  Then the result goes through the single-rank
+// Gather / Scatter of a uint8 grid (a type the reference's MPI path does not take,
+// Communication.hpp:67-83) and a binary file round trip (Array::SaveBinaryFile /
+// LoadBinaryFile): the gathered full grid must be the subgrid's interior, and a fresh
+// grid loaded from the file and scattered must hold it again.  Prints one JSON line
+// (live counts, GLUP/s, identities); -output also writes the gathered field to
+// output/gol.bin (uint8, row-major, nx x ny).
+//   gol2d [-dimx=8192] [-dimy=8192] [-iteration=1000] [-chosen] [-output]
 #include <cstdio>
 #include <cstring>
+#include <string>
 
 #include "Appfis/Appfis.hpp"
 
@@ -62,12 +69,29 @@ int main(int argc, char* argv[])
     APPFIS::FinalizeExecution(ga, gb, gw);
     const size_t n = (size_t)(nx + 2) * (ny + 2);
     const bool same = memcmp(ga.GetSubGrid()->GetData(), gb.GetSubGrid()->GetData(), n) == 0;
+    // uint8 Gather of the result, a binary file round trip, and a Scatter into a new grid
+    ga.AllocateFullGrid();
+    APPFIS::Gather(ga.GetFullGrid(), ga.GetSubGrid());
+    bool sg = true;
+    const uint8_t* sub = ga.GetSubGrid()->GetData();
+    const uint8_t* full = ga.GetFullGrid()->GetData();
+    for (int j = 0; j < ny && sg; j++) sg = memcmp(full + (size_t)j * nx, sub + (size_t)(j + 1) * (nx + 2) + 1, nx) == 0;
+    const std::string path = APPFIS::CheckArgument(argc, argv, "output") ? "./output/gol.bin" : "./gol_roundtrip.bin";
+    ga.GetFullGrid()->SaveBinaryFile(path);
+    APPFIS::Grid<uint8_t> gc(APPFIS::DIM_2D, nx, ny, APPFIS::SINGLE);
+    gc.AllocateFullGrid();
+    gc.GetFullGrid()->LoadBinaryFile(path);
+    APPFIS::Scatter(gc.GetFullGrid(), gc.GetSubGrid());
+    const uint8_t* sc = gc.GetSubGrid()->GetData();
+    for (int j = 1; j <= ny && sg; j++)
+        sg = memcmp(sc + (size_t)j * (nx + 2) + 1, sub + (size_t)j * (nx + 2) + 1, nx) == 0;
+    if (!APPFIS::CheckArgument(argc, argv, "output")) std::remove(path.c_str());
     printf("{\"app\": \"gol2d\", \"nx\": %d, \"ny\": %d, \"generations\": %d, \"live_count\": %lld, "
            "\"live_count_kernel\": %lld, \"generic_glups\": %.3f, \"kernel_glups\": %.3f, \"kernel_form\": \"%s\", "
-           "\"bit_identical\": %s}\n",
+           "\"bit_identical\": %s, \"scatter_gather_u8\": %s}\n",
            nx, ny, gens, (long long)la, (long long)lb, timer.CalculateBLUPS("GENERIC", nx * ny, gens),
            timer.CalculateBLUPS("KERNEL", nx * ny, gens), variant == APPFIS_PLAIN ? "plain" : "chosen",
-           same ? "true" : "false");
+           same ? "true" : "false", sg ? "true" : "false");
     APPFIS::Finalize();
-    return same ? 0 : 1;
+    return same && sg ? 0 : 1;
 }

@@ -18,6 +18,19 @@
  *     in 3D UP/DOWN are the z faces and FRONT/BACK the y faces.
  *   - Reduction ops follow APPFIS::REDUCTION (reference Appfis/Env.hpp:32):
  *     MAX=0 MIN=1 SUM=2.
+ *   - Graph capture.  While the context's stream is captured into a graph
+ *     (appfis_graph_begin, or the caller's own hipStreamBeginCapture on
+ *     appfis_ctx_stream), the entry points that only enqueue kernels can be captured:
+ *     appfis_exchange, appfis_set_boundary(_copy), appfis_fill_*, appfis_memcpy_d2d,
+ *     appfis_reduce_device_noalloc, appfis_reduce_device (once an earlier uncaptured
+ *     call on the context has sized its scratch), and appfis_heat2d / appfis_gol /
+ *     appfis_fdtd3d in every variant (the CHOSEN forms once an uncaptured call has tuned
+ *     the shape and sized the workspace).  Everything that would allocate, free,
+ *     synchronise, memset, return a host result or capture a graph of its own
+ *     (appfis_malloc/free/memset/memcpy_h2d/d2h/sync/timer_stop, appfis_reduce,
+ *     appfis_reduce_sum_wide, appfis_himeno, appfis_heat2d_ensemble, the flood entry
+ *     points, a first CHOSEN call) returns APPFIS_ERR_UNSUPPORTED under capture and
+ *     enqueues nothing.
  */
 #ifndef APPFIS_HIP_H
 #define APPFIS_HIP_H

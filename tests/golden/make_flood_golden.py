@@ -2,9 +2,10 @@
 cases of flood_cases.py.
 
 The app is the reference's Triton/src/main.cpp on the reference's APPFIS OpenMP runtime,
-compiled from its sources by `make -C oracle ref` into oracle/_ref/triton_acpu (needs
-/root/reference; run in the build container, never on the GPU machine).  It runs as an
-MPI singleton with `-output`, which writes h, u (= qx) and v (= qy) of the interior with
+compiled from its sources by `make -C oracle ref` into oracle/_ref/triton_acpu (so this
+runs only where the reference sources are; the fixture it writes is what the tests
+read).  It runs as an MPI singleton with `-output`, which writes h, u (= qx) and v (= qy)
+of the interior with
 six decimals (Array::SaveAsciiFile, Appfis/Array.hpp:447-471) and prints the number of
 steps.  The fixture stores those printed values as integers in units of 1e-6 (exactly
 what was printed) and the step count per case.

@@ -6,7 +6,7 @@ x fastest) and the live-cell count.
 The GPU test runs both HIP forms (byte kernel and bit-packed kernel) in chunks of 100
 generations and checks each chunk's interior against these digests, so the whole
 1000-generation run is compared with the CPU path, not only with itself.  The oracle
-takes a few minutes on the build container's cores:
+takes about ten minutes on eight CPU cores:
 
     python tests/golden/make_gol_golden.py
 """

@@ -10,8 +10,8 @@ MI355X:
                          flood kernels while simtime < sim_duration; its printed outputs
                          against the reference Triton app's own (tests/golden/flood_ref.npz).
 
-The reference's own apps are compiled unchanged against Appfis/ in the build container
-only (tests/test_apps_build.py): nothing built from the reference runs here.
+The reference's own apps are only compiled unchanged against Appfis/, where the
+reference sources are (tests/test_apps_build.py): nothing built from the reference runs here.
 """
 import json
 import os
@@ -66,7 +66,7 @@ def test_flood_app_reproduces_reference_triton_app(tmp_path, make, variant):
 
 @pytest.mark.parametrize("name,args", [
     ("heat2d", ["-dimx=300", "-dimy=211", "-iteration=37"]),
-    ("gol2d", ["-dimx=510", "-dimy=257", "-iteration=41"]),
+    ("gol2d", ["-dimx=510", "-dimy=257", "-iteration=41", "-output"]),
     ("fdtd3d", ["-dimx=70", "-dimy=33", "-dimz=41", "-iteration=5"]),
 ])
 def test_api_apps_generic_path_matches_hand_kernels(tmp_path, name, args):
@@ -83,6 +83,11 @@ def test_api_apps_generic_path_matches_hand_kernels(tmp_path, name, args):
         oracle.fill_bernoulli(a, 1, 1002, 0.25)
         ref = oracle.gol_u8(a, np.zeros_like(a), nx, ny, gens)
         assert res["live_count"] == int(ref[1:-1, 1:-1].sum())
+        # uint8 Gather / Scatter and the binary file round trip (single rank): the file
+        # written from the gathered full grid is the oracle's interior
+        assert res["scatter_gather_u8"] is True, res
+        got = np.fromfile(tmp_path / "output" / "gol.bin", dtype=np.uint8).reshape(ny, nx)
+        assert np.array_equal(got, ref[1:-1, 1:-1])
     if name == "heat2d":
         nx, ny, it = 300, 211, 37
         a = np.zeros((ny + 2, nx + 2))

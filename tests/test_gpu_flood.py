@@ -18,6 +18,16 @@ pytestmark = pytest.mark.gpu
 
 FLOOD_F64_RTOL = 1e-9
 FLOOD_F32_H_RTOL = 1e-3
+# Stated accuracy of the FP32 bulk form on the config workload (2000 steps from a dry
+# start under rain) against the FP64 oracle: relative L2 of h and of the Kahan volume.
+# The FP32 error depends on the scenario: 9e-6 on scenario 23, 1.8e-3 on scenario 4, where
+# water collects in a fast jet along the west wall and the jet's cells differ by up to
+# 3.5 cm.  That is the cost of FP32 storage, not of the kernel's arithmetic: the oracle's
+# own FP32 restatement (libm-accurate operations) is off by 1.6e-3, and FP64 arithmetic
+# with h, qx, qy rounded to FP32 after every step by 5.3e-4 (both measured with the oracle
+# on the CPU; too slow to repeat in this test).
+FLOOD_F32_CONFIG_H_RTOL = 5e-3
+FLOOD_F32_CONFIG_VOL_RTOL = 1e-4
 DX, CN, HEXTRA, DT_FALLBACK, RAIN = 10.0, 0.5, 1e-3, 1.0, 50e-3 / 3600.0
 
 
@@ -148,3 +158,66 @@ def test_flood_dem_device_matches_oracle(ctx):
     ref = oracle.flood_dem(256, 256, DX, 77)
     # device exp vs glibc exp: a few ulps
     assert np.allclose(z.download(), ref, rtol=1e-14, atol=1e-12)
+
+
+# ---------------------------------------------------------------------------------
+# The config workload itself: scenarios of the 24-scenario ensemble (sides drawn from
+# [512, 4096] by splitmix64 seed 1005, DEM seed 1005 + s), 2000 steps from a dry start
+# under 50 mm/h of rain, against the FP64 oracle.  The two smallest sides (785 and 870,
+# scenarios 23 and 4) keep the oracle to seconds.  Perturbing n^2 by a few ulps moves the
+# oracle's fields by at most 3e-13 on these runs, so the FP64 bar is FLOOD_F64_RTOL.
+# ---------------------------------------------------------------------------------
+CONFIG_STEPS = 2000
+CONFIG_SCEN = [(23, 785), (4, 870)]
+_oracle_cache = {}
+
+
+def config_scenario(s, n):
+    z = oracle.flood_dem(n, n, DX, 1005 + s)
+    return [np.zeros_like(z), np.zeros_like(z), np.zeros_like(z), z, np.full_like(z, 0.035 ** 2)]
+
+
+def config_oracle(s, n):
+    if (s, n) not in _oracle_cache:
+        print(f"[flood config] oracle scenario {s} ({n}^2) x {CONFIG_STEPS} steps", flush=True)
+        _oracle_cache[(s, n)] = cpu_run(config_scenario(s, n), CONFIG_STEPS)
+    return _oracle_cache[(s, n)]
+
+
+def test_flood_config_scenario_fp64_against_oracle(ctx):
+    s, n = CONFIG_SCEN[0]
+    (h, qx, qy), tc, vc = config_oracle(s, n)
+    for variant in (hip.PLAIN, hip.CHOSEN):
+        print(f"[flood config] GPU FP64 scenario {s} ({n}^2) variant {variant}", flush=True)
+        g, tg, vg = gpu_run(ctx, config_scenario(s, n), CONFIG_STEPS, variant)
+        for a, b in zip(g, (h, qx, qy)):
+            assert np.abs(a - b).max() <= FLOOD_F64_RTOL * max(1.0, float(np.abs(b).max()))
+        assert np.allclose(tg, tc, rtol=1e-12, atol=0) and np.allclose(vg, vc, rtol=1e-12, atol=0)
+    assert h.max() > 0.1 and np.abs(qx).max() > 1e-3  # 2000 steps of rain made runoff
+
+
+def test_flood_config_ensemble_against_oracle(ctx):
+    """Config-range scenarios batched in one ensemble call (tile work-list over both,
+    graph-captured step chunks), FP64 and the FP32 bulk form, each against the FP64
+    oracle of its scenario."""
+    refs = [config_oracle(s, n) for s, n in CONFIG_SCEN]
+    for dtype in (np.float64, np.float32):
+        dt = hip.F64 if dtype == np.float64 else hip.F32
+        scen = []
+        for s, n in CONFIG_SCEN:
+            d = [ctx.from_numpy(a.astype(dtype), dt) for a in config_scenario(s, n)]
+            scen.append({"nx": n, "ny": n, "h": d[0], "qx": d[1], "qy": d[2], "z": d[3], "n2": d[4]})
+        print(f"[flood config] GPU ensemble dtype {dt}: {[n for _, n in CONFIG_SCEN]}", flush=True)
+        t, v = ctx.flood_ensemble_run(scen, params(), CONFIG_STEPS, dt)
+        for k, ((h, qx, qy), tc, vc) in enumerate(refs):
+            gh = scen[k]["h"].download().astype(np.float64)
+            if dtype == np.float64:
+                for a, b in zip((gh, scen[k]["qx"].download(), scen[k]["qy"].download()), (h, qx, qy)):
+                    assert np.abs(a - b).max() <= FLOOD_F64_RTOL * max(1.0, float(np.abs(b).max()))
+                assert np.allclose(v[k], vc, rtol=1e-12, atol=0)
+            else:
+                rel = float(np.linalg.norm(gh - h) / np.linalg.norm(h))
+                vrel = float(abs(v[k][-1] - vc[-1]) / vc[-1])
+                print(f"[flood config] FP32 scenario {CONFIG_SCEN[k]}: h rel-L2 {rel:.3e}, "
+                      f"volume rel {vrel:.3e}, simulated time {t[k][-1]:.3f} s (FP64 {tc[-1]:.3f} s)", flush=True)
+                assert rel <= FLOOD_F32_CONFIG_H_RTOL and vrel <= FLOOD_F32_CONFIG_VOL_RTOL

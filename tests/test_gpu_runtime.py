@@ -18,7 +18,7 @@ NPDT = {hip.U8: np.uint8, hip.I32: np.int32, hip.F32: np.float32, hip.F64: np.fl
 
 
 def test_abi_version_and_roundtrip(ctx):
-    assert hip.lib().appfis_abi_version() == 1
+    assert hip.lib().appfis_abi_version() == 2
     a = np.arange(1000, dtype=np.float64)
     d = ctx.from_numpy(a)
     assert np.array_equal(d.download(), a)
@@ -131,3 +131,49 @@ def test_reduce_max_quirk_and_large(ctx):
     exact = math.fsum(big[1:256, 1:256, 1:129].ravel())
     for _ in range(3):
         assert abs(ctx.reduce_sum_wide(db, box) - exact) <= 4 * np.spacing(exact)
+
+
+def test_graph_capture_of_plain_heat2d_and_device_reduction(ctx):
+    """The capture-safe entry points captured into one graph (PLAIN Heat2D sweeps, then a
+    device-side SUM of the result) and replayed three times, with eager uploads and
+    synchronisations between the replays: every replay equals the oracle.  Entry points
+    that would synchronise or allocate refuse under the capture and enqueue nothing."""
+    import ctypes as C
+
+    nx, ny, it = 130, 67, 10
+    a = np.zeros((ny + 2, nx + 2))
+    oracle.fill_uniform(a, 1, 1001)
+    d0, d1 = ctx.from_numpy(a), ctx.array(a.shape, hip.F64)
+    out = ctx.array((1,), hip.F64)
+    L = hip.lib()
+    dims = (C.c_int * 2)(nx + 2, ny + 2)
+    box = (C.c_int * 4)(1, nx + 1, 1, ny + 1)
+    # an uncaptured call sizes the context scratch of appfis_reduce_device
+    hip.check(L.appfis_reduce_device(ctx.handle, d1.ptr, hip.F64, 2, dims, box, hip.SUM, out.ptr), "warm")
+    ctx.sync()
+    ctx.graph_begin()
+    try:
+        ctx.heat2d(d0, d1, nx, ny, 0.1, it, hip.PLAIN)
+        st_dev = L.appfis_reduce_device(ctx.handle, d0.ptr, hip.F64, 2, dims, box, hip.SUM, out.ptr)
+        host = np.zeros(1)
+        st_host = L.appfis_reduce(ctx.handle, d0.ptr, hip.F64, 2, dims, box, hip.SUM, host.ctypes.data)
+        msg = L.appfis_last_error().decode()
+    finally:
+        g = ctx.graph_end()
+    try:
+        assert st_dev == 0
+        assert st_host == 3 and "captured" in msg, (st_host, msg)  # APPFIS_ERR_UNSUPPORTED
+        for rep in range(3):
+            scale = 1.0 + rep
+            src = a * scale
+            d0.upload(src)
+            ctx.graph_launch(g, 1)
+            ctx.sync()
+            ref = oracle.heat2d(src.copy(), np.zeros_like(src), nx, ny, 0.1, it)
+            got = d0.download()
+            print(f"[graph] replay {rep}: field bit-exact {np.array_equal(got, ref)}", flush=True)
+            assert np.array_equal(got, ref)
+            exp = math.fsum(ref[1:-1, 1:-1].ravel())
+            assert abs(out.download()[0] - exp) <= 1e-14 * float(np.abs(ref).sum())
+    finally:
+        ctx.graph_destroy(g)
