@@ -79,9 +79,18 @@ def test_valu_lookup(tmp_path, monkeypatch):
 def test_gol_kernel_names_match_the_profiled_symbols():
     names = json.load(open(os.path.join(ROOT, "profiles", "traffic.json")))
     for tname, want in (("tw4 k4 g8 w12", "k_step_tw<4, 4, 8, 12>"), ("tb1 k12 g8 w16", "k_step_tb1<12, 8, 16>"),
-                        ("tb k10 g4 w16", "k_step_tb<10, 4, 16>"), (None, "k_step<8>")):
+                        ("tb k10 g4 w16", "k_step_tb<10, 4, 16>")):
         assert bench.gol_kernel_name(tname) == want
         assert want in names, want
+    # widths without a blocked candidate run the one-generation kernel (not profiled at 8192)
+    assert bench.gol_kernel_name(None) == "k_step<8>"
+
+
+def test_cpu_baseline_reports_median_of_runs_and_the_machine():
+    r = bench.cpu_baseline_gol(nx=128, ny=96, target_s=0.2, runs=3)
+    assert len(r["runs"]) == 3 and r["value"] == round(sorted(r["runs"])[1], 4)
+    assert r["kind"] == "port" and r["cores"] >= 1 and r["cpu_model"]
+    assert set(r["omp_env"]) == {"OMP_NUM_THREADS", "OMP_PROC_BIND", "OMP_PLACES"}
 
 
 def test_heat_kernel_names():
