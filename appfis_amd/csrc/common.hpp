@@ -39,6 +39,14 @@ struct appfis_ctx {
     int num_cu = 0;
     std::vector<appfis_tune_entry> tuned;  // autotuner cache
     std::string tune_report;               // text of the most recent decision
+    // instantiated launch chains of the ensemble and flood loops, keyed by every value
+    // their kernel arguments hold (the tables they read are uploaded by each call); a
+    // later call with the same key replays the graph instead of capturing it again
+    struct graph_entry {
+        std::vector<uint64_t> key;
+        hipGraphExec_t exec;
+    };
+    std::vector<graph_entry> graphs;
 };
 
 namespace appfis {
@@ -56,6 +64,24 @@ int refuse_if_capturing(appfis_ctx* ctx, const char* what);
 int ensure_scratch(appfis_ctx* ctx, size_t bytes);
 int ensure_work(appfis_ctx* ctx, size_t bytes);
 int ensure_pinned(appfis_ctx* ctx, size_t bytes);
+
+// The launch chain of `key` as an instantiated graph: the cached one, else `enqueue`
+// (which only launches kernels on ctx->stream) captured, instantiated and cached.  The
+// cache holds a few graphs per context; the oldest is destroyed, after the stream has
+// drained, to make room, and all of them when the workspace their arguments point into
+// is reallocated.
+int cached_graph(appfis_ctx* ctx, std::vector<uint64_t> key, const std::function<int()>& enqueue,
+                 hipGraphExec_t* exec);
+// Destroy every cached graph (the stream must have drained).
+int clear_graphs(appfis_ctx* ctx);
+
+inline uint64_t key_bits(double x)
+{
+    uint64_t u;
+    memcpy(&u, &x, sizeof u);
+    return u;
+}
+inline uint64_t key_ptr(const void* p) { return (uint64_t)(uintptr_t)p; }
 
 inline size_t dtype_size(int dtype)
 {

@@ -70,6 +70,9 @@ int ensure_work(appfis_ctx* ctx, size_t bytes)
     APPFIS_NOT_CAPTURING(ctx, "growing the context workspace");
     if (ctx->work) {
         APPFIS_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+        // the cached graphs hold pointers into the old workspace
+        int st = clear_graphs(ctx);
+        if (st) return st;
         APPFIS_HIP_CHECK(hipFree(ctx->work));
         ctx->work = nullptr;
         ctx->work_bytes = 0;
@@ -92,6 +95,62 @@ int ensure_pinned(appfis_ctx* ctx, size_t bytes)
     size_t want = bytes < 4096 ? 4096 : bytes;
     APPFIS_HIP_CHECK(hipHostMalloc(&ctx->pinned, want, hipHostMallocDefault));
     ctx->pinned_bytes = want;
+    return APPFIS_OK;
+}
+
+constexpr size_t kMaxGraphs = 8;
+
+int clear_graphs(appfis_ctx* ctx)
+{
+    hipError_t first = hipSuccess;
+    for (auto& g : ctx->graphs) {
+        hipError_t e = hipGraphExecDestroy(g.exec);
+        if (first == hipSuccess) first = e;
+    }
+    ctx->graphs.clear();
+    if (first != hipSuccess) return hip_fail(first, "graph cache teardown", __FILE__, __LINE__);
+    return APPFIS_OK;
+}
+
+int cached_graph(appfis_ctx* ctx, std::vector<uint64_t> key, const std::function<int()>& enqueue,
+                 hipGraphExec_t* exec)
+{
+    for (auto& g : ctx->graphs)
+        if (g.key == key) {
+            *exec = g.exec;
+            return APPFIS_OK;
+        }
+    APPFIS_NOT_CAPTURING(ctx, "capturing a launch chain");
+    APPFIS_HIP_CHECK(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+    const int st = enqueue();
+    hipGraph_t graph = nullptr;
+    hipError_t e = hipStreamEndCapture(ctx->stream, &graph);
+    if (st != APPFIS_OK) {
+        if (graph) (void)hipGraphDestroy(graph);
+        return st;
+    }
+    if (e != hipSuccess) return hip_fail(e, "graph capture", __FILE__, __LINE__);
+    hipGraphExec_t ge = nullptr;
+    e = hipGraphInstantiate(&ge, graph, nullptr, nullptr, 0);
+    hipError_t e2 = hipGraphDestroy(graph);
+    if (e != hipSuccess) return hip_fail(e, "graph instantiate", __FILE__, __LINE__);
+    if (e2 != hipSuccess) {
+        (void)hipGraphExecDestroy(ge);
+        return hip_fail(e2, "graph destroy", __FILE__, __LINE__);
+    }
+    if (ctx->graphs.size() >= kMaxGraphs) {
+        // the oldest may still be running: drain the stream before destroying it
+        hipError_t es = hipStreamSynchronize(ctx->stream);
+        hipError_t ed = hipGraphExecDestroy(ctx->graphs.front().exec);
+        ctx->graphs.erase(ctx->graphs.begin());
+        if (es == hipSuccess) es = ed;
+        if (es != hipSuccess) {
+            (void)hipGraphExecDestroy(ge);
+            return hip_fail(es, "graph cache eviction", __FILE__, __LINE__);
+        }
+    }
+    ctx->graphs.push_back({std::move(key), ge});
+    *exec = ge;
     return APPFIS_OK;
 }
 
@@ -250,6 +309,8 @@ int appfis_ctx_destroy(appfis_ctx* ctx)
     };
     note(hipSetDevice(ctx->device));
     if (ctx->stream) note(hipStreamSynchronize(ctx->stream));
+    for (auto& g : ctx->graphs) note(hipGraphExecDestroy(g.exec));
+    ctx->graphs.clear();
     if (ctx->scratch) note(hipFree(ctx->scratch));
     if (ctx->work) note(hipFree(ctx->work));
     if (ctx->pinned) note(hipHostFree(ctx->pinned));

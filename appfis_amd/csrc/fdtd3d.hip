@@ -12,7 +12,10 @@
 // the x neighbours and the y neighbours across thread rows are read from the plane
 // staged in LDS.  The LDS plane is double-buffered and the next plane's halo and queue
 // entries are prefetched into registers before the barrier, so each z step has one
-// barrier and its global-load latency overlaps the previous step's arithmetic.
+// barrier and its global-load latency overlaps the previous step's arithmetic.  Every
+// global load and store is a buffer instruction at a byte offset that each thread fixes
+// once for the march (out-of-array slots read 0 and non-interior stores are dropped by
+// the buffer range check), so the march spends no VALU work on addresses or validity.
 //
 // dtype F32 / F64: storage and arithmetic in that type.  F16 / BF16: storage only,
 // arithmetic in FP32, round-to-nearest-even on store.
@@ -44,39 +47,107 @@ template <> struct Cvt<__hip_bfloat16> {
     __device__ static __hip_bfloat16 st(float v) { return __float2bfloat16(v); }
 };
 
+// Buffer access to one z-plane of an array: the resource spans the plane's bytes, and an
+// offset of 2^31 -- past any plane, since the launch checks keep the whole array under
+// 2^31 elements and so a plane under 2^31 / 9 elements of at most 8 bytes -- reads 0 and
+// drops a store (the hardware range check of buffer instructions).  The kernel keeps
+// one fixed in-plane byte offset per access slot for the whole march, 2^31 where the
+// slot is outside the array or not stored, so the accesses need no validity selects, no
+// branches and no 64-bit address arithmetic.
+constexpr uint32_t kOut = 0x80000000u;
+typedef unsigned int u32x2 __attribute__((ext_vector_type(2)));
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+
+template <typename S>
+__device__ inline __amdgpu_buffer_rsrc_t plane_rsrc(const S* base, int plane_bytes)
+{
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<S*>(base), (short)0, plane_bytes, 0x00020000);
+}
+
+__device__ inline double u2d(unsigned lo, unsigned hi) { return __hiloint2double((int)hi, (int)lo); }
+
+// one element at byte offset `off`, widened to the arithmetic type
+template <typename S>
+__device__ inline typename Cvt<S>::C bload(__amdgpu_buffer_rsrc_t r, uint32_t off)
+{
+    if constexpr (sizeof(S) == 2) {
+        const uint16_t b = __builtin_amdgcn_raw_buffer_load_b16(r, off, 0, 0);
+        S x;
+        __builtin_memcpy(&x, &b, 2);
+        return Cvt<S>::ld(x);
+    } else if constexpr (sizeof(S) == 4) {
+        return __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(r, off, 0, 0));
+    } else {
+        const u32x2 b = __builtin_amdgcn_raw_buffer_load_b64(r, off, 0, 0);
+        return u2d(b[0], b[1]);
+    }
+}
+
 constexpr int R = 4;     // stencil radius = ghost width
 constexpr int TXT = 64;  // threads along x (one wave per thread row)
 
-// VX consecutive storage elements of a point group; VX = 2 with 16-bit storage moves
-// both through one 32-bit access (the pair offset is even: see the launch checks).
+// VX consecutive storage elements of a point group, moved by one buffer access (a pair
+// sits at an even element offset, so its address is 4-byte aligned once the buffer is:
+// the launch checks ask 4-byte aligned buffers of the 16-bit pair tilings).
 template <typename S, int VX>
 struct Vec {
-    __device__ static void load(const S* __restrict__ base, int off, typename Cvt<S>::C (&out)[VX])
+    using C = typename Cvt<S>::C;
+    __device__ static void load(__amdgpu_buffer_rsrc_t r, uint32_t off, C (&out)[VX])
     {
-        if constexpr (VX == 2 && sizeof(S) == 2) {
-            const uint32_t w = reinterpret_cast<const uint32_t*>(base)[off >> 1];
+        if constexpr (VX == 1) {
+            out[0] = bload<S>(r, off);
+        } else if constexpr (sizeof(S) == 2) {
+            const uint32_t w = __builtin_amdgcn_raw_buffer_load_b32(r, off, 0, 0);
             S lo, hi;
             const uint16_t l16 = (uint16_t)(w & 0xFFFFu), h16 = (uint16_t)(w >> 16);
             __builtin_memcpy(&lo, &l16, 2);
             __builtin_memcpy(&hi, &h16, 2);
             out[0] = Cvt<S>::ld(lo);
             out[1] = Cvt<S>::ld(hi);
+        } else if constexpr (sizeof(S) == 4) {
+            const u32x2 w = __builtin_amdgcn_raw_buffer_load_b64(r, off, 0, 0);
+            out[0] = __uint_as_float(w[0]);
+            out[1] = __uint_as_float(w[1]);
         } else {
-#pragma unroll
-            for (int e = 0; e < VX; e++) out[e] = Cvt<S>::ld(base[off + e]);
+            const u32x4 w = __builtin_amdgcn_raw_buffer_load_b128(r, off, 0, 0);
+            out[0] = u2d(w[0], w[1]);
+            out[1] = u2d(w[2], w[3]);
         }
     }
-    __device__ static void store(S* __restrict__ base, int off, const typename Cvt<S>::C (&in)[VX])
+    __device__ static void store(__amdgpu_buffer_rsrc_t r, uint32_t off, const C (&in)[VX])
     {
-        if constexpr (VX == 2 && sizeof(S) == 2) {
-            const S lo = Cvt<S>::st(in[0]), hi = Cvt<S>::st(in[1]);
-            uint16_t l16, h16;
-            __builtin_memcpy(&l16, &lo, 2);
-            __builtin_memcpy(&h16, &hi, 2);
-            reinterpret_cast<uint32_t*>(base)[off >> 1] = (uint32_t)l16 | ((uint32_t)h16 << 16);
-        } else {
+        if constexpr (sizeof(S) == 2) {
+            uint16_t b[VX];
 #pragma unroll
-            for (int e = 0; e < VX; e++) base[off + e] = Cvt<S>::st(in[e]);
+            for (int e = 0; e < VX; e++) {
+                const S x = Cvt<S>::st(in[e]);
+                __builtin_memcpy(&b[e], &x, 2);
+            }
+            if constexpr (VX == 1) __builtin_amdgcn_raw_buffer_store_b16(b[0], r, off, 0, 0);
+            else __builtin_amdgcn_raw_buffer_store_b32((uint32_t)b[0] | ((uint32_t)b[1] << 16), r, off, 0, 0);
+        } else if constexpr (sizeof(S) == 4) {
+            if constexpr (VX == 1) {
+                __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(in[0]), r, off, 0, 0);
+            } else {
+                u32x2 w;
+                w[0] = __float_as_uint(in[0]);
+                w[1] = __float_as_uint(in[1]);
+                __builtin_amdgcn_raw_buffer_store_b64(w, r, off, 0, 0);
+            }
+        } else {
+            if constexpr (VX == 1) {
+                u32x2 w;
+                w[0] = (unsigned)__double2loint(in[0]);
+                w[1] = (unsigned)__double2hiint(in[0]);
+                __builtin_amdgcn_raw_buffer_store_b64(w, r, off, 0, 0);
+            } else {
+                u32x4 w;
+                w[0] = (unsigned)__double2loint(in[0]);
+                w[1] = (unsigned)__double2hiint(in[0]);
+                w[2] = (unsigned)__double2loint(in[1]);
+                w[3] = (unsigned)__double2hiint(in[1]);
+                __builtin_amdgcn_raw_buffer_store_b128(w, r, off, 0, 0);
+            }
         }
     }
 };
@@ -117,6 +188,7 @@ __global__ __launch_bounds__(TXT* TY) void k_fdtd3d(const S* __restrict__ u, S* 
     tile_of(tbx, tby, tbz);
     const int W = nx + 2 * R, H = ny + 2 * R;
     const long long szl = (long long)W * H;
+    const int plane_bytes = (int)(szl * (long long)sizeof(S));
     const int tx = threadIdx.x, ty = threadIdx.y;
     const int tid = ty * TXT + tx;
     const int bx = tbx * PX + R, by = tby * PY + R;  // padded origin of the tile
@@ -124,25 +196,27 @@ __global__ __launch_bounds__(TXT* TY) void k_fdtd3d(const S* __restrict__ u, S* 
     const int z1 = min(z0 + ZC, nz + R);
     const C cc[5] = {c0, c1, c2, c3, c4};
 
-    // own point group: x = gx .. gx+VX-1, rows y = gy0 + r; in-plane 32-bit offsets
+    // own point group: x = gx .. gx+VX-1, rows y = gy0 + r
     const int gx = bx + tx * VX;
     const bool xload = gx < W, xstore = gx < nx + R;
     const int gy0 = by + ty * RY;
     const int own0 = gx + W * gy0;
 
-    // halo slots of this thread: LDS position and in-plane offset (fixed for the march)
-    int hoff[HPT], hlds[HPT];
-    bool hval[HPT], huse[HPT];
+    // halo slots of this thread: LDS position and in-plane byte offset (fixed for the
+    // march; kOut outside the array, and for slots past the halo, whose load then reads
+    // 0 into a dummy LDS cell)
+    uint32_t hb[HPT];
+    int hlds[HPT];
 #pragma unroll
     for (int s = 0; s < HPT; s++) {
         const int p = tid + s * NT;
         int lx = 0, ly = 0;
-        huse[s] = p < NHALO;
+        const bool huse = p < NHALO;
         if (p < 2 * R * PX) {
             const int hy = p / PX;
             lx = p - hy * PX + R;
             ly = hy < R ? hy : PY + hy;
-        } else if (huse[s]) {
+        } else if (huse) {
             // side halos row by row: 8 consecutive slots are the R cells left and the R
             // cells right of one row, so a wave touches few cache lines
             const int p2 = p - 2 * R * PX;
@@ -151,18 +225,17 @@ __global__ __launch_bounds__(TXT* TY) void k_fdtd3d(const S* __restrict__ u, S* 
             lx = hx < R ? hx : PX + hx;
         }
         const int xg = bx - R + lx, yg = by - R + ly;
-        hval[s] = huse[s] && xg < W && yg < H;
-        hoff[s] = hval[s] ? xg + W * yg : 0;  // an in-bounds address either way
-        hlds[s] = huse[s] ? ly * LX + lx : LY * LX + tx;
+        hb[s] = huse && xg < W && yg < H ? (uint32_t)(xg + W * yg) * (uint32_t)sizeof(S) : kOut;
+        hlds[s] = huse ? ly * LX + lx : LY * LX + tx;
     }
-    // own rows: in-bounds (clamped) offsets and a validity flag, so that the loads need
-    // no branch (an invalid row loads a valid cell and is replaced by 0)
-    int qoff[RY];
-    bool qval[RY];
+    // own rows: byte offsets of the loads (rows and columns of the array; outside them
+    // the queue takes 0) and of the stores (interior only)
+    uint32_t qb[RY], sb[RY];
 #pragma unroll
     for (int r = 0; r < RY; r++) {
-        qval[r] = xload && gy0 + r < H;
-        qoff[r] = qval[r] ? own0 + r * W : 0;
+        const uint32_t off = (uint32_t)(own0 + r * W) * (uint32_t)sizeof(S);
+        qb[r] = xload && gy0 + r < H ? off : kOut;
+        sb[r] = xstore && gy0 + r < ny + R ? off : kOut;
     }
 
     // register queues.  U (the unroll of the march) a multiple of 9: plane p sits in slot
@@ -172,7 +245,6 @@ __global__ __launch_bounds__(TXT* TY) void k_fdtd3d(const S* __restrict__ u, S* 
     // fewer live registers).
     constexpr int QN = 2 * R + 1;
     constexpr int UNROLL = U;
-    constexpr bool BRANCHFREE = sizeof(S) == 2;
     constexpr bool ROTATE = U % QN == 0;
     static_assert(U % D == 0 && U % 2 == 0, "the unroll must be a multiple of the prefetch depth and of 2");
     C q[RY][VX][QN];
@@ -181,9 +253,9 @@ __global__ __launch_bounds__(TXT* TY) void k_fdtd3d(const S* __restrict__ u, S* 
 #pragma unroll
         for (int m = 0; m < 2 * R; m++) {
             C t[VX];
-            Vec<S, VX>::load(u + (z0 - R + m) * szl, qoff[r], t);
+            Vec<S, VX>::load(plane_rsrc(u + (z0 - R + m) * szl, plane_bytes), qb[r], t);
 #pragma unroll
-            for (int e = 0; e < VX; e++) q[r][e][ROTATE ? m : m + 1] = qval[r] ? t[e] : (C)0;  // shift: before its first shift
+            for (int e = 0; e < VX; e++) q[r][e][ROTATE ? m : m + 1] = t[e];  // shift: before its first shift
         }
     }
     // prefetch ring: slot d holds the queue entry (plane z+R) and halo (plane z) of the
@@ -191,33 +263,12 @@ __global__ __launch_bounds__(TXT* TY) void k_fdtd3d(const S* __restrict__ u, S* 
     C qp[D][RY][VX];
     C hp[D][HPT];
     auto prefetch = [&](int z, C(&qd)[RY][VX], C(&hd)[HPT]) {
-        const S* pq = u + (long long)(z + R) * szl;
-        const S* ph = u + (long long)z * szl;
-        // 16-bit storage: unconditional loads of in-bounds addresses and selects measure
-        // faster; 32/64-bit: predicated loads (exec-masked branches) measure faster
+        const auto rq = plane_rsrc(u + (long long)(z + R) * szl, plane_bytes);
+        const auto rh = plane_rsrc(u + (long long)z * szl, plane_bytes);
 #pragma unroll
-        for (int r = 0; r < RY; r++) {
-            if constexpr (BRANCHFREE) {
-                C t[VX];
-                Vec<S, VX>::load(pq, qoff[r], t);
+        for (int r = 0; r < RY; r++) Vec<S, VX>::load(rq, qb[r], qd[r]);
 #pragma unroll
-                for (int e = 0; e < VX; e++) qd[r][e] = qval[r] ? t[e] : (C)0;
-            } else {
-                if (qval[r]) Vec<S, VX>::load(pq, qoff[r], qd[r]);
-                else
-#pragma unroll
-                    for (int e = 0; e < VX; e++) qd[r][e] = (C)0;
-            }
-        }
-#pragma unroll
-        for (int s = 0; s < HPT; s++) {
-            if constexpr (BRANCHFREE) {
-                const C t = Cvt<S>::ld(ph[hoff[s]]);
-                hd[s] = hval[s] ? t : (C)0;
-            } else {
-                hd[s] = hval[s] ? Cvt<S>::ld(ph[hoff[s]]) : (C)0;
-            }
-        }
+        for (int s = 0; s < HPT; s++) hd[s] = bload<S>(rh, hb[s]);
     };
 #pragma unroll
     for (int d = 0; d < D; d++)
@@ -252,14 +303,11 @@ __global__ __launch_bounds__(TXT* TY) void k_fdtd3d(const S* __restrict__ u, S* 
 #pragma unroll
                 for (int e = 0; e < VX; e++) buf[(ty * RY + r + R) * LX + tx * VX + e + R] = q[r][e][QS(R)];
 #pragma unroll
-            for (int s = 0; s < HPT; s++) {
-                if constexpr (BRANCHFREE) buf[hlds[s]] = hp[d][s];  // unused slots hit a dummy cell
-                else if (huse[s]) buf[hlds[s]] = hp[d][s];
-            }
+            for (int s = 0; s < HPT; s++) buf[hlds[s]] = hp[d][s];  // unused slots hit a dummy cell
             // refill this slot with step z + D
             if (z + D < z1) prefetch(z + D, qp[d], hp[d]);
             __syncthreads();
-            S* vz = v + (long long)z * szl;
+            const auto rv = plane_rsrc(v + (long long)z * szl, plane_bytes);
 #pragma unroll
             for (int r = 0; r < RY; r++) {
                 const int ly = ty * RY + r + R;
@@ -290,7 +338,7 @@ __global__ __launch_bounds__(TXT* TY) void k_fdtd3d(const S* __restrict__ u, S* 
                     }
                     out[e] = acc;
                 }
-                if (xstore && gy0 + r < ny + R) Vec<S, VX>::store(vz, own0 + r * W, out);
+                Vec<S, VX>::store(rv, sb[r], out);
             }
 #undef QS
         }

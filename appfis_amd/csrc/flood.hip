@@ -873,35 +873,43 @@ int run_flood(appfis_ctx* ctx, int nscen, const int* nx, const int* ny, void* co
     };
     int done = 0;
     if (graph && steps >= 2 * CHUNK) {
-        APPFIS_HIP_CHECK(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
-        int cst = APPFIS_OK;
-        for (int k = 0; k < CHUNK && cst == APPFIS_OK; k++) cst = step_pair(k & 1);
-        hipGraph_t g = nullptr;
-        hipError_t e = hipStreamEndCapture(ctx->stream, &g);
-        if (cst != APPFIS_OK) return cst;
-        if (e != hipSuccess) return hip_fail(e, "flood graph capture", __FILE__, __LINE__);
+        // CHUNK steps as one graph, captured on the first call with these arguments and
+        // replayed by later ones (the kernels read the scenarios from the table)
+        const std::vector<uint64_t> key = {2,
+                                           sizeof(R),
+                                           key_ptr(tab),
+                                           (uint64_t)nscen,
+                                           (uint64_t)nstrips,
+                                           (uint64_t)(nsrc > 0),
+                                           key_bits(P.dx),
+                                           key_bits(P.cn),
+                                           key_bits(P.hextra),
+                                           key_bits(P.dt_fallback),
+                                           key_bits(P.print_interval),
+                                           key_bits(P.rain),
+                                           (uint64_t)P.nsteps_rec,
+                                           (uint64_t)P.fixed_dt,
+                                           key_bits(P.end_time)};
         hipGraphExec_t ge = nullptr;
-        e = hipGraphInstantiate(&ge, g, nullptr, nullptr, 0);
-        hipError_t e2 = hipGraphDestroy(g);
-        if (e != hipSuccess) return hip_fail(e, "flood graph instantiate", __FILE__, __LINE__);
-        if (e2 != hipSuccess) return hip_fail(e2, "flood graph destroy", __FILE__, __LINE__);
+        int cst = cached_graph(ctx, key,
+                               [&]() {
+                                   int s2 = APPFIS_OK;
+                                   for (int k = 0; k < CHUNK && s2 == APPFIS_OK; k++) s2 = step_pair(k & 1);
+                                   return s2;
+                               },
+                               &ge);
+        if (cst) return cst;
         const int reps = steps / CHUNK;
         bool stop = false;
         for (int r = 0; r < reps && !stop; r++) {
-            e = hipGraphLaunch(ge, ctx->stream);
-            int rst = e == hipSuccess ? APPFIS_OK : hip_fail(e, "flood graph launch", __FILE__, __LINE__);
-            if (rst == APPFIS_OK && until && (r + 1) * CHUNK % CHECK_EVERY == 0) {
-                rst = read_states();
-                stop = rst == APPFIS_OK && all_finished();
-            }
-            if (rst != APPFIS_OK) {
-                (void)hipGraphExecDestroy(ge);
-                return rst;
+            APPFIS_HIP_CHECK(hipGraphLaunch(ge, ctx->stream));
+            if (until && (r + 1) * CHUNK % CHECK_EVERY == 0) {
+                cst = read_states();
+                if (cst) return cst;
+                stop = all_finished();
             }
             done = (r + 1) * CHUNK;
         }
-        APPFIS_HIP_CHECK(hipStreamSynchronize(ctx->stream));
-        APPFIS_HIP_CHECK(hipGraphExecDestroy(ge));
         if (stop) done = steps;
     }
     for (int k = done; k < steps; k++) {

@@ -484,25 +484,25 @@ int run_ensemble(appfis_ctx* ctx, ens_kernel_t<T> kern, int ngrid, const int* nx
     std::vector<int> counts((size_t)iters);
     const int n = plan_launches(iters, TS, counts.data(), iters);
     if (n <= 0) return invalid("heat2d: cannot split %d sweeps", iters);
-    // the whole launch chain is captured once and replayed as one graph
-    APPFIS_HIP_CHECK(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
-    for (int l = 0; l < n; l++)
-        hipLaunchKernelGGL(kern, dim3((unsigned)tiles), dim3(NT), 0, ctx->stream, tab, ngrid, l & 1, (T)c, counts[l]);
-    hipGraph_t graph = nullptr;
-    hipError_t e = hipStreamEndCapture(ctx->stream, &graph);
-    if (e != hipSuccess) return hip_fail(e, "heat2d ensemble capture", __FILE__, __LINE__);
+    // the whole launch chain is one graph, captured on the first call with these
+    // arguments and replayed by later ones (it reads the grids from the table)
+    const std::vector<uint64_t> key = {1, key_ptr((const void*)kern), key_ptr(tab), (uint64_t)ngrid,
+                                       (uint64_t)tiles, key_bits(c), (uint64_t)iters};
     hipGraphExec_t exec = nullptr;
-    e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
-    hipError_t e2 = hipGraphDestroy(graph);
-    if (e != hipSuccess) return hip_fail(e, "heat2d ensemble instantiate", __FILE__, __LINE__);
-    if (e2 != hipSuccess) return hip_fail(e2, "heat2d ensemble graph destroy", __FILE__, __LINE__);
-    e = hipGraphLaunch(exec, ctx->stream);
+    st = cached_graph(ctx, key,
+                      [&]() {
+                          for (int l = 0; l < n; l++) {
+                              hipLaunchKernelGGL(kern, dim3((unsigned)tiles), dim3(NT), 0, ctx->stream, tab, ngrid,
+                                                 l & 1, (T)c, counts[l]);
+                              APPFIS_LAUNCH_CHECK();
+                          }
+                          return APPFIS_OK;
+                      },
+                      &exec);
+    if (st) return st;
+    APPFIS_HIP_CHECK(hipGraphLaunch(exec, ctx->stream));
     // the table stays in use until the graph ran: wait before the workspace can be reused
-    hipError_t e3 = hipStreamSynchronize(ctx->stream);
-    hipError_t e4 = hipGraphExecDestroy(exec);
-    if (e != hipSuccess) return hip_fail(e, "heat2d ensemble launch", __FILE__, __LINE__);
-    if (e3 != hipSuccess) return hip_fail(e3, "heat2d ensemble sync", __FILE__, __LINE__);
-    if (e4 != hipSuccess) return hip_fail(e4, "heat2d ensemble exec destroy", __FILE__, __LINE__);
+    APPFIS_HIP_CHECK(hipStreamSynchronize(ctx->stream));
     return APPFIS_OK;
 }
 

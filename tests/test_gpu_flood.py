@@ -96,12 +96,17 @@ def test_flood_graph_form_matches_direct_launches(ctx):
 
 
 def test_flood_graph_replays_repeatedly(ctx):
-    # 5 replays of the captured 20-step chunk per call, three calls in a row with eager
+    # 5 replays of the captured 20-step chunk per call, four calls in a row with eager
     # work and synchronisation between them: every per-step record of every call must
-    # equal the direct-launch run
-    arrs = scenario(64, 3211, wet=0.2)
-    ref, tr, vr = gpu_run(ctx, arrs, 100, hip.PLAIN)
-    for call in range(3):
+    # equal the direct-launch run.  The chunk is captured once and cached: later calls
+    # with the same shape and parameters replay it on their own (new) buffers, the last
+    # one on another scenario of the same size
+    runs = [scenario(64, 3211, wet=0.2)] * 3 + [scenario(64, 3212, wet=0.4)]
+    refs = {}
+    for call, arrs in enumerate(runs):
+        if id(arrs) not in refs:
+            refs[id(arrs)] = gpu_run(ctx, arrs, 100, hip.PLAIN)
+        ref, tr, vr = refs[id(arrs)]
         scratch = ctx.array((1000,), hip.F64)
         ctx.fill_value(scratch, float(call))
         ctx.sync()
@@ -110,6 +115,8 @@ def test_flood_graph_replays_repeatedly(ctx):
             assert np.array_equal(x, y), call
         assert np.array_equal(tr, tg) and np.array_equal(vr, vg), call
         scratch.free()
+    a, b = (refs[id(r)][0][0] for r in (runs[0], runs[3]))
+    assert not np.array_equal(a, b)  # the two scenarios differ
 
 
 def test_flood_print_interval_clamp(ctx):

@@ -100,6 +100,47 @@ def test_heat2d_ensemble_repeated_calls(ctx):
             ctx.reduce_sum_wide(d0, [1, nx + 1, 1, ny + 1])
 
 
+def test_heat2d_ensemble_graph_reuse():
+    # the launch chain is captured on the first call and replayed by later calls with
+    # the same shapes and sweep count: a second set of grids (new buffers, new values)
+    # must be advanced, not the first; after the context workspace is reallocated (a
+    # flood run in between, on a fresh context whose workspace only held the heat
+    # table) the chain is captured again
+    sizes = [(40, 33), (129, 70), (7, 200)]
+    iters = 16
+    keep = []
+
+    def one_set(c, seed):
+        grids, arrs = [], []
+        for k, (nx, ny) in enumerate(sizes):
+            a, d0, d1 = _heat_pair(c, nx, ny, hip.F64, seed=seed + k)
+            grids.append((d0, d1, nx, ny))
+            arrs.append(a)
+            keep.extend((d0, d1))
+        return grids, arrs
+
+    def check(c, grids, arrs, tag):
+        c.heat2d_ensemble(grids, 0.1, iters)
+        for (d0, d1, nx, ny), a in zip(grids, arrs):
+            ref = oracle.heat2d(a.copy(), np.zeros_like(a), nx, ny, 0.1, iters)
+            assert np.array_equal((d0, d1)[iters % 2].download(), ref), (tag, nx, ny)
+
+    with hip.Context(0) as c:
+        check(c, *one_set(c, 2200), "first")
+        check(c, *one_set(c, 2300), "same shapes, new buffers")
+        # grow the workspace: a flood run keeps three field copies of its grid there
+        n = 300
+        z = oracle.flood_dem(n, n, 10.0, 2400)
+        zero = np.zeros_like(z)
+        d = [c.from_numpy(x, hip.F64) for x in (zero, zero, zero, z, np.full_like(z, 0.035 ** 2))]
+        keep.extend(d)
+        c.flood_run(d[0], d[1], d[2], d[3], d[4], n, n, hip.FloodParams(10.0, 0.5, 1e-3, 1.0, 0.0, 1e-5, 0.0), 2,
+                    hip.PLAIN)
+        check(c, *one_set(c, 2500), "after the workspace grew")
+        for x in keep:
+            x.free()
+
+
 def test_heat2d_full_config(ctx):
     # 1024^2 FP64, 500 sweeps: the whole config against the oracle
     nx = ny = 1024
