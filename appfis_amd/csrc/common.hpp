@@ -47,6 +47,11 @@ struct appfis_ctx {
         hipGraphExec_t exec;
     };
     std::vector<graph_entry> graphs;
+    // appfis_kernel_timing: events around the generation kernel's launches of the most
+    // recent appfis_gol call, with their launch and generation counts
+    bool ktime = false;
+    hipEvent_t kt0 = nullptr, kt1 = nullptr;
+    int kt_launches = -1, kt_units = 0;
 };
 
 namespace appfis {
@@ -74,6 +79,11 @@ int cached_graph(appfis_ctx* ctx, std::vector<uint64_t> key, const std::function
                  hipGraphExec_t* exec);
 // Destroy every cached graph (the stream must have drained).
 int clear_graphs(appfis_ctx* ctx);
+
+// appfis_kernel_timing: the events before the first and after the last timed launch of
+// a call (no-ops unless enabled, and while the stream is being captured).
+int ktime_begin(appfis_ctx* ctx);
+int ktime_end(appfis_ctx* ctx, int launches, int units);
 
 inline uint64_t key_bits(double x)
 {

@@ -98,6 +98,28 @@ int ensure_pinned(appfis_ctx* ctx, size_t bytes)
     return APPFIS_OK;
 }
 
+static bool capturing(appfis_ctx* ctx)
+{
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    return hipStreamIsCapturing(ctx->stream, &cs) == hipSuccess && cs != hipStreamCaptureStatusNone;
+}
+
+int ktime_begin(appfis_ctx* ctx)
+{
+    if (!ctx->ktime || capturing(ctx)) return APPFIS_OK;
+    APPFIS_HIP_CHECK(hipEventRecord(ctx->kt0, ctx->stream));
+    return APPFIS_OK;
+}
+
+int ktime_end(appfis_ctx* ctx, int launches, int units)
+{
+    if (!ctx->ktime || capturing(ctx)) return APPFIS_OK;
+    APPFIS_HIP_CHECK(hipEventRecord(ctx->kt1, ctx->stream));
+    ctx->kt_launches = launches;
+    ctx->kt_units = units;
+    return APPFIS_OK;
+}
+
 constexpr size_t kMaxGraphs = 8;
 
 int clear_graphs(appfis_ctx* ctx)
@@ -316,6 +338,8 @@ int appfis_ctx_destroy(appfis_ctx* ctx)
     if (ctx->pinned) note(hipHostFree(ctx->pinned));
     if (ctx->ev_start) note(hipEventDestroy(ctx->ev_start));
     if (ctx->ev_stop) note(hipEventDestroy(ctx->ev_stop));
+    if (ctx->kt0) note(hipEventDestroy(ctx->kt0));
+    if (ctx->kt1) note(hipEventDestroy(ctx->kt1));
     if (ctx->stream) note(hipStreamDestroy(ctx->stream));
     delete ctx;
     if (first != hipSuccess) return hip_fail(first, "context teardown", __FILE__, __LINE__);
@@ -438,6 +462,33 @@ int appfis_timer_stop(appfis_ctx* ctx, float* ms)
     APPFIS_HIP_CHECK(hipEventRecord(ctx->ev_stop, ctx->stream));
     APPFIS_HIP_CHECK(hipEventSynchronize(ctx->ev_stop));
     APPFIS_HIP_CHECK(hipEventElapsedTime(ms, ctx->ev_start, ctx->ev_stop));
+    return APPFIS_OK;
+}
+
+int appfis_kernel_timing(appfis_ctx* ctx, int on)
+{
+    APPFIS_REQUIRE(ctx, "ctx is NULL");
+    if (on && !ctx->kt0) {
+        APPFIS_NOT_CAPTURING(ctx, "appfis_kernel_timing");
+        APPFIS_HIP_CHECK(hipEventCreate(&ctx->kt0));
+        APPFIS_HIP_CHECK(hipEventCreate(&ctx->kt1));
+    }
+    ctx->ktime = on != 0;
+    ctx->kt_launches = -1;
+    return APPFIS_OK;
+}
+
+int appfis_kernel_time(appfis_ctx* ctx, double* ms, int* launches, int* generations)
+{
+    APPFIS_REQUIRE(ctx && ms && launches && generations, "NULL argument");
+    APPFIS_REQUIRE(ctx->kt_launches >= 0, "appfis_kernel_time: no call was timed since appfis_kernel_timing");
+    APPFIS_NOT_CAPTURING(ctx, "appfis_kernel_time");
+    float t = 0.0f;
+    APPFIS_HIP_CHECK(hipEventSynchronize(ctx->kt1));
+    APPFIS_HIP_CHECK(hipEventElapsedTime(&t, ctx->kt0, ctx->kt1));
+    *ms = t;
+    *launches = ctx->kt_launches;
+    *generations = ctx->kt_units;
     return APPFIS_OK;
 }
 

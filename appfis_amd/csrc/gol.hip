@@ -248,10 +248,12 @@ extern "C" int appfis_gol(appfis_ctx* ctx, uint8_t* buf0, uint8_t* buf1, int nx,
     uint8_t* buf[2] = {buf0, buf1};
     const unsigned blocks = ceil_div((size_t)nchunks, 256);
     const gol::RowDiv rd = gol::make_rowdiv(W);
+    st = ktime_begin(ctx);
+    if (st) return st;
     for (int g = 0; g < gens; g++) {
         hipLaunchKernelGGL(k, dim3(blocks), dim3(256), 0, ctx->stream, buf[g & 1], buf[(g + 1) & 1], W, H, nchunks,
                            nbytes, rd);
         APPFIS_LAUNCH_CHECK();
     }
-    return APPFIS_OK;
+    return ktime_end(ctx, gens, gens);
 }

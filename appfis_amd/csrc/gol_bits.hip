@@ -632,8 +632,17 @@ int gol_bits_run(appfis_ctx* ctx, uint8_t* b0, uint8_t* b1, int nx, int ny, int 
     hipLaunchKernelGGL(golbits::k_pack, dim3(pblocks), dim3(256), 0, ctx->stream, b0, P[0], nx, ny, NW);
     APPFIS_LAUNCH_CHECK();
     int cur = 0;
-    if (pick >= 0 && gens > 1) st = run_tb(ctx, golbits::kTb[pick], P, &cur, NW, ny, gens - 1);
-    else st = run_step(ctx, P, &cur, NW, ny, gens - 1);
+    st = ktime_begin(ctx);
+    if (st) return st;
+    int launches = gens - 1;
+    if (pick >= 0 && gens > 1) {
+        st = run_tb(ctx, golbits::kTb[pick], P, &cur, NW, ny, gens - 1);
+        launches = (gens - 1 + golbits::kTb[pick].g - 1) / golbits::kTb[pick].g;
+    } else {
+        st = run_step(ctx, P, &cur, NW, ny, gens - 1);
+    }
+    if (st) return st;
+    st = ktime_end(ctx, launches, gens - 1);
     if (st) return st;
     // generation gens-1 is in P[cur]: the last generation runs fused with the unpacking
     // of generations gens (into buf[gens % 2]) and gens-1 (the other buffer), rings

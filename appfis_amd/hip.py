@@ -129,6 +129,8 @@ def lib():
         "appfis_reduce_device_noalloc": ([vp, vp, i, i, ip, ip, i, vp], i),
         "appfis_heat2d": ([vp, i, vp, vp, i, i, d, i, i, i], i),
         "appfis_gol": ([vp, vp, vp, i, i, i, i, i], i),
+        "appfis_kernel_timing": ([vp, i], i),
+        "appfis_kernel_time": ([vp, dp, ip, ip], i),
         "appfis_fdtd3d": ([vp, i, vp, vp, i, i, i, dp, i, i], i),
         "appfis_fdtd3d_candidates": ([i, C.c_char_p, i], i),
         "appfis_heat2d_candidates": ([i, C.c_char_p, i], i),
@@ -272,6 +274,16 @@ class Context:
         return DeviceArray(self, a.shape, dt).upload(a)
 
     # ---- timing / graphs -------------------------------------------------------
+    def kernel_timing(self, on: bool):
+        """Record events around the generation kernel's launches of later gol calls."""
+        check(lib().appfis_kernel_timing(self.handle, 1 if on else 0), "kernel_timing")
+
+    def kernel_time(self):
+        """(ms, launches, generations) of the generation kernel in the last timed gol call."""
+        ms, n, g = C.c_double(), C.c_int(), C.c_int()
+        check(lib().appfis_kernel_time(self.handle, C.byref(ms), C.byref(n), C.byref(g)), "kernel_time")
+        return ms.value, n.value, g.value
+
     def timer_start(self):
         check(lib().appfis_timer_start(self.handle), "timer_start")
 

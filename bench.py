@@ -15,7 +15,9 @@ plain FP64 one scenario at a time and in the batched FP32 form), each with the
 roofline of its dominant kernel; `--headline-only` skips them.
 
 Roofline: achieved = algorithmic bytes per launch of the dominant kernel / its average
-launch time (timed region / launches); peak = 8 TB/s HBM.  `traffic` is the HBM bytes
+launch time (for GameOfLife: HIP events that the library records around the generation
+kernel's launches inside the timed call, over the launch count; for the suite: timed
+region / launches); peak = 8 TB/s HBM.  `traffic` is the HBM bytes
 per launch measured with rocprofv3 --pmc (FETCH_SIZE + WRITE_SIZE, calibrated on a copy
 kernel), read from profiles/traffic.json when present.  `valu` is the vector-ALU issue
 rate: the wave instructions rocprofv3 counted per dispatch (SQ_INSTS_VALU, same file)
@@ -108,47 +110,55 @@ def timed(ctx, fn):
 # GameOfLife (headline)
 # ------------------------------------------------------------------------------------
 def bench_gol(ctx, hip, steps, warmup, variant, nx=8192, ny=8192):
-    """One call of `steps` generations, timed.  The per-launch time of the generation
-    kernel is the difference quotient against a call of steps // 10 generations, which
-    removes the per-call work (the PLAIN form's initial ring exchange; the CHOSEN form's
-    pack, two unpacks and exchanges)."""
+    """One call of `steps` generations, timed.  Inside that call the library records HIP
+    events on its stream before the first and after the last launch of the generation
+    kernel (appfis_kernel_timing): their interval over the launch count is the kernel's
+    average launch duration, without the call's other work (the PLAIN form's initial ring
+    exchange; the CHOSEN form's pack and its last generation fused with the unpacking of
+    both buffers)."""
     d0 = ctx.array((ny + 2, nx + 2), hip.U8)
     d1 = ctx.array((ny + 2, nx + 2), hip.U8)
     ctx.fill_bernoulli(d0, 1, 1002, 0.25)
     if warmup:
         ctx.gol(d0, d1, nx, ny, warmup, variant)
-    short = max(1, steps // 10)
     ctx.fill_bernoulli(d0, 1, 1002, 0.25)
-    ms_short, _ = timed(ctx, lambda: ctx.gol(d0, d1, nx, ny, short, variant))
-    ctx.fill_bernoulli(d0, 1, 1002, 0.25)
+    ctx.kernel_timing(True)
     ms, wall = timed(ctx, lambda: ctx.gol(d0, d1, nx, ny, steps, variant))
+    k_ms, k_launches, k_gens = ctx.kernel_time()
+    ctx.kernel_timing(False)
     final = (d0, d1)[steps % 2]
     live = int(ctx.reduce_sum_wide(final, [1, nx + 1, 1, ny + 1]))
-    launch_ms = (ms - ms_short) / (steps - short) if steps > short else ms / steps
     res = {"form": "plain" if variant == hip.PLAIN else "chosen", "ms": round(ms, 4), "wall_s": round(wall, 4),
-           "glups": round(nx * ny * steps / (ms * 1e-3) / 1e9, 2), "live_count": live}
+           "glups": round(nx * ny * steps / (ms * 1e-3) / 1e9, 2), "live_count": live,
+           "generation_kernel": {"ms": round(k_ms, 4), "launches": k_launches, "generations": k_gens}}
+    if k_launches == 0:  # a one-generation call runs no separate generation kernel
+        return res
+    launch_ms = k_ms / k_launches
+    gens_per_launch = k_gens / k_launches
     if variant == hip.PLAIN:
         # uint8 field read + written once per generation
         res["roofline"] = roofline(2.0 * nx * ny, launch_ms, f"k_gol_plain<{(nx + 2) & 15}, 16>")
     else:
         # bit-packed field (1 bit per cell) read + written once per launch; the blocked
-        # kernel advances G generations per launch (its autotuned name says "gG")
+        # kernel advances up to G generations per launch (its autotuned name says "gG";
+        # the call's last launch may run fewer)
         tname = _tuned(ctx, 4, hip.U8)
-        g = int(tname.split(" g", 1)[1].split()[0]) if tname else 1
         res["tuned"] = tname
         res["autotune"] = [l for l in ctx.tune_report().splitlines() if l.startswith("autotune kernel 4 ")]
         kname = gol_kernel_name(tname)
         # Algorithmic bytes on the workload's own basis: 2 B per cell update (the uint8
-        # field read and written once per generation), G generations per launch.  The
-        # blocked kernel moves the field packed (1 bit per cell) and keeps it in the
+        # field read and written once per generation), the launches' generations each.
+        # The blocked kernel moves the field packed (1 bit per cell) and keeps it in the
         # 256 MB last-level cache across generations, so this rate can exceed the HBM
         # peak: HBM does not bind it; integer VALU issue does (roofline.valu, measured
         # from SQ_INSTS_VALU).  The packed field's own bytes per launch are reported apart.
-        rf = roofline(2.0 * nx * ny * g, launch_ms * g, kname)
+        g = gens_per_launch
+        rf = roofline(2.0 * nx * ny * g, launch_ms, kname)
         rf["bound"] = "valu"
-        rf["basis"] = ("2 B per cell update (uint8 read + write per generation) x G generations per launch; "
-                       "bound = VALU issue, whose fraction is roofline.valu.frac")
-        packed = roofline(2.0 * nx * ny / 8, launch_ms * g, kname)
+        rf["basis"] = ("2 B per cell update (uint8 read + write per generation) x the generations per launch "
+                       "(generation_kernel: generations / launches); bound = VALU issue, whose fraction is "
+                       "roofline.valu.frac")
+        packed = roofline(2.0 * nx * ny / 8, launch_ms, kname)
         rf["packed_field"] = {"alg_bytes_per_launch": packed["alg_bytes_per_launch"],
                               "achieved": packed["achieved"], "unit": "GB/s", "frac": packed["frac"],
                               "note": "the 1-bit field read + written once per launch"}
@@ -795,7 +805,7 @@ def main():
         "data": "synthetic (splitmix64 Bernoulli p=0.25, seed 1002, generated on device)",
         "config": {"workload": "GameOfLife 2D 8192x8192 uint8, periodic, B3/S23, 1 step = 1 generation",
                    "nx": 8192, "ny": 8192, "form": best["form"], "device": device},
-        "roofline": best["roofline"],
+        "roofline": best.get("roofline"),
         "cpu_baseline": cpu,
         "variants": variants,
         "suite": suite,

@@ -103,6 +103,16 @@ int appfis_memset(appfis_ctx* ctx, void* dst, int value, size_t bytes);
 /* ---- timing on the context stream (hipEvents) ---------------------------------- */
 int appfis_timer_start(appfis_ctx* ctx);
 int appfis_timer_stop(appfis_ctx* ctx, float* ms); /* waits for the stop event */
+/* Timing of a call's generation kernel (a measurement aid; no reference counterpart).
+ * With on != 0, each later appfis_gol call records a hipEvent on the context stream
+ * before the first and after the last launch of its generation kernel -- the byte
+ * kernel of the PLAIN form, the temporally blocked (or one-generation) kernel of the
+ * bit-packed form, which advances generations 1 .. gens-1 -- and appfis_kernel_time
+ * gives, for the most recent such call, the time between the two events, the number
+ * of launches and the generations they advanced (it waits for the second event).
+ * Nothing is recorded while the stream is being captured. */
+int appfis_kernel_timing(appfis_ctx* ctx, int on);
+int appfis_kernel_time(appfis_ctx* ctx, double* ms, int* launches, int* generations);
 
 /* ---- hipGraph capture of the context stream -------------------------------------- */
 int appfis_graph_begin(appfis_ctx* ctx);

@@ -177,3 +177,27 @@ def test_graph_capture_of_plain_heat2d_and_device_reduction(ctx):
             assert abs(out.download()[0] - exp) <= 1e-14 * float(np.abs(ref).sum())
     finally:
         ctx.graph_destroy(g)
+
+
+def test_kernel_timing_of_gol_calls(ctx):
+    # appfis_kernel_timing: events around the generation kernel's launches of a gol call
+    # (what bench.py's headline roofline divides by); the call's results are unchanged
+    nx, ny = 2048, 64
+    a = np.zeros((ny + 2, nx + 2), dtype=np.uint8)
+    oracle.fill_bernoulli(a, 1, 77, 0.3)
+    d0, d1 = ctx.from_numpy(a), ctx.array(a.shape, hip.U8)
+    with pytest.raises(hip.AppfisError):
+        ctx.kernel_timing(True)
+        ctx.kernel_time()  # nothing timed yet
+    ctx.gol(d0, d1, nx, ny, 5, hip.PLAIN)
+    ms, launches, gens = ctx.kernel_time()
+    assert (launches, gens) == (5, 5) and ms > 0.0
+    d0.upload(a)
+    ctx.gol(d0, d1, nx, ny, 20, hip.CHOSEN)
+    ms, launches, gens = ctx.kernel_time()
+    assert gens == 19 and 1 <= launches <= 19 and ms > 0.0
+    assert np.array_equal(d0.download(), oracle.gol_u8(a.copy(), np.zeros_like(a), nx, ny, 20))
+    ctx.kernel_timing(False)
+    ctx.gol(d0, d1, nx, ny, 2, hip.PLAIN)
+    with pytest.raises(hip.AppfisError):
+        ctx.kernel_time()  # timing is off: no call was timed
