@@ -107,6 +107,7 @@ static bool capturing(appfis_ctx* ctx)
 int ktime_begin(appfis_ctx* ctx)
 {
     if (!ctx->ktime || capturing(ctx)) return APPFIS_OK;
+    ctx->kt_launches = -1;  // until ktime_end: a call that fails in between leaves no time
     APPFIS_HIP_CHECK(hipEventRecord(ctx->kt0, ctx->stream));
     return APPFIS_OK;
 }
