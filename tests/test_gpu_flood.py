@@ -20,7 +20,7 @@ FLOOD_F64_RTOL = 1e-9
 FLOOD_F32_H_RTOL = 1e-3
 # Stated accuracy of the FP32 bulk form on the config workload (2000 steps from a dry
 # start under rain) against the FP64 oracle: relative L2 of h and of the Kahan volume.
-# The FP32 error depends on the scenario: 9e-6 on scenario 23, 1.8e-3 on scenario 4, where
+# The FP32 error depends on the scenario: 9e-6 on scenario 23, 1.7e-3 on scenario 4, where
 # water collects in a fast jet along the west wall and the jet's cells differ by up to
 # 3.5 cm.  That is the cost of FP32 storage, not of the kernel's arithmetic: the oracle's
 # own FP32 restatement (libm-accurate operations) is off by 1.6e-3, and FP64 arithmetic
