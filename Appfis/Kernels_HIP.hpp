@@ -151,6 +151,53 @@ namespace APPFIS
                                   "appfis_flood_run_src");
             return done;
         }
+
+        // One scenario of FloodEnsemble: its SINGLE grids, as Flood takes them.
+        template <typename T>
+        struct FloodScenario {
+            Grid<T>* h;
+            Grid<T>* qx;
+            Grid<T>* qy;
+            Grid<T>* z;
+            Grid<T>* n2;
+        };
+
+        // Many flood scenarios of any sizes advanced together: every phase of a step is
+        // one launch over a strip work-list spanning all scenarios, each scenario with its
+        // own dt, and the steps run as replayed 20-step hipGraph chunks.  Each scenario's
+        // values are the ones Flood gives it alone (no point sources).  Runs `steps`
+        // steps, or fewer for a scenario whose simulated time reaches params.end_time
+        // (when > 0) first.
+        template <typename T>
+        void FloodEnsemble(std::vector<FloodScenario<T>>& scen, const appfis_flood_params& params, int steps)
+        {
+            std::vector<int> nx, ny;
+            std::vector<void*> h, qx, qy;
+            std::vector<const void*> z, n2;
+            for (FloodScenario<T>& s : scen) {
+                Grid<T>* gs[5] = {s.h, s.qx, s.qy, s.z, s.n2};
+                for (Grid<T>* g : gs)
+                    if (!g || !EXECUTION_FLAG || !g->HasDeviceMirror()) {
+                        std::cerr << "APPFIS::kernels::FloodEnsemble needs its grids inside InitializeExecution"
+                                  << std::endl;
+                        exit(EXIT_FAILURE);
+                    }
+                Array<T>* a = s.h->GetSubGridCPU();
+                nx.push_back(a->GetDim(X) - 2);
+                ny.push_back(a->GetDim(Y) - 2);
+                h.push_back(s.h->GetSubGridsGPUData());
+                qx.push_back(s.qx->GetSubGridsGPUData());
+                qy.push_back(s.qy->GetSubGridsGPUData());
+                z.push_back(s.z->GetSubGridsGPUData());
+                n2.push_back(s.n2->GetSubGridsGPUData());
+            }
+            if (scen.empty()) return;
+            APPFIS::detail::Check(appfis_flood_ensemble_run(GetContext(), APPFIS::detail::DtypeOf<T>(),
+                                                            (int)scen.size(), nx.data(), ny.data(), h.data(),
+                                                            qx.data(), qy.data(), z.data(), n2.data(), &params,
+                                                            steps, nullptr, nullptr),
+                                  "appfis_flood_ensemble_run");
+        }
     }  // namespace kernels
 }  // namespace APPFIS
 

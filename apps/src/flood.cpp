@@ -11,14 +11,19 @@
 // them (triton_appfis.hpp:99-290): DEM ghost ring 1e6, n squared with its ring copied
 // from the inner layer, a source at x, y on full-grid column ceil((x - xllcorner) /
 // cellsize) - 1 and row ceil(nrows - (y - yllcorner) / cellsize) - 1 (a later source on
-// the same cell replaces an earlier one).
-//   flood <case.cfg> [-output] [-fp32] [-plain]
+// the same cell replaces an earlier one).  With -ensemble=N (a case without point
+// sources) it also advances N copies of the case together through
+// APPFIS::kernels::FloodEnsemble and checks that every copy ends identical to the single
+// run.
+//   flood <case.cfg> [-output] [-fp32] [-plain] [-ensemble=N]
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <cstring>
 #include <fstream>
 #include <iostream>
 #include <map>
+#include <memory>
 #include <sstream>
 #include <string>
 #include <vector>
@@ -121,16 +126,39 @@ namespace
         return rows;
     }
 
+    // Triton's binary files hold doubles (header included), whatever the run's precision:
+    // an FP64 run reads them straight into the grid (Array::LoadBinaryFile reads elements
+    // of the grid's own type), an FP32 run reads doubles and rounds them.
     template <typename T>
     void LoadInto(APPFIS::Grid<T>& g, const std::string& path, bool binary, int header)
     {
         APPFIS::Array<T>* full = g.GetFullGrid();
-        if (binary) full->LoadBinaryFile(path, header);
-        else full->LoadAsciiFile(path, header);
+        if (!binary) {
+            full->LoadAsciiFile(path, header);
+        } else if (sizeof(T) == sizeof(double)) {
+            full->LoadBinaryFile(path, header);
+        } else {
+            std::ifstream in(path.c_str(), std::ios::binary);
+            if (!in.good()) Fail("cannot read " + path);
+            std::vector<double> v((size_t)full->GetSize());
+            in.seekg((std::streamoff)header * (std::streamoff)sizeof(double), std::ios::beg);
+            in.read((char*)v.data(), (std::streamsize)(v.size() * sizeof(double)));
+            if (!in) Fail("short binary file " + path);
+            std::copy(v.begin(), v.end(), full->GetData());  // rounds each value to T
+        }
     }
 
     template <typename T>
-    int Run(const Cfg& cfg, bool output, int variant)
+    bool SameFullGrid(APPFIS::Grid<T>& a, APPFIS::Grid<T>& b)
+    {
+        APPFIS::Array<T>* x = a.GetFullGrid();
+        APPFIS::Array<T>* y = b.GetFullGrid();
+        return x->GetSize() == y->GetSize() &&
+               std::memcmp(x->GetData(), y->GetData(), sizeof(T) * (size_t)x->GetSize()) == 0;
+    }
+
+    template <typename T>
+    int Run(const Cfg& cfg, bool output, int variant, int ensemble)
     {
         const bool binary = cfg.Str("input_format") == "BIN";
         const std::string dem_path = cfg.Str("dem_filename");
@@ -184,6 +212,18 @@ namespace
             }
         }
         appfis_flood_sources src{(int)cells.size(), cells.data(), (int)times.size(), times.data(), flows.data()};
+        if (ensemble > 1 && !cells.empty()) Fail("-ensemble needs a case without point sources");
+
+        // -ensemble=N: N copies of the initial state (the DEM and Manning grids are shared)
+        std::vector<std::unique_ptr<APPFIS::Grid<T>>> copies;
+        for (int k = 0; ensemble > 1 && k < ensemble; k++)
+            for (APPFIS::Grid<T>* g : {&hin, &uin, &vin}) {
+                copies.emplace_back(new APPFIS::Grid<T>(APPFIS::DIM_2D, nx, ny, APPFIS::SINGLE));
+                copies.back()->AllocateFullGrid();
+                APPFIS::Array<T>* from = g->GetFullGrid();
+                std::copy(from->GetData(), from->GetData() + from->GetSize(), copies.back()->GetFullGrid()->GetData());
+                APPFIS::Scatter(copies.back()->GetFullGrid(), copies.back()->GetSubGrid());
+            }
 
         appfis_flood_params prm{};
         prm.dx = cs;
@@ -198,13 +238,32 @@ namespace
         if (!(prm.end_time > prm.start_time)) Fail("sim_duration must exceed sim_start_time");
 
         APPFIS::InitializeExecution(dem, nin, hin, uin, vin);
+        for (auto& g : copies) APPFIS::InitializeExecution(*g);
         APPFIS::Timer timer;
         timer.Start("COMPUTE");
         const int steps = APPFIS::kernels::Flood(hin, uin, vin, dem, nin, prm, cells.empty() ? nullptr : &src,
                                                  1 << 30, variant);
         timer.Stop("COMPUTE");
+        if (!copies.empty()) {
+            std::vector<APPFIS::kernels::FloodScenario<T>> scen;
+            for (size_t k = 0; k < copies.size(); k += 3)
+                scen.push_back({copies[k].get(), copies[k + 1].get(), copies[k + 2].get(), &dem, &nin});
+            APPFIS::kernels::FloodEnsemble(scen, prm, 1 << 30);
+        }
         APPFIS::FinalizeExecution(hin, uin, vin);
+        for (auto& g : copies) APPFIS::FinalizeExecution(*g);
         for (APPFIS::Grid<T>* g : {&hin, &uin, &vin}) APPFIS::Gather(g->GetFullGrid(), g->GetSubGrid());
+        if (!copies.empty()) {
+            bool same = true;
+            for (size_t k = 0; k < copies.size(); k++) {
+                APPFIS::Gather(copies[k]->GetFullGrid(), copies[k]->GetSubGrid());
+                APPFIS::Grid<T>* single[3] = {&hin, &uin, &vin};
+                same = same && SameFullGrid(*copies[k], *single[k % 3]);
+            }
+            std::cout << "Ensemble of " << ensemble << " copies: " << (same ? "identical" : "DIFFERENT")
+                      << " to the single run" << std::endl;
+            if (!same) return 1;
+        }
         std::cout << "Number of iteration: " << steps << std::endl;
         std::cout << "Compute: " << timer.GetCustomTime("COMPUTE") << std::endl;
         std::cout << "BLUPS: " << timer.CalculateBLUPS("COMPUTE", nx * ny, steps) << std::endl;
@@ -219,17 +278,18 @@ namespace
 
 int main(int argc, char* argv[])
 {
-    if (argc < 2) Fail("usage: flood <case.cfg> [-output] [-fp32] [-plain]");
+    if (argc < 2) Fail("usage: flood <case.cfg> [-output] [-fp32] [-plain] [-ensemble=N]");
     Cfg cfg;
     cfg.kv = ReadCfg(argv[1]);
     const bool output = APPFIS::CheckArgument(argc, argv, "output");
     const bool fp32 = APPFIS::CheckArgument(argc, argv, "fp32");
     const int variant = APPFIS::CheckArgument(argc, argv, "plain") ? APPFIS_PLAIN : APPFIS_CHOSEN;
+    const int ensemble = APPFIS::CheckArgument(argc, argv, "ensemble") ? APPFIS::GetArgument(argc, argv, "ensemble") : 0;
     APPFIS::Attribute at;
     at.ghostLayer = 1;
     at.periodic = false;
     APPFIS::Initialize(argc, argv, APPFIS::DIM_2D, APPFIS::DIM_2D, at);
-    const int rc = fp32 ? Run<float>(cfg, output, variant) : Run<double>(cfg, output, variant);
+    const int rc = fp32 ? Run<float>(cfg, output, variant, ensemble) : Run<double>(cfg, output, variant, ensemble);
     APPFIS::Finalize();
     return rc;
 }
