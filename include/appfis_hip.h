@@ -27,10 +27,10 @@
  *     appfis_fdtd3d in every variant (the CHOSEN forms once an uncaptured call has tuned
  *     the shape and sized the workspace).  Everything that would allocate, free,
  *     synchronise, memset, return a host result or capture a graph of its own
- *     (appfis_malloc/free/memset/memcpy_h2d/d2h/sync/timer_stop, appfis_reduce,
- *     appfis_reduce_sum_wide, appfis_himeno, appfis_heat2d_ensemble, the flood entry
- *     points, a first CHOSEN call) returns APPFIS_ERR_UNSUPPORTED under capture and
- *     enqueues nothing.
+ *     (appfis_malloc/free/memset/memcpy_h2d/d2h/sync/timer_stop, appfis_kernel_time,
+ *     appfis_kernel_timing's first enabling call, appfis_reduce, appfis_reduce_sum_wide,
+ *     appfis_himeno, appfis_heat2d_ensemble, the flood entry points, a first CHOSEN
+ *     call) returns APPFIS_ERR_UNSUPPORTED under capture and enqueues nothing.
  */
 #ifndef APPFIS_HIP_H
 #define APPFIS_HIP_H
