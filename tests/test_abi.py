@@ -85,3 +85,20 @@ int main(void) {
     r = subprocess.run([str(exe)], capture_output=True, text=True, timeout=60)
     assert r.returncode == 0, (r.returncode, r.stdout, r.stderr)
     assert r.stdout.startswith("ok")
+
+
+def test_build_keeps_fp_contraction_off_except_the_fp32_flood_form():
+    # the parity contract: every HIP object is compiled without FMA contraction (the
+    # reference functors' rounding), except the FP32 bulk flood form, whose results are
+    # held to a tolerance against FP64 and which contracts its multiply-adds
+    out = subprocess.run(["make", "-n", "-B", "-C", os.path.join(ROOT, "appfis_amd", "csrc")], capture_output=True,
+                         text=True, check=True).stdout
+    compiles = [l for l in out.splitlines() if "hipcc" in l and " -c " in l]
+    srcs = {l.split(" -c ")[1].split()[0]: l for l in compiles}
+    assert "flood.hip" in srcs and "flood_f32.hip" in srcs and len(srcs) >= 10, sorted(srcs)
+    for src, line in srcs.items():
+        assert "-Xclang -target-feature -Xclang -packed-fp32-ops" in line, src
+        if src == "flood_f32.hip":
+            assert "-ffp-contract=fast" in line and "-ffp-contract=off" not in line, line
+        else:
+            assert "-ffp-contract=off" in line and "-ffp-contract=fast" not in line, line
