@@ -345,6 +345,192 @@ __global__ __launch_bounds__(TXT* TY) void k_fdtd3d(const S* __restrict__ u, S* 
     }
 }
 
+// The same sweep with each plane tile loaded straight into LDS (FP32 storage, one x
+// point per thread; the FP64 form of it ran 13% slower than FP64's two points per lane).
+// The block's waves copy the whole (64+8) x (TY*RY+8) tile of a plane, halo included,
+// with 16-byte buffer-to-LDS loads (`buffer_load_dwordx4 ... lds`: lane l of a load
+// writes bytes 16l .. 16l+15 of its 1 KB block, and the tile's rows are contiguous in
+// LDS, so chunk c of the tile is LDS bytes 16c .. 16c+15), A = 4 + D planes ahead of the
+// plane being computed, into a ring of NB = A + 2 plane buffers.  No registers hold the
+// tile on its way, and there is no LDS store.
+//
+// At step z a wave issues the loads of plane z+A, waits until its own loads of plane
+// z+4 have landed, then meets the others at a plain barrier (no fence, which would wait
+// for every load in flight): planes z and z+4 are then complete for every wave.  The
+// wait is an explicit vmcnt.  On CDNA the vector memory counter retires loads and
+// stores in issue order (as the compiler's own waitcnt placement for gfx9 assumes), so
+// "at most N outstanding" means all but the last N operations are done.  A step issues
+// the nload loads of one plane and then RY stores; after the loads of plane z+4 came
+// the stores of that step and A-5 whole steps, then this step's loads: N =
+// (A-4)*(nload+RY).  In the first A-4 steps of a chunk fewer stores have been issued,
+// and N = (A-4)*nload (loads only) is used, which can only wait longer.  The wave then
+// takes its own z+4 values from LDS into the register z-queue and computes plane z from
+// LDS (x, y neighbours) and the queue (z neighbours), in the operation order of
+// k_fdtd3d.
+//
+// The ring index of a plane is static in a loop unrolled NB times, so the compiler sees
+// which buffer each load fills.  Between two barriers the waves are at most one step
+// apart, and NB = A + 2 keeps the buffer a step's loads fill apart from the two that
+// the step before reads.  Chunks past the row end read the next row (the values only
+// feed points that are not stored), rows past the plane read 0 (buffer range check),
+// and the padding of a buffer's last 1 KB block takes the lanes past the tile.  Launch
+// checks: nx + 8 a multiple of 4, 16-byte aligned buffers.
+__device__ inline void load_lds16(__amdgpu_buffer_rsrc_t r, void* lds, uint32_t off)
+{
+    __builtin_amdgcn_raw_ptr_buffer_load_lds(r, (__attribute__((address_space(3))) void*)lds, 16, off, 0, 0, 0);
+}
+
+template <typename S, int TY, int RY, int ZC, int D>
+__global__ __launch_bounds__(TXT* TY) void k_fdtd3d_dma(const S* __restrict__ u, S* __restrict__ v, int nx, int ny,
+                                                         int nz, S c0, S c1, S c2, S c3, S c4)
+{
+    static_assert(sizeof(S) == 4, "FP32 storage");
+    using C = S;
+    constexpr int PY = TY * RY;
+    constexpr int LX = TXT + 2 * R;               // 72 elements per tile row
+    constexpr int LY = PY + 2 * R;
+    constexpr int EPC = 16 / (int)sizeof(S);      // elements per 16-byte chunk
+    constexpr int CPR = LX / EPC;                 // chunks per tile row
+    static_assert(CPR * EPC == LX, "a tile row is whole chunks");
+    constexpr int NCH = LY * CPR;                 // chunks per tile
+    constexpr int NI = (NCH + 63) / 64;           // 64-lane loads per tile
+    constexpr int L = (NI + TY - 1) / TY;         // loads per wave and plane, at most
+    constexpr int A = 4 + D;                      // planes loaded ahead
+    constexpr int NB = A + 2;                     // ring of plane buffers
+    constexpr int BUF = NI * 64 * EPC;            // elements per buffer (whole 1 KB blocks)
+    constexpr int QN = 2 * R + 1;
+    __shared__ __align__(16) C ring[NB][BUF];
+
+    int tbx, tby, tbz;
+    tile_of(tbx, tby, tbz);
+    const int W = nx + 2 * R, H = ny + 2 * R;
+    const long long szl = (long long)W * H;
+    const int plane_bytes = (int)(szl * (long long)sizeof(S));
+    const int tx = threadIdx.x, ty = threadIdx.y;
+    const int bx = tbx * TXT + R, by = tby * PY + R;  // padded origin of the tile
+    const int z0 = tbz * ZC + R;
+    const int z1 = min(z0 + ZC, nz + R);
+    const int zlast = z1 + R - 1;  // the last plane the chunk reads
+    const C cc[5] = {c0, c1, c2, c3, c4};
+    const int gx = bx + tx;
+    const int gy0 = by + ty * RY;
+
+    // this wave's tile loads: load i = ty + TY*j covers chunks 64i .. 64i+63
+    uint32_t lo[L];
+    int nload = 0;
+#pragma unroll
+    for (int j = 0; j < L; j++) {
+        const int i = ty + TY * j;
+        const int c = 64 * i + tx;
+        lo[j] = kOut;
+        if (i < NI) {
+            nload = j + 1;
+            if (c < NCH) {
+                const int row = c / CPR, cin = c - row * CPR;
+                lo[j] = (uint32_t)((bx - R + cin * EPC) + W * (by - R + row)) * (uint32_t)sizeof(S);
+            }
+        }
+    }
+    auto load_plane = [&](int p, int b) {  // tile of plane p into ring[b] (b static)
+        const auto rs = plane_rsrc(u + (long long)p * szl, plane_bytes);
+#pragma unroll
+        for (int j = 0; j < L; j++)
+            if (j < nload) load_lds16(rs, &ring[b][(ty + TY * j) * 64 * EPC], lo[j]);
+    };
+    // own rows: loads of the initial queue and stores (as in k_fdtd3d)
+    uint32_t qb[RY], sb[RY];
+#pragma unroll
+    for (int r = 0; r < RY; r++) {
+        const uint32_t off = (uint32_t)(gx + W * (gy0 + r)) * (uint32_t)sizeof(S);
+        qb[r] = gx < W && gy0 + r < H ? off : kOut;
+        sb[r] = gx < nx + R && gy0 + r < ny + R ? off : kOut;
+    }
+    // z-queue: slot m holds plane z + m - 4 at step z; before the first step, planes
+    // z0-4 .. z0+3 in slots 1 .. 8
+    C q[RY][QN];
+#pragma unroll
+    for (int r = 0; r < RY; r++)
+#pragma unroll
+        for (int m = 0; m < 2 * R; m++) q[r][m + 1] = bload<S>(plane_rsrc(u + (z0 - R + m) * szl, plane_bytes), qb[r]);
+#pragma unroll
+    for (int b = 0; b < A; b++)
+        if (z0 + b <= zlast) load_plane(z0 + b, b);
+
+    for (int zb = z0; zb < z1; zb += NB) {
+#pragma unroll
+        for (int k = 0; k < NB; k++) {
+            const int z = zb + k;
+            if (z >= z1) break;
+            const bool ahead = z + A <= zlast;
+            if (ahead) load_plane(z + A, (k + A) % NB);
+            // this wave's loads of plane z+4 (and of every earlier plane) have landed
+            if (nload > 0) {
+                constexpr int N1 = (A - 4) * L, N2 = (A - 4) * (L - 1);
+                constexpr int E1 = (N1 & 15) | 0x70 | 0xF00 | ((N1 >> 4) & 3) << 14;
+                constexpr int E2 = (N2 & 15) | 0x70 | 0xF00 | ((N2 >> 4) & 3) << 14;
+                static_assert(N1 < 64, "vmcnt range");
+                constexpr int M1 = (A - 4) * (L + RY), M2 = (A - 4) * (L - 1 + RY);
+                constexpr int F1 = (M1 & 15) | 0x70 | 0xF00 | ((M1 >> 4) & 3) << 14;
+                constexpr int F2 = (M2 & 15) | 0x70 | 0xF00 | ((M2 >> 4) & 3) << 14;
+                static_assert(M1 < 64, "vmcnt range");
+                if (!ahead) __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0)
+                else if (z - z0 >= A - 4) {  // steady state: loads and stores of A-4 steps
+                    if (nload == L) __builtin_amdgcn_s_waitcnt(F1);
+                    else __builtin_amdgcn_s_waitcnt(F2);
+                } else if (nload == L) __builtin_amdgcn_s_waitcnt(E1);  // loads only: fewer
+                else __builtin_amdgcn_s_waitcnt(E2);
+            }
+            __builtin_amdgcn_s_barrier();
+            const C* cur = ring[k];
+            const C* nxt4 = ring[(k + 4) % NB];
+#pragma unroll
+            for (int r = 0; r < RY; r++) {
+#pragma unroll
+                for (int m = 0; m < 2 * R; m++) q[r][m] = q[r][m + 1];
+                q[r][2 * R] = nxt4[(ty * RY + r + R) * LX + tx + R];
+            }
+            const auto rv = plane_rsrc(v + (long long)z * szl, plane_bytes);
+#pragma unroll
+            for (int r = 0; r < RY; r++) {
+                const int ly = ty * RY + r + R;
+                C xw[2 * R + 1];
+#pragma unroll
+                for (int j = 0; j < R; j++) {
+                    xw[j] = cur[ly * LX + tx + j];
+                    xw[R + 1 + j] = cur[ly * LX + tx + R + 1 + j];
+                }
+                xw[R] = q[r][R];
+                C acc = cc[0] * q[r][R];
+#pragma unroll
+                for (int kk = 1; kk <= R; kk++) {
+                    const C sx = xw[R - kk] + xw[R + kk];
+                    const C ym = (r - kk >= 0) ? q[r - kk][R] : cur[(ly - kk) * LX + tx + R];
+                    const C yp = (r + kk < RY) ? q[r + kk][R] : cur[(ly + kk) * LX + tx + R];
+                    const C sY = ym + yp;
+                    const C sZ = q[r][R - kk] + q[r][R + kk];
+                    const C sr = (sx + sY) + sZ;
+                    acc = acc + cc[kk] * sr;
+                }
+                C out[1] = {acc};
+                Vec<S, 1>::store(rv, sb[r], out);
+            }
+        }
+    }
+}
+
+template <typename S, int TY, int RY, int ZC, int D>
+int run_dma(appfis_ctx* ctx, S* b0, S* b1, int nx, int ny, int nz, const double* cf, int steps)
+{
+    dim3 block(TXT, TY), grid(ceil_div(nx, TXT), ceil_div(ny, TY * RY), ceil_div(nz, ZC));
+    S* buf[2] = {b0, b1};
+    for (int s = 0; s < steps; s++) {
+        hipLaunchKernelGGL((k_fdtd3d_dma<S, TY, RY, ZC, D>), grid, block, 0, ctx->stream, buf[s & 1],
+                           buf[(s + 1) & 1], nx, ny, nz, (S)cf[0], (S)cf[1], (S)cf[2], (S)cf[3], (S)cf[4]);
+        APPFIS_LAUNCH_CHECK();
+    }
+    return APPFIS_OK;
+}
+
 template <typename S, int VX, int TY, int RY, int ZC, int D, int U>
 int run(appfis_ctx* ctx, S* b0, S* b1, int nx, int ny, int nz, const double* cf, int steps)
 {
@@ -363,14 +549,22 @@ typedef int (*fdtd_fn)(appfis_ctx*, void*, void*, int, int, int, const double*, 
 struct FdtdVariant {
     const char* name;
     fdtd_fn fn;
-    bool pairs;  // 16-bit storage moved in 32-bit pairs: needs 4-byte aligned buffers
-    int vx;      // x points per thread: needs nx % vx == 0 (a lane stores all vx or none)
+    bool pairs;    // 16-bit storage moved in 32-bit pairs: needs 4-byte aligned buffers
+    int vx;        // x points per thread: needs nx % vx == 0 (a lane stores all vx or none)
+    int wmod = 0;  // tile rows loaded as 16-byte chunks: nx + 8 a multiple of wmod, 16-byte
+                   // aligned buffers (0: no such needs)
 };
 
 template <typename S, int VX, int TY, int RY, int ZC, int D, int U>
 int run_v(appfis_ctx* ctx, void* b0, void* b1, int nx, int ny, int nz, const double* cf, int steps)
 {
     return run<S, VX, TY, RY, ZC, D, U>(ctx, (S*)b0, (S*)b1, nx, ny, nz, cf, steps);
+}
+
+template <typename S, int TY, int RY, int ZC, int D>
+int run_dma_v(appfis_ctx* ctx, void* b0, void* b1, int nx, int ny, int nz, const double* cf, int steps)
+{
+    return run_dma<S, TY, RY, ZC, D>(ctx, (S*)b0, (S*)b1, nx, ny, nz, cf, steps);
 }
 
 // Candidate tilings (VX x-points per thread, TY thread rows, RY rows per thread, ZC
@@ -396,6 +590,11 @@ static const FdtdVariant kF32[] = {
     {"vx2 ty8 ry2 zc256 d4 u4", run_v<float, 2, 8, 2, 256, 4, 4>, false, 2},
     {"vx2 ty16 ry1 zc256 d4 u4", run_v<float, 2, 16, 1, 256, 4, 4>, false, 2},
     {"vx2 ty16 ry1 zc256 d6 u6", run_v<float, 2, 16, 1, 256, 6, 6>, false, 2},
+    // tiles loaded straight into LDS (k_fdtd3d_dma): "dma ty.. ry.. zc.. d.."
+    {"dma ty16 ry2 zc256 d2", run_dma_v<float, 16, 2, 256, 2>, false, 1, 4},
+    {"dma ty16 ry2 zc256 d3", run_dma_v<float, 16, 2, 256, 3>, false, 1, 4},
+    {"dma ty8 ry4 zc256 d2", run_dma_v<float, 8, 4, 256, 2>, false, 1, 4},
+    {"dma ty16 ry2 zc128 d2", run_dma_v<float, 16, 2, 128, 2>, false, 1, 4},
 };
 static const FdtdVariant kF64[] = {
     {"vx1 ty8 ry2 zc256 d4 u4", run_v<double, 1, 8, 2, 256, 4, 4>, false, 1},
@@ -464,10 +663,16 @@ extern "C" int appfis_fdtd3d(appfis_ctx* ctx, int dtype, void* buf0, void* buf1,
     default: return invalid("fdtd3d: dtype must be F32, F64, F16 or BF16");
     }
     const bool aligned4 = (((uintptr_t)buf0 | (uintptr_t)buf1) & 3) == 0;
-    auto fits = [&](const FdtdVariant& v) { return nx % v.vx == 0 && (!v.pairs || aligned4); };
+    const bool aligned16 = (((uintptr_t)buf0 | (uintptr_t)buf1) & 15) == 0;
+    auto fits = [&](const FdtdVariant& v) {
+        return nx % v.vx == 0 && (!v.pairs || aligned4) && (v.wmod == 0 || ((nx + 8) % v.wmod == 0 && aligned16));
+    };
     if (variant >= APPFIS_CANDIDATE0) {
         const int i = variant - APPFIS_CANDIDATE0;
         APPFIS_REQUIRE(i < ntab, "fdtd3d: candidate %d out of range (%d candidates)", i, ntab);
+        if (tab[i].wmod)
+            APPFIS_REQUIRE(fits(tab[i]), "fdtd3d: candidate %s needs nx + 8 divisible by %d and 16-byte aligned buffers",
+                           tab[i].name, tab[i].wmod);
         APPFIS_REQUIRE(fits(tab[i]), "fdtd3d: candidate %s needs nx %% %d == 0%s", tab[i].name, tab[i].vx,
                        tab[i].pairs ? " and 4-byte aligned buffers" : "");
         return steps == 0 ? APPFIS_OK : tab[i].fn(ctx, buf0, buf1, nx, ny, nz, coeff5, steps);
@@ -483,7 +688,7 @@ extern "C" int appfis_fdtd3d(appfis_ctx* ctx, int dtype, void* buf0, void* buf1,
             cand.push_back(i);
             names.push_back(tab[i].name);
         }
-    const int key[6] = {3, dtype, nx, ny, nz, aligned4 ? 1 : 0};
+    const int key[6] = {3, dtype, nx, ny, nz, aligned16 ? 2 : aligned4 ? 1 : 0};
     int pick = 0;
     const size_t bytes = (size_t)(nx + 8) * (ny + 8) * (nz + 8) * dtype_size(dtype);
     int st = APPFIS_OK;

@@ -409,9 +409,14 @@ def bench_fdtd(ctx, hip, n=512, steps=100):
             ctx.fill_uniform(a, 4, 1004)
             ms, _ = timed(ctx, lambda: ctx.fdtd3d(a, b, n, n, n, coeff, steps, variant))
             tname = FDTD_PLAIN[name] if variant == hip.PLAIN else _tuned(ctx, 3, dt)
-            vx, ty, ry, zc, dd, uu = (int(t[2:]) if t[:2] in ("vx", "ty", "ry", "zc") else int(t[1:])
-                                      for t in tname.split())
-            kname = f"k_fdtd3d<{FDTD_CTYPE[name]}, {vx}, {ty}, {ry}, {zc}, {dd}, {uu}>"
+            if tname.startswith("dma "):  # direct-to-LDS form: "dma ty16 ry2 zc256 d2"
+                ty, ry, zc, dd = (int(t[2:]) if t[:2] in ("ty", "ry", "zc") else int(t[1:])
+                                  for t in tname.split()[1:])
+                kname = f"k_fdtd3d_dma<{FDTD_CTYPE[name]}, {ty}, {ry}, {zc}, {dd}>"
+            else:
+                vx, ty, ry, zc, dd, uu = (int(t[2:]) if t[:2] in ("vx", "ty", "ry", "zc") else int(t[1:])
+                                          for t in tname.split())
+                kname = f"k_fdtd3d<{FDTD_CTYPE[name]}, {vx}, {ty}, {ry}, {zc}, {dd}, {uu}>"
             out[f"{name}_{form}"] = {"ms": round(ms, 3), "glups": round(n ** 3 * steps / (ms * 1e-3) / 1e9, 2),
                                      "tiling": tname, "roofline": roofline(float(bpl) * n ** 3, ms / steps, kname)}
             if form == "chosen":
