@@ -226,8 +226,12 @@ int appfis_gol_candidates(char* names, int len);
  * dtype F32 / F64 (arithmetic in that type) or F16 / BF16 (storage only, FP32
  * arithmetic).  Field in bufs[steps % 2] at exit.  PLAIN runs the default tiling;
  * CHOSEN runs the tiling the autotuner picked for this dtype and shape (timed once on
- * scratch buffers, cached in the context); APPFIS_CANDIDATE0 + i runs candidate i of
- * appfis_fdtd3d_candidates -- all compute identical values. */
+ * scratch buffers, cached in the context, among the candidates the shape and the
+ * buffers' alignment admit); APPFIS_CANDIDATE0 + i runs candidate i of
+ * appfis_fdtd3d_candidates -- all compute identical values.  A forced candidate that
+ * does not fit fails with APPFIS_ERR_INVALID: "vxN" needs nx % N == 0 (16-bit pairs also
+ * 4-byte aligned buffers); "dma ..." (FP32 direct-to-LDS) needs (nx + 8) % 4 == 0 and
+ * 16-byte aligned buffers. */
 int appfis_fdtd3d(appfis_ctx* ctx, int dtype, void* buf0, void* buf1, int nx, int ny, int nz,
                   const double* coeff5, int steps, int variant);
 
