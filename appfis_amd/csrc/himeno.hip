@@ -5,11 +5,13 @@
 // Layout: 14 dense arrays of dims (nx, ny, nz) including the 1-cell boundary shell.
 //
 // Kernel shape (2.5D k-march): a block owns full x rows (TX = 256 threads along x) and
-// RY rows per thread, and marches a chunk of zc planes in k (zc autotuned per shape:
-// it sets how evenly the blocks fill the chip).  p is staged through an LDS ring of
-// planes (k-1, k, k+1 with their 1-cell x/y halo), so each p value is read from memory
-// about once; the 12 coefficient arrays and wrk1 are pure coalesced streams.  Tiles
-// are ordered XCD-contiguously.  gosa: per-thread FP64 accumulation, wave64 DPP tree, one LDS stage, one
+// RY rows per thread, and marches a chunk of zc planes in k with the coefficients of
+// the next D planes in flight (zc and D autotuned per shape: zc sets how many blocks
+// share the chip and how many distinct plane regions they stream at once).  p is
+// staged through an LDS ring of planes (k-1, k, k+1 with their 1-cell x/y halo), so
+// each p value is read from memory about once; the 12 coefficient arrays and wrk1 are
+// pure coalesced streams, read with buffer loads that the compiler can count in
+// flight.  Tiles are ordered XCD-contiguously.  gosa: per-thread FP64 accumulation, wave64 DPP tree, one LDS stage, one
 // partial per block, and the last block to arrive (atomic ticket) sums the partials in
 // block order into gosa[it] -- deterministic, on the device, no host round trip.
 //
@@ -105,22 +107,64 @@ __device__ inline double block_sum_last(double v, double* partials, unsigned* ti
     return 0.0;
 }
 
+// Buffer accesses: every load and store of the sweep is a buffer instruction on a
+// resource of one plane (or, for the blocked coefficients, of the blocks from the plane's
+// first point on), at a byte offset each thread fixes once.  Cells outside the array or
+// not updated carry offset kOut = 2^31, past any resource's range, so their loads return
+// 0 and their stores are dropped: no branches around memory accesses, and the compiler
+// can count the loads in flight (a load under a lane branch makes it wait for all of
+// them before the next plane).
+constexpr uint32_t kOut = 0x80000000u;
+typedef unsigned int u32x2 __attribute__((ext_vector_type(2)));
+
+template <typename T>
+__device__ inline __amdgpu_buffer_rsrc_t rsrc(const T* base, uint32_t bytes)
+{
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<T*>(base), (short)0, (int)bytes, 0x00020000);
+}
+template <typename T>
+__device__ inline T ld(__amdgpu_buffer_rsrc_t r, uint32_t off)
+{
+    if constexpr (sizeof(T) == 4) {
+        return __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(r, off, 0, 0));
+    } else {
+        const u32x2 b = __builtin_amdgcn_raw_buffer_load_b64(r, off, 0, 0);
+        return __hiloint2double((int)b[1], (int)b[0]);
+    }
+}
+template <typename T>
+__device__ inline void st(__amdgpu_buffer_rsrc_t r, uint32_t off, T v)
+{
+    if constexpr (sizeof(T) == 4) {
+        __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(v), r, off, 0, 0);
+    } else {
+        const u32x2 b = {(unsigned)__double2loint(v), (unsigned)__double2hiint(v)};
+        __builtin_amdgcn_raw_buffer_store_b64(b, r, off, 0, 0);
+    }
+}
+
 // MODE 0 (PLAIN):  ss = (s0*a3 - p)*bnd with s0 including wrk1; pn = p + omega*ss
-// MODE 1 (inner):  p is the correction q, wrk1 slot holds r;
+// MODE 2 (inner sweep of the defect correction, FP32): p is the correction q and the
+//                  coefficients and r come from the blocked array Coef::inter;
 //                  ss = r + (s0*a3 - q)*bnd with s0 excluding wrk1; qn = q + omega*ss
 //
 // Software pipeline along k: while plane k is computed, the p values of plane k+2 (for
-// the LDS ring) and the 13 coefficient values of plane k+1 are already in flight in
-// registers.  The ring has 4 slots, so one barrier per plane separates the staging of
-// plane k+1 from its use (the slot it overwrites was last read two planes ago).
-template <typename T, int MODE>
+// the LDS ring) and the 12 coefficient values of planes k+1 .. k+D are already in flight
+// in registers (a ring of D+1 coefficient planes; the k loop is unrolled D+1 times so
+// that every slot index is static).  The p ring has 4 slots, so one barrier per plane
+// separates the staging of plane k+1 from its use (the slot it overwrites was last read
+// two planes ago).  A thread outside the interior computes with zero coefficients, so
+// its ss is 0 (it adds nothing to gosa) and its store is dropped.
+template <typename T, int MODE, int D>
 __global__ __launch_bounds__(TX) void k_himeno(const T* __restrict__ p, T* __restrict__ pn, Coef cf, int nx, int ny,
                                                int nz, int zc, T omega, double* partials, unsigned* ticket,
                                                double* gosa_out)
 {
-    constexpr int NR = 4;                    // ring slots
+    static_assert(MODE == 0 || (MODE == 2 && sizeof(T) == 4), "PLAIN in F32 / F64, inner sweep in F32");
+    constexpr int NR = 4;                         // p ring slots
     constexpr int PPT = (LW * LH + TX - 1) / TX;  // staged p values per thread
-    constexpr int NC = 13;                   // coefficient streams per point
+    constexpr int NC = 12;                        // coefficient streams per point
+    constexpr int NS = D + 1;                     // coefficient ring slots
     __shared__ T ring[NR][LH][LW];
     __shared__ double red[TX / 64];
     const int tx = threadIdx.x;
@@ -130,51 +174,63 @@ __global__ __launch_bounds__(TX) void k_himeno(const T* __restrict__ p, T* __res
     const int j0 = by * RY + 1;      // first interior row of the block
     const int k0 = bz * zc + 1;
     const int k1 = min(k0 + zc, nz - 1);
-    const int sy = nx, sz = nx * ny;         // < 2^31 elements (checked on the host)
+    const int sy = nx, sz = nx * ny;  // < 2^31 elements (checked on the host)
+    const uint32_t pbytes = (uint32_t)sz * (uint32_t)sizeof(T);  // < 2^31 (checked on the host)
     const bool xin = i < nx - 1;
     const T* arr[NC] = {(const T*)cf.a0, (const T*)cf.a1, (const T*)cf.a2, (const T*)cf.a3, (const T*)cf.b0,
                         (const T*)cf.b1, (const T*)cf.b2, (const T*)cf.c0, (const T*)cf.c1, (const T*)cf.c2,
-                        (const T*)cf.bnd, (const T*)cf.wrk1, p};
-    // staged cells of this thread: in-plane offsets (clamped in bounds) and validity
-    int soff[PPT], slds[PPT];
-    bool sval[PPT];
+                        (const T*)cf.bnd, (const T*)cf.wrk1};
+    // staged cells of this thread: in-plane byte offsets (kOut outside the array)
+    uint32_t soff[PPT];
+    int slds[PPT];
     const int xb = bx * TX;
 #pragma unroll
     for (int q = 0; q < PPT; q++) {
         const int e = tx + q * TX;
         const int ly = e / LW, lx = e - ly * LW;
         const int gx = xb + lx, gy = j0 - 1 + ly;
-        sval[q] = e < LW * LH && gx < nx && gy < ny;
-        soff[q] = sval[q] ? gx + sy * gy : 0;
-        slds[q] = e < LW * LH ? e : 0;
+        soff[q] = e < LW * LH && gx < nx && gy < ny ? (uint32_t)(gx + sy * gy) * (uint32_t)sizeof(T) : kOut;
+        slds[q] = e < LW * LH ? e : -1;
     }
     bool rin[RY];
-    int own[RY];
+    int own[RY];  // in-plane element index of the thread's points
 #pragma unroll
     for (int r = 0; r < RY; r++) {
         rin[r] = xin && j0 + r < ny - 1;
-        own[r] = rin[r] ? i + sy * (j0 + r) : 0;
+        own[r] = i + sy * (j0 + r);
     }
     T* ringf = &ring[0][0][0];
     auto stage_load = [&](int k, T(&v)[PPT]) {
+        const auto rs = rsrc(p + (long long)sz * k, pbytes);
 #pragma unroll
-        for (int q = 0; q < PPT; q++) v[q] = sval[q] ? p[soff[q] + sz * k] : (T)0;
+        for (int q = 0; q < PPT; q++) v[q] = ld<T>(rs, soff[q]);
     };
     auto stage_store = [&](int slot, const T(&v)[PPT]) {
 #pragma unroll
         for (int q = 0; q < PPT; q++)
-            if (tx + q * TX < LW * LH) ringf[slot * (LH * LW) + slds[q]] = v[q];
+            if (slds[q] >= 0) ringf[slot * (LH * LW) + slds[q]] = v[q];
     };
-    auto coef_load = [&](int k, T(&c)[RY][NC - 1]) {
+    auto coef_load = [&](int k, T(&c)[RY][NC]) {
+        if constexpr (MODE == 2) {
+            // the blocks from the one holding the plane's first point: value a of point
+            // q at ((q >> 6) - (b >> 6)) * BLK + a * 64 + (q & 63), b = the first point
+            const int b = sz * k;
+            const auto rs = rsrc((const float*)cf.inter + (size_t)(b >> 6) * BLK,
+                                 (uint32_t)((sz >> 6) + 2) * (uint32_t)(BLK * 4));
 #pragma unroll
-        for (int r = 0; r < RY; r++) {
-            if constexpr (MODE == 2) {
-                const float* ci = (const float*)cf.inter + blk_idx((size_t)(own[r] + sz * k), 0);
+            for (int r = 0; r < RY; r++) {
+                const int q = own[r] + b;
+                const uint32_t o = rin[r] ? (uint32_t)(((q >> 6) - (b >> 6)) * BLK + (q & 63)) * 4u : kOut;
 #pragma unroll
-                for (int a = 0; a < NC - 1; a++) c[r][a] = rin[r] ? ci[a * 64] : (T)0;
-            } else {
+                for (int a = 0; a < NC; a++) c[r][a] = ld<T>(rs, o + (uint32_t)(a * 256));
+            }
+        } else {
 #pragma unroll
-                for (int a = 0; a < NC - 1; a++) c[r][a] = rin[r] ? arr[a][own[r] + sz * k] : (T)0;
+            for (int a = 0; a < NC; a++) {
+                const auto rs = rsrc(arr[a] + (long long)sz * k, pbytes);
+#pragma unroll
+                for (int r = 0; r < RY; r++)
+                    c[r][a] = ld<T>(rs, rin[r] ? (uint32_t)own[r] * (uint32_t)sizeof(T) : kOut);
             }
         }
     };
@@ -187,20 +243,28 @@ __global__ __launch_bounds__(TX) void k_himeno(const T* __restrict__ p, T* __res
     }
     T pnext[PPT];
     stage_load(k0 + 1, pnext);  // k0 + 1 <= nz - 1: always a plane of the array
-    T cc[RY][NC - 1], cn[RY][NC - 1];
-    coef_load(k0, cc);
-    double g = 0.0;
-    for (int k = k0; k < k1; k++) {
-        stage_store((k + 1) % NR, pnext);
-        if (k + 2 < nz) stage_load(k + 2, pnext);
-        if (k + 1 < k1) coef_load(k + 1, cn);
-        __syncthreads();
-        const int sm = (k - 1) % NR, s0s = k % NR, sp = (k + 1) % NR;
+    // coefficient ring: slot s holds plane k0 + s (mod NS); planes k0 .. k0+D-1 now
+    T cr[NS][RY][NC];
 #pragma unroll
-        for (int r = 0; r < RY; r++) {
-            const int ly = r + 1, lx = tx + 1;
-            if (rin[r]) {
-                const T* c = cc[r];
+    for (int s = 0; s < D; s++) coef_load(min(k0 + s, nz - 2), cr[s]);
+    double g = 0.0;
+    for (int kb = k0; kb < k1; kb += NS) {
+#pragma unroll
+        for (int s = 0; s < NS; s++) {
+            const int k = kb + s;
+            if (k >= k1) break;
+            stage_store((k + 1) % NR, pnext);
+            stage_load(min(k + 2, nz - 1), pnext);
+            // plane k+D into the slot plane k-1 used (past the chunk: a plane of the
+            // array whose values go unused)
+            coef_load(min(k + D, nz - 2), cr[(s + D) % NS]);
+            __syncthreads();
+            const int sm = (k - 1) % NR, s0s = k % NR, sp = (k + 1) % NR;
+            const auto rw = rsrc(pn + (long long)sz * k, pbytes);
+#pragma unroll
+            for (int r = 0; r < RY; r++) {
+                const int ly = r + 1, lx = tx + 1;
+                const T* c = cr[s][r];
                 const T pc = ring[s0s][ly][lx];
                 T s0 = c[0] * ring[s0s][ly][lx + 1] + c[1] * ring[s0s][ly + 1][lx] + c[2] * ring[sp][ly][lx]
                        + c[4] * (ring[s0s][ly + 1][lx + 1] - ring[s0s][ly - 1][lx + 1] - ring[s0s][ly + 1][lx - 1]
@@ -217,14 +281,10 @@ __global__ __launch_bounds__(TX) void k_himeno(const T* __restrict__ p, T* __res
                 } else {
                     ss = c[11] + (s0 * c[3] - pc) * c[10];
                 }
-                pn[own[r] + sz * k] = pc + omega * ss;
+                st<T>(rw, rin[r] ? (uint32_t)own[r] * (uint32_t)sizeof(T) : kOut, pc + omega * ss);
                 g += (double)ss * (double)ss;
             }
         }
-#pragma unroll
-        for (int r = 0; r < RY; r++)
-#pragma unroll
-            for (int a = 0; a < NC - 1; a++) cc[r][a] = cn[r][a];
     }
     block_sum_last(g, partials, ticket, gosa_out, red);
 }
@@ -336,12 +396,49 @@ dim3 grid_of(int nx, int ny, int nz, int zc)
     return dim3(ceil_div(nx - 2, TX) * ceil_div(ny - 2, RY) * ceil_div(nz - 2, zc));
 }
 
-// candidate z-chunk lengths of the sweep kernels (planes per block)
-constexpr int kZc[] = {8, 16, 32, 64, 128};
-constexpr int NZC = (int)(sizeof(kZc) / sizeof(kZc[0]));
-const char* const kZcNames[NZC] = {"zc8", "zc16", "zc32", "zc64", "zc128"};
+// candidate sweep shapes: planes per block (zc) and coefficient prefetch depth (D).
+// Long chunks win on the 257x257x513 config (zc256: one 4-wave block per CU, 0.371 ms
+// per FP32 sweep against 0.404 at zc128 and 0.46 at zc64, at any D); short ones are
+// there for shallow grids, where long chunks leave CUs without a block.
+struct SweepShape {
+    int zc, d;
+};
+constexpr SweepShape kShapes[] = {{8, 1},   {16, 1},  {32, 1},  {64, 1},  {128, 1}, {128, 2},
+                                  {256, 1}, {256, 2}, {256, 3}, {256, 4}, {512, 1}, {512, 2}};
+constexpr int NSHAPE = (int)(sizeof(kShapes) / sizeof(kShapes[0]));
+const char* const kShapeNames[NSHAPE] = {"zc8 d1",   "zc16 d1",  "zc32 d1",  "zc64 d1",  "zc128 d1", "zc128 d2",
+                                         "zc256 d1", "zc256 d2", "zc256 d3", "zc256 d4", "zc512 d1", "zc512 d2"};
+constexpr int kMinZc = 8;
+
+template <typename T, int MODE>
+void launch_sweep(appfis_ctx* ctx, SweepShape sh, const T* pin, T* pout, const Coef& cf, int nx, int ny, int nz,
+                  T omega, double* partials, unsigned* ticket, double* gosa)
+{
+    const dim3 grid = grid_of(nx, ny, nz, sh.zc);
+    if (sh.d == 1)
+        hipLaunchKernelGGL((k_himeno<T, MODE, 1>), grid, dim3(TX), 0, ctx->stream, pin, pout, cf, nx, ny, nz, sh.zc,
+                           omega, partials, ticket, gosa);
+    else if (sh.d == 2)
+        hipLaunchKernelGGL((k_himeno<T, MODE, 2>), grid, dim3(TX), 0, ctx->stream, pin, pout, cf, nx, ny, nz, sh.zc,
+                           omega, partials, ticket, gosa);
+    else if (sh.d == 3)
+        hipLaunchKernelGGL((k_himeno<T, MODE, 3>), grid, dim3(TX), 0, ctx->stream, pin, pout, cf, nx, ny, nz, sh.zc,
+                           omega, partials, ticket, gosa);
+    else
+        hipLaunchKernelGGL((k_himeno<T, MODE, 4>), grid, dim3(TX), 0, ctx->stream, pin, pout, cf, nx, ny, nz, sh.zc,
+                           omega, partials, ticket, gosa);
+}
 
 }  // namespace
+
+extern "C" int appfis_himeno_candidates(char* names, int len)
+{
+    std::string all;
+    for (int f = 0; f < 2; f++)
+        for (int i = 0; i < NSHAPE; i++) all += std::string(f ? "dc " : "plain ") + kShapeNames[i] + "\n";
+    if (names && len > 0) snprintf(names, (size_t)len, "%s", all.c_str());
+    return 2 * NSHAPE;
+}
 
 extern "C" int appfis_himeno(appfis_ctx* ctx, int dtype, void* const* arrays, int nx, int ny, int nz, double omega,
                              int iters, int variant, int outer, double* gosa_host)
@@ -353,8 +450,10 @@ extern "C" int appfis_himeno(appfis_ctx* ctx, int dtype, void* const* arrays, in
     APPFIS_REQUIRE(nx >= 3 && ny >= 3 && nz >= 3, "himeno needs at least 3 points per axis (shell included)");
     APPFIS_REQUIRE((double)nx * ny * nz < 2147483647.0, "grid exceeds the int index range");
     APPFIS_REQUIRE(iters >= 0, "iters must be >= 0");
+    APPFIS_REQUIRE((double)nx * ny * (dtype == APPFIS_F32 ? 4.0 : 8.0) < 2147483648.0,
+                   "a plane of the grid exceeds 2 GiB (32-bit buffer offsets)");
     if (iters == 0) return APPFIS_OK;
-    const size_t nblk = grid_of(nx, ny, nz, kZc[0]).x;  // the most blocks any candidate launches
+    const size_t nblk = grid_of(nx, ny, nz, kMinZc).x;  // the most blocks any candidate launches
     // scratch: ticket | partials[nblk] | gosa[iters]
     const size_t off_part = 256, off_gosa = off_part + ((nblk * 8 + 255) / 256) * 256;
     int st = ensure_scratch(ctx, off_gosa + (size_t)iters * 8 + 256);
@@ -365,37 +464,53 @@ extern "C" int appfis_himeno(appfis_ctx* ctx, int dtype, void* const* arrays, in
     Coef cf{arrays[2], arrays[3], arrays[4], arrays[5], arrays[6], arrays[7], arrays[8],
             arrays[9], arrays[10], arrays[11], arrays[12], arrays[13]};
 
-    // planes per block: timed once per (dtype, form, shape) -- a sweep from p0 into p1
+    // APPFIS_CANDIDATE0 + i: the PLAIN sweep with shape i forced (i < NSHAPE), or the
+    // defect correction with inner-sweep shape i - NSHAPE forced
+    int forced = -1;
+    if (variant >= APPFIS_CANDIDATE0) {
+        forced = variant - APPFIS_CANDIDATE0;
+        APPFIS_REQUIRE(forced < 2 * NSHAPE, "himeno: candidate %d out of range (%d candidates)", forced, 2 * NSHAPE);
+        variant = forced < NSHAPE ? APPFIS_PLAIN : APPFIS_CHOSEN;
+        forced %= NSHAPE;
+    }
+    // sweep shape: timed once per (dtype, form, grid shape) -- a sweep from p0 into p1
     // (PLAIN; the first real sweep rewrites p1) or between the correction buffers (CHOSEN)
-    auto tune_zc = [&](int form, const std::function<int(int)>& sweep) -> int {
+    auto tune_shape = [&](int form, const std::function<int(int)>& sweep, SweepShape* out) -> int {
+        if (forced >= 0) {
+            *out = kShapes[forced];
+            return APPFIS_OK;
+        }
         const int key[6] = {5, dtype, nx, ny, nz, form};
-        int pick = 2;
-        const int tst = tune_select(ctx, key, NZC, kZcNames, sweep, 2, &pick);
-        return tst ? -tst : kZc[pick];
+        int pick = 4;
+        const int tst = tune_select(ctx, key, NSHAPE, kShapeNames, sweep, 2, &pick);
+        *out = kShapes[pick];
+        return tst;
     };
     if (variant == APPFIS_PLAIN) {
         if (dtype != APPFIS_F32 && dtype != APPFIS_F64) return invalid("himeno PLAIN: dtype must be F32 or F64");
-        auto sweep = [&](int zc, int it) -> int {
+        auto sweep = [&](SweepShape sh, int it) -> int {
             void* pin = arrays[it & 1];
             void* pout = arrays[(it + 1) & 1];
-            const dim3 grid = grid_of(nx, ny, nz, zc);
             if (dtype == APPFIS_F32)
-                hipLaunchKernelGGL((k_himeno<float, 0>), grid, dim3(TX), 0, ctx->stream, (const float*)pin,
-                                   (float*)pout, cf, nx, ny, nz, zc, (float)omega, partials, ticket, gosa_dev + it);
+                launch_sweep<float, 0>(ctx, sh, (const float*)pin, (float*)pout, cf, nx, ny, nz, (float)omega,
+                                       partials, ticket, gosa_dev + it);
             else
-                hipLaunchKernelGGL((k_himeno<double, 0>), grid, dim3(TX), 0, ctx->stream, (const double*)pin,
-                                   (double*)pout, cf, nx, ny, nz, zc, omega, partials, ticket, gosa_dev + it);
+                launch_sweep<double, 0>(ctx, sh, (const double*)pin, (double*)pout, cf, nx, ny, nz, omega, partials,
+                                        ticket, gosa_dev + it);
             APPFIS_LAUNCH_CHECK();
             return APPFIS_OK;
         };
-        const int zc = tune_zc(0, [&](int c) { return sweep(kZc[c], 0); });
-        if (zc < 0) return -zc;
+        SweepShape sh;
+        st = tune_shape(0, [&](int c) { return sweep(kShapes[c], 0); }, &sh);
+        if (st) return st;
         for (int it = 0; it < iters; it++) {
-            st = sweep(zc, it);
+            st = sweep(sh, it);
             if (st) return st;
         }
     } else if (variant == APPFIS_CHOSEN) {
         APPFIS_REQUIRE(dtype == APPFIS_F64, "himeno CHOSEN (defect correction) takes the F64 problem");
+        APPFIS_REQUIRE(((double)nx * ny / 64.0 + 2.0) * BLK * 4.0 < 2147483648.0,
+                       "himeno CHOSEN: a plane of the blocked coefficients exceeds 2 GiB");
         // default rebase interval: 16 sweeps (gosa 1.3e-6 from FP64 at the 257x257x513
         // config, against 7.5e-7 at 8 and 2.6e-6 at 24; 1.08x the speed of 8)
         const int K = outer > 0 ? outer : 16;
@@ -421,15 +536,17 @@ extern "C" int appfis_himeno(appfis_ctx* ctx, int dtype, void* const* arrays, in
         APPFIS_HIP_CHECK(hipMemsetAsync(q[1], 0, n * 4, ctx->stream));
         Coef c32s{};
         c32s.inter = ci;
-        // z-chunk of the inner FP32 sweep, timed between the correction buffers (q[1] is
-        // rewritten by the first rebase; q[0] stays 0)
-        const int zc = tune_zc(1, [&](int c) {
-            hipLaunchKernelGGL((k_himeno<float, 2>), grid_of(nx, ny, nz, kZc[c]), dim3(TX), 0, ctx->stream, q[0],
-                               q[1], c32s, nx, ny, nz, kZc[c], (float)omega, partials, ticket, gosa_dev);
+        // shape of the inner FP32 sweep, timed between the correction buffers (q[1] is
+        // rewritten by the first rebase; q[0] stays 0); the rebase sweep takes its zc
+        SweepShape sh;
+        st = tune_shape(1, [&](int c) {
+            launch_sweep<float, 2>(ctx, kShapes[c], q[0], q[1], c32s, nx, ny, nz, (float)omega, partials, ticket,
+                                   gosa_dev);
             APPFIS_LAUNCH_CHECK();
             return APPFIS_OK;
-        });
-        if (zc < 0) return -zc;
+        }, &sh);
+        if (st) return st;
+        const int zc = sh.zc;
         const dim3 grid = grid_of(nx, ny, nz, zc);
         int bi = 0, qi = 0;
         for (int it = 0; it < iters; it++) {
@@ -438,8 +555,8 @@ extern "C" int appfis_himeno(appfis_ctx* ctx, int dtype, void* const* arrays, in
                                    q[qi ^ 1], cf, nx, ny, nz, zc, omega, partials, ticket, gosa_dev + it);
                 bi ^= 1;
             } else {
-                hipLaunchKernelGGL((k_himeno<float, 2>), grid, dim3(TX), 0, ctx->stream, q[qi], q[qi ^ 1], c32s, nx,
-                                   ny, nz, zc, (float)omega, partials, ticket, gosa_dev + it);
+                launch_sweep<float, 2>(ctx, sh, q[qi], q[qi ^ 1], c32s, nx, ny, nz, (float)omega, partials, ticket,
+                                       gosa_dev + it);
             }
             APPFIS_LAUNCH_CHECK();
             qi ^= 1;

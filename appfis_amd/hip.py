@@ -137,6 +137,7 @@ def lib():
         "appfis_heat2d_ensemble": ([vp, i, i, ip, ip, C.POINTER(vp), C.POINTER(vp), d, i], i),
         "appfis_gol_candidates": ([C.c_char_p, i], i),
         "appfis_himeno": ([vp, i, C.POINTER(vp), i, i, i, d, i, i, i, dp], i),
+        "appfis_himeno_candidates": ([C.c_char_p, i], i),
         "appfis_flood_run": ([vp, i, vp, vp, vp, vp, vp, i, i, C.POINTER(FloodParams), i, i, dp, dp], i),
         "appfis_flood_run_src": ([vp, i, vp, vp, vp, vp, vp, i, i, C.POINTER(FloodParams), C.POINTER(FloodSources),
                                   i, i, dp, dp, ip], i),
@@ -173,6 +174,12 @@ def heat2d_candidates(dtype: int):
 def gol_candidates():
     """Names of the bit-packed GoL blocked kernels (variant CANDIDATE0 + index)."""
     return _candidates(lib().appfis_gol_candidates)
+
+
+def himeno_candidates():
+    """Names of the Himeno sweep shapes, PLAIN then defect correction (variant
+    CANDIDATE0 + index)."""
+    return _candidates(lib().appfis_himeno_candidates)
 
 
 def check(status: int, what: str = "") -> None:

@@ -246,9 +246,17 @@ int appfis_fdtd3d_candidates(int dtype, char* names, int len);
  * Field in p[iters % 2] at exit.  gosa_host (iters doubles) receives each sweep's
  * gosa.  variant PLAIN: dtype F32 or F64, reference operation order.
  * variant CHOSEN (defect correction): dtype F64 inputs; FP32 inner sweeps on the
- * correction, FP64 outer residual every `outer` sweeps (<=0: every 16).  */
+ * correction, FP64 outer residual every `outer` sweeps (<=0: every 16).  Both tune
+ * the sweep shape (planes per block, coefficient prefetch depth) on first use per
+ * shape.  APPFIS_CANDIDATE0 + i runs candidate i of appfis_himeno_candidates: the
+ * PLAIN form or the defect correction with that sweep shape forced (same values).
+ * A plane of the grid must stay under 2 GiB.  */
 int appfis_himeno(appfis_ctx* ctx, int dtype, void* const* arrays, int nx, int ny, int nz,
                   double omega, int iters, int variant, int outer, double* gosa_host);
+
+/* Candidates of appfis_himeno ("plain zc.. d.." then "dc zc.. d.."), one name per
+ * line, into names[len] (may be NULL); returns their number.  Needs no context. */
+int appfis_himeno_candidates(char* names, int len);
 
 /* ---- Triton shallow-water flood step chain (Triton/src/triton_appfis.hpp:316-386
  * with the functors of triton_appfis.hpp:567-1112) plus uniform rainfall.

@@ -147,3 +147,42 @@ def test_himeno_full_config(ctx):
     print(f"[himeno full] gosa[199] FP64 {g64[-1]:.15e}; defect-correction rel err {gerr:.3e}, "
           f"p max-abs {perr:.3e} (bound {p_bound:.3e}); plain FP32 gosa rel err {gerr32:.3e}")
     assert gerr < HIMENO_DC_GOSA_RTOL and perr <= p_bound
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+def test_himeno_every_candidate(ctx, dtype):
+    # every sweep shape the autotuner may pick (planes per block 8 .. 512, coefficient
+    # prefetch depth 1 .. 4), forced one at a time on seeded coefficients: two x tiles
+    # (nx - 2 = 298 > 256 columns) and 278 interior planes, so the 256-plane chunks end
+    # with a short one and the 512-plane chunk holds the whole depth.  PLAIN: field
+    # bit-exact against the oracle, gosa to 1e-12 (its block partials follow the chunk
+    # length).  Defect correction (FP64 problem): every inner-sweep shape gives the same
+    # field, bit for bit, and gosa to 1e-12; the field is within the a-priori bound of
+    # the FP64 oracle.
+    nx, ny, nz, iters = 300, 21, 280, 4
+    names = hip.himeno_candidates()
+    plain = [i for i, n in enumerate(names) if n.startswith("plain ")]
+    dc = [i for i, n in enumerate(names) if n.startswith("dc ")]
+    assert len(plain) == len(dc) >= 8
+    arrs = himeno_arrays(nx, ny, nz, dtype, seed=2001)
+    ref_arrs = [a.copy() for a in arrs]
+    ref_gosa = oracle.himeno(ref_arrs, nx, ny, nz, 0.8, iters)
+    for i in plain:
+        print(f"[himeno candidate] {np.dtype(dtype).name} {names[i]}", flush=True)
+        p, gosa = run_gpu(ctx, arrs, nx, ny, nz, iters, hip.CANDIDATE0 + i)
+        assert np.array_equal(p, ref_arrs[iters % 2]), names[i]
+        assert np.allclose(gosa, ref_gosa, rtol=1e-12, atol=0), names[i]
+    if dtype != np.float64:
+        return
+    ref_p, ref_g, rmax = fp64_sweeps(arrs, nx, ny, nz, iters)
+    assert np.all(np.diff(ref_g) <= 0)  # the premise of dc_p_bound's carry-forward
+    first = None
+    for i in dc:
+        print(f"[himeno candidate] {names[i]}", flush=True)
+        p, gosa = run_gpu(ctx, arrs, nx, ny, nz, iters, hip.CANDIDATE0 + i)
+        if first is None:
+            first = (p, gosa)
+            assert np.max(np.abs(p - ref_p)) <= dc_p_bound(iters, 0, 0.8, rmax)
+            assert np.max(np.abs(gosa - ref_g) / ref_g) < HIMENO_DC_GOSA_RTOL
+        assert np.array_equal(p, first[0]), names[i]
+        assert np.allclose(gosa, first[1], rtol=1e-12, atol=0), names[i]
