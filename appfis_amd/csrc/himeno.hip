@@ -53,6 +53,10 @@ struct Coef {
     // a (a0 a1 a2 a3 b0 b1 b2 c0 c1 c2 bnd r) of point p at blk_idx(p, a), so a wave's 12
     // coefficient loads fall in one 3 KB region instead of 12 arrays
     const void* inter = nullptr;
+    // rebase sweep, MODE 1: the FP32 correction q added to the FP64 base while staging,
+    // and the two FP32 outputs (r into slot 11 of `inter`, q' = omega * ss)
+    const float* q_in = nullptr;
+    float* q_out = nullptr;
 };
 
 constexpr int BLK = 12 * 64;  // floats per block of 64 points
@@ -144,6 +148,10 @@ __device__ inline void st(__amdgpu_buffer_rsrc_t r, uint32_t off, T v)
 }
 
 // MODE 0 (PLAIN):  ss = (s0*a3 - p)*bnd with s0 including wrk1; pn = p + omega*ss
+// MODE 1 (rebase sweep of the defect correction, FP64): p is the base B, and the
+//                  staged plane is P = B + q (q = Coef::q_in, FP32); ss as in MODE 0
+//                  from P; writes B' = P into pn, r = (float)ss into slot 11 of the
+//                  blocked array Coef::inter and q' = (float)(omega*ss) into Coef::q_out
 // MODE 2 (inner sweep of the defect correction, FP32): p is the correction q and the
 //                  coefficients and r come from the blocked array Coef::inter;
 //                  ss = r + (s0*a3 - q)*bnd with s0 excluding wrk1; qn = q + omega*ss
@@ -160,7 +168,8 @@ __global__ __launch_bounds__(TX) void k_himeno(const T* __restrict__ p, T* __res
                                                int nz, int zc, T omega, double* partials, unsigned* ticket,
                                                double* gosa_out)
 {
-    static_assert(MODE == 0 || (MODE == 2 && sizeof(T) == 4), "PLAIN in F32 / F64, inner sweep in F32");
+    static_assert(MODE == 0 || (MODE == 1 && sizeof(T) == 8) || (MODE == 2 && sizeof(T) == 4),
+                  "PLAIN in F32 / F64, rebase in F64, inner sweep in F32");
     constexpr int NR = 4;                         // p ring slots
     constexpr int PPT = (LW * LH + TX - 1) / TX;  // staged p values per thread
     constexpr int NC = 12;                        // coefficient streams per point
@@ -204,6 +213,12 @@ __global__ __launch_bounds__(TX) void k_himeno(const T* __restrict__ p, T* __res
         const auto rs = rsrc(p + (long long)sz * k, pbytes);
 #pragma unroll
         for (int q = 0; q < PPT; q++) v[q] = ld<T>(rs, soff[q]);
+        if constexpr (MODE == 1) {  // P = B + q, q in FP32 (half the byte offset)
+            const auto rq = rsrc(cf.q_in + (long long)sz * k, pbytes / 2);
+#pragma unroll
+            for (int q = 0; q < PPT; q++)
+                v[q] = __dadd_rn(v[q], (double)ld<float>(rq, soff[q] == kOut ? kOut : soff[q] / 2));
+        }
     };
     auto stage_store = [&](int slot, const T(&v)[PPT]) {
 #pragma unroll
@@ -275,91 +290,28 @@ __global__ __launch_bounds__(TX) void k_himeno(const T* __restrict__ p, T* __res
                                  + ring[sm][ly][lx - 1])
                        + c[7] * ring[s0s][ly][lx - 1] + c[8] * ring[s0s][ly - 1][lx] + c[9] * ring[sm][ly][lx];
                 T ss;
-                if constexpr (MODE == 0) {
+                if constexpr (MODE == 2) {
+                    ss = c[11] + (s0 * c[3] - pc) * c[10];
+                } else {
                     s0 = s0 + c[11];
                     ss = (s0 * c[3] - pc) * c[10];
-                } else {
-                    ss = c[11] + (s0 * c[3] - pc) * c[10];
                 }
-                st<T>(rw, rin[r] ? (uint32_t)own[r] * (uint32_t)sizeof(T) : kOut, pc + omega * ss);
+                const uint32_t so = rin[r] ? (uint32_t)own[r] * (uint32_t)sizeof(T) : kOut;
+                if constexpr (MODE == 1) {
+                    st<T>(rw, so, pc);
+                    st<float>(rsrc(cf.q_out + (long long)sz * k, pbytes / 2), so == kOut ? kOut : so / 2,
+                              (float)(omega * ss));
+                    const int b = sz * k, q = own[r] + b;
+                    st<float>(rsrc((const float*)cf.inter + (size_t)(b >> 6) * BLK,
+                                   (uint32_t)((sz >> 6) + 2) * (uint32_t)(BLK * 4)),
+                              rin[r] ? (uint32_t)(((q >> 6) - (b >> 6)) * BLK + 11 * 64 + (q & 63)) * 4u : kOut,
+                              (float)ss);
+                } else {
+                    st<T>(rw, so, pc + omega * ss);
+                }
                 g += (double)ss * (double)ss;
             }
         }
-    }
-    block_sum_last(g, partials, ticket, gosa_out, red);
-}
-
-// CHOSEN rebase sweep (FP64): P = B + q; ss = ss(P) in FP64 with the FP64 coefficients;
-// writes B = P, r = (float)ss, qn = (float)(omega*ss); gosa from the FP64 ss.
-__global__ __launch_bounds__(TX) void k_himeno_rebase(const double* __restrict__ B, const float* __restrict__ q,
-                                                      double* __restrict__ Bn, float* __restrict__ r,
-                                                      float* __restrict__ qn, Coef cf, int nx, int ny, int nz, int zc,
-                                                      double omega, double* partials, unsigned* ticket,
-                                                      double* gosa_out)
-{
-    __shared__ double ring[3][LH][LW];
-    __shared__ double red[TX / 64];
-    const int tx = threadIdx.x;
-    int bx, by, bz;
-    tile_of(nx, ny, bx, by, bz);
-    const int i = bx * TX + tx + 1;
-    const int j0 = by * RY + 1;
-    const int k0 = bz * zc + 1;
-    const int k1 = min(k0 + zc, nz - 1);
-    const size_t sy = (size_t)nx, sz = (size_t)nx * (size_t)ny;
-    const bool xin = i < nx - 1;
-    const double* a0 = (const double*)cf.a0; const double* a1 = (const double*)cf.a1;
-    const double* a2 = (const double*)cf.a2; const double* a3 = (const double*)cf.a3;
-    const double* b0 = (const double*)cf.b0; const double* b1 = (const double*)cf.b1;
-    const double* b2 = (const double*)cf.b2;
-    const double* c0 = (const double*)cf.c0; const double* c1 = (const double*)cf.c1;
-    const double* c2 = (const double*)cf.c2;
-    const double* bnd = (const double*)cf.bnd; const double* w1 = (const double*)cf.wrk1;
-
-    auto stage = [&](int k, int slot) {
-        const int xb = bx * TX;
-        for (int t = tx; t < LW * LH; t += TX) {
-            int ly = t / LW, lx = t - ly * LW;
-            int gx = xb + lx, gy = j0 - 1 + ly;
-            double v = 0.0;
-            if (gx < nx && gy < ny) {
-                size_t o = (size_t)gx + sy * gy + sz * k;
-                v = __dadd_rn(B[o], (double)q[o]);
-            }
-            ring[slot][ly][lx] = v;
-        }
-    };
-    stage(k0 - 1, (k0 - 1) % 3);
-    stage(k0, k0 % 3);
-    double g = 0.0;
-    for (int k = k0; k < k1; k++) {
-        stage(k + 1, (k + 1) % 3);
-        __syncthreads();
-        const int sm = (k - 1) % 3, s0s = k % 3, sp = (k + 1) % 3;
-#pragma unroll
-        for (int rr = 0; rr < RY; rr++) {
-            const int jj = j0 + rr;
-            const int ly = rr + 1, lx = tx + 1;
-            if (xin && jj < ny - 1) {
-                const size_t o = (size_t)i + sy * jj + sz * k;
-                const double pc = ring[s0s][ly][lx];
-                double s0 = a0[o] * ring[s0s][ly][lx + 1] + a1[o] * ring[s0s][ly + 1][lx] + a2[o] * ring[sp][ly][lx]
-                            + b0[o] * (ring[s0s][ly + 1][lx + 1] - ring[s0s][ly - 1][lx + 1]
-                                       - ring[s0s][ly + 1][lx - 1] + ring[s0s][ly - 1][lx - 1])
-                            + b1[o] * (ring[sp][ly + 1][lx] - ring[sp][ly - 1][lx] - ring[sm][ly + 1][lx]
-                                       + ring[sm][ly - 1][lx])
-                            + b2[o] * (ring[sp][ly][lx + 1] - ring[sp][ly][lx - 1] - ring[sm][ly][lx + 1]
-                                       + ring[sm][ly][lx - 1])
-                            + c0[o] * ring[s0s][ly][lx - 1] + c1[o] * ring[s0s][ly - 1][lx]
-                            + c2[o] * ring[sm][ly][lx] + w1[o];
-                const double ss = (s0 * a3[o] - pc) * bnd[o];
-                Bn[o] = pc;
-                r[blk_idx(o, 11)] = (float)ss;  // the r slot of the blocked coefficients
-                qn[o] = (float)(omega * ss);
-                g += ss * ss;
-            }
-        }
-        __syncthreads();
     }
     block_sum_last(g, partials, ticket, gosa_out, red);
 }
@@ -537,7 +489,7 @@ extern "C" int appfis_himeno(appfis_ctx* ctx, int dtype, void* const* arrays, in
         Coef c32s{};
         c32s.inter = ci;
         // shape of the inner FP32 sweep, timed between the correction buffers (q[1] is
-        // rewritten by the first rebase; q[0] stays 0); the rebase sweep takes its zc
+        // rewritten by the first rebase; q[0] stays 0); the rebase sweep takes it too
         SweepShape sh;
         st = tune_shape(1, [&](int c) {
             launch_sweep<float, 2>(ctx, kShapes[c], q[0], q[1], c32s, nx, ny, nz, (float)omega, partials, ticket,
@@ -546,13 +498,15 @@ extern "C" int appfis_himeno(appfis_ctx* ctx, int dtype, void* const* arrays, in
             return APPFIS_OK;
         }, &sh);
         if (st) return st;
-        const int zc = sh.zc;
-        const dim3 grid = grid_of(nx, ny, nz, zc);
         int bi = 0, qi = 0;
         for (int it = 0; it < iters; it++) {
             if (it % K == 0) {
-                hipLaunchKernelGGL(k_himeno_rebase, grid, dim3(TX), 0, ctx->stream, B[bi], q[qi], B[bi ^ 1], ci,
-                                   q[qi ^ 1], cf, nx, ny, nz, zc, omega, partials, ticket, gosa_dev + it);
+                Coef cr = cf;
+                cr.inter = ci;
+                cr.q_in = q[qi];
+                cr.q_out = q[qi ^ 1];
+                launch_sweep<double, 1>(ctx, sh, B[bi], B[bi ^ 1], cr, nx, ny, nz, omega, partials, ticket,
+                                        gosa_dev + it);
                 bi ^= 1;
             } else {
                 launch_sweep<float, 2>(ctx, sh, q[qi], q[qi ^ 1], c32s, nx, ny, nz, (float)omega, partials, ticket,

@@ -494,7 +494,7 @@ def bench_himeno(ctx, hip, nx=257, ny=257, nz=513, iters=200):
             # 8 + 4 + 4 B: 124 B/LUP)
             K = HIMENO_DC_OUTER
             r["roofline"] = roofline(((K - 1) * 56.0 + 124.0) / K * pts, ms / iters,
-                                     {f"k_himeno<float, 2, {d}>": (K - 1) / K, "k_himeno_rebase": 1 / K})
+                                     {f"k_himeno<float, 2, {d}>": (K - 1) / K, f"k_himeno<double, 1, {d}>": 1 / K})
         out[name] = r
         for a in arrs:
             a.free()
