@@ -33,8 +33,7 @@ using namespace appfis;
 namespace {
 
 constexpr int TX = 256;  // threads along x (one block spans 256 interior columns)
-constexpr int RY = 2;    // rows per thread
-constexpr int LW = TX + 2, LH = RY + 2;
+constexpr int LW = TX + 2;  // staged row: the block's columns and their x halo
 
 struct Coef {
     const void* a0;
@@ -64,9 +63,9 @@ __host__ __device__ inline size_t blk_idx(size_t p, int a) { return (p >> 6) * B
 
 // Block -> (x, y, z) tile of a 1-D launch: tiles x fastest, then y, then z chunks, and
 // every XCD takes a contiguous run, so y-neighbour tiles (which share p rows) share an L2.
-__device__ inline void tile_of(int nx, int ny, int& bx, int& by, int& bz)
+__device__ inline void tile_of(int nx, int ny, int ry, int& bx, int& by, int& bz)
 {
-    const int gx = (nx - 2 + TX - 1) / TX, gy = (ny - 2 + RY - 1) / RY;
+    const int gx = (nx - 2 + TX - 1) / TX, gy = (ny - 2 + ry - 1) / ry;
     const int t = (int)xcd_tile(blockIdx.x, gridDim.x);
     bx = t % gx;
     const int r = t / gx;
@@ -163,13 +162,14 @@ __device__ inline void st(__amdgpu_buffer_rsrc_t r, uint32_t off, T v)
 // separates the staging of plane k+1 from its use (the slot it overwrites was last read
 // two planes ago).  A thread outside the interior computes with zero coefficients, so
 // its ss is 0 (it adds nothing to gosa) and its store is dropped.
-template <typename T, int MODE, int D>
+template <typename T, int MODE, int D, int RY>
 __global__ __launch_bounds__(TX) void k_himeno(const T* __restrict__ p, T* __restrict__ pn, Coef cf, int nx, int ny,
                                                int nz, int zc, T omega, double* partials, unsigned* ticket,
                                                double* gosa_out)
 {
     static_assert(MODE == 0 || (MODE == 1 && sizeof(T) == 8) || (MODE == 2 && sizeof(T) == 4),
                   "PLAIN in F32 / F64, rebase in F64, inner sweep in F32");
+    constexpr int LH = RY + 2;                    // staged rows: the block's rows and their y halo
     constexpr int NR = 4;                         // p ring slots
     constexpr int PPT = (LW * LH + TX - 1) / TX;  // staged p values per thread
     constexpr int NC = 12;                        // coefficient streams per point
@@ -178,7 +178,7 @@ __global__ __launch_bounds__(TX) void k_himeno(const T* __restrict__ p, T* __res
     __shared__ double red[TX / 64];
     const int tx = threadIdx.x;
     int bx, by, bz;
-    tile_of(nx, ny, bx, by, bz);
+    tile_of(nx, ny, RY, bx, by, bz);
     const int i = bx * TX + tx + 1;  // interior x of this thread
     const int j0 = by * RY + 1;      // first interior row of the block
     const int k0 = bz * zc + 1;
@@ -343,42 +343,49 @@ __global__ void k_add_correction(const double* __restrict__ B, const float* __re
 }
 
 // 1-D launch over (x tiles) * (y tiles) * (z chunks of zc planes)
-dim3 grid_of(int nx, int ny, int nz, int zc)
+dim3 grid_of(int nx, int ny, int nz, int zc, int ry)
 {
-    return dim3(ceil_div(nx - 2, TX) * ceil_div(ny - 2, RY) * ceil_div(nz - 2, zc));
+    return dim3(ceil_div(nx - 2, TX) * ceil_div(ny - 2, ry) * ceil_div(nz - 2, zc));
 }
 
-// candidate sweep shapes: planes per block (zc) and coefficient prefetch depth (D).
-// Long chunks win on the 257x257x513 config (zc256: one 4-wave block per CU, 0.371 ms
-// per FP32 sweep against 0.404 at zc128 and 0.46 at zc64, at any D); short ones are
-// there for shallow grids, where long chunks leave CUs without a block.
+// candidate sweep shapes: planes per block (zc), coefficient prefetch depth (D) and
+// rows per thread (RY).  Long chunks win on the 257x257x513 config (zc256: one 4-wave
+// block per CU, 0.371 ms per FP32 sweep against 0.404 at zc128 and 0.46 at zc64, at
+// any D; one 511-plane chunk of 1-row tiles is 1.5% faster again for FP64, 8% slower
+// for FP32); short ones are there for shallow grids, where long chunks leave CUs
+// without a block.
 struct SweepShape {
-    int zc, d;
+    int zc, d, ry;
 };
-constexpr SweepShape kShapes[] = {{8, 1},   {16, 1},  {32, 1},  {64, 1},  {128, 1}, {128, 2},
-                                  {256, 1}, {256, 2}, {256, 3}, {256, 4}, {512, 1}, {512, 2}};
+constexpr SweepShape kShapes[] = {{8, 1, 2},   {16, 1, 2},  {32, 1, 2},  {64, 1, 2},  {128, 1, 2},
+                                  {128, 2, 2}, {256, 1, 2}, {256, 2, 2}, {256, 3, 2}, {256, 4, 2},
+                                  {512, 1, 2}, {512, 2, 2}, {512, 2, 1}};
 constexpr int NSHAPE = (int)(sizeof(kShapes) / sizeof(kShapes[0]));
-const char* const kShapeNames[NSHAPE] = {"zc8 d1",   "zc16 d1",  "zc32 d1",  "zc64 d1",  "zc128 d1", "zc128 d2",
-                                         "zc256 d1", "zc256 d2", "zc256 d3", "zc256 d4", "zc512 d1", "zc512 d2"};
-constexpr int kMinZc = 8;
+const char* const kShapeNames[NSHAPE] = {"zc8 d1",      "zc16 d1",     "zc32 d1",     "zc64 d1",     "zc128 d1",
+                                         "zc128 d2",    "zc256 d1",    "zc256 d2",    "zc256 d3",    "zc256 d4",
+                                         "zc512 d1",    "zc512 d2",    "zc512 d2 ry1"};
+constexpr int kMinZc = 8, kMinRy = 1;
+
+template <typename T, int MODE, int D, int RY>
+void launch_one(appfis_ctx* ctx, SweepShape sh, const T* pin, T* pout, const Coef& cf, int nx, int ny, int nz,
+                T omega, double* partials, unsigned* ticket, double* gosa)
+{
+    hipLaunchKernelGGL((k_himeno<T, MODE, D, RY>), grid_of(nx, ny, nz, sh.zc, RY), dim3(TX), 0, ctx->stream, pin,
+                       pout, cf, nx, ny, nz, sh.zc, omega, partials, ticket, gosa);
+}
 
 template <typename T, int MODE>
 void launch_sweep(appfis_ctx* ctx, SweepShape sh, const T* pin, T* pout, const Coef& cf, int nx, int ny, int nz,
                   T omega, double* partials, unsigned* ticket, double* gosa)
 {
-    const dim3 grid = grid_of(nx, ny, nz, sh.zc);
-    if (sh.d == 1)
-        hipLaunchKernelGGL((k_himeno<T, MODE, 1>), grid, dim3(TX), 0, ctx->stream, pin, pout, cf, nx, ny, nz, sh.zc,
-                           omega, partials, ticket, gosa);
-    else if (sh.d == 2)
-        hipLaunchKernelGGL((k_himeno<T, MODE, 2>), grid, dim3(TX), 0, ctx->stream, pin, pout, cf, nx, ny, nz, sh.zc,
-                           omega, partials, ticket, gosa);
-    else if (sh.d == 3)
-        hipLaunchKernelGGL((k_himeno<T, MODE, 3>), grid, dim3(TX), 0, ctx->stream, pin, pout, cf, nx, ny, nz, sh.zc,
-                           omega, partials, ticket, gosa);
-    else
-        hipLaunchKernelGGL((k_himeno<T, MODE, 4>), grid, dim3(TX), 0, ctx->stream, pin, pout, cf, nx, ny, nz, sh.zc,
-                           omega, partials, ticket, gosa);
+    auto go = [&](auto fn) { fn(ctx, sh, pin, pout, cf, nx, ny, nz, omega, partials, ticket, gosa); };
+    switch (sh.ry * 10 + sh.d) {
+    case 21: go(launch_one<T, MODE, 1, 2>); break;
+    case 22: go(launch_one<T, MODE, 2, 2>); break;
+    case 23: go(launch_one<T, MODE, 3, 2>); break;
+    case 24: go(launch_one<T, MODE, 4, 2>); break;
+    default: go(launch_one<T, MODE, 2, 1>); break;
+    }
 }
 
 }  // namespace
@@ -405,7 +412,7 @@ extern "C" int appfis_himeno(appfis_ctx* ctx, int dtype, void* const* arrays, in
     APPFIS_REQUIRE((double)nx * ny * (dtype == APPFIS_F32 ? 4.0 : 8.0) < 2147483648.0,
                    "a plane of the grid exceeds 2 GiB (32-bit buffer offsets)");
     if (iters == 0) return APPFIS_OK;
-    const size_t nblk = grid_of(nx, ny, nz, kMinZc).x;  // the most blocks any candidate launches
+    const size_t nblk = grid_of(nx, ny, nz, kMinZc, kMinRy).x;  // at least the blocks any candidate launches
     // scratch: ticket | partials[nblk] | gosa[iters]
     const size_t off_part = 256, off_gosa = off_part + ((nblk * 8 + 255) / 256) * 256;
     int st = ensure_scratch(ctx, off_gosa + (size_t)iters * 8 + 256);
