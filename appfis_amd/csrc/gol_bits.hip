@@ -410,7 +410,11 @@ __global__ __launch_bounds__(64 * NWAVES) void k_step_tb1(const uint32_t* __rest
 // block's window).  A wave still reading generation g's slots is never overwritten:
 // the next writes to them come after the barrier of generation g+1, which every wave
 // reaches only after finishing g.  G generations per launch, OUT = NWAVES*K - 2G
-// output rows per block.
+// output rows per block.  Window row r holds a valid state after generation g (counted
+// from 0) only for g + 1 <= r <= NWAVES*K - 2 - g: the waves at the window's ends skip
+// the rule once all their rows are past that range, and their horizontal sums once
+// their rows were already past it before the generation (nothing valid reads them then;
+// they still meet every barrier).
 template <int WPL>
 struct WordVec;
 template <> struct WordVec<1> {
@@ -450,9 +454,13 @@ __global__ __launch_bounds__(64 * NWAVES) void k_step_tw(const uint32_t* __restr
         while (y >= ny) y -= ny;
         VW::get(reinterpret_cast<const V*>(P + (size_t)y * NW)[lane], R[k]);
     }
+    const int top = wv * K, bot = wv * K + K - 1, last = NWAVES * K - 1;  // window rows of the wave
     for (int g = 0; g < gens; g++) {
         const int b = g & 1;
+        const bool sums = bot >= g && top <= last - g;        // some row valid before generation g
+        const bool rule = bot >= g + 1 && top <= last - 1 - g;  // some row valid after it
         uint32_t h0[K + 2][WPL], h1[K + 2][WPL];
+        if (sums) {
 #pragma unroll
         for (int k = 0; k < K; k++) {
             // the words left of the lane's first and right of its last (periodic row; a
@@ -474,7 +482,9 @@ __global__ __launch_bounds__(64 * NWAVES) void k_step_tw(const uint32_t* __restr
         reinterpret_cast<V*>(hs[b][wv][0][1])[lane] = VW::put(h1[1]);
         reinterpret_cast<V*>(hs[b][wv][1][0])[lane] = VW::put(h0[K]);
         reinterpret_cast<V*>(hs[b][wv][1][1])[lane] = VW::put(h1[K]);
+        }
         __syncthreads();
+        if (!rule) continue;
         {
             // rows -1 and K: whole-vector reads from the neighbouring waves' slots.  The
             // block's first and last waves read their own slots instead: rows outside the
