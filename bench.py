@@ -480,12 +480,14 @@ def bench_himeno(ctx, hip, nx=257, ny=257, nz=513, iters=200):
         ms, _ = timed(ctx, run)
         r = {"ms": round(ms, 3), "glups": round(lups / (ms * 1e-3) / 1e9, 2), "gosa_last": float(gosas[name][-1])}
         pts = (nx - 2) * (ny - 2) * (nz - 2)
-        shape = _tuned(ctx, 5, dt)  # "zc<planes per block> d<coefficient prefetch depth>"
-        d = int(shape.split()[1][1:]) if shape else 1
+        # "zc<planes per block> d<coefficient prefetch depth>[ ry<rows per thread>]"
+        shape = _tuned(ctx, 5, dt) or "zc128 d1"
+        d = int(shape.split()[1][1:])
+        ry = int(shape.split()[2][2:]) if len(shape.split()) > 2 else 2
         r["tiling"] = shape
         if bpl:
             r["roofline"] = roofline(float(bpl) * pts, ms / iters,
-                                     f"k_himeno<{'float' if dt == hip.F32 else 'double'}, 0, {d}>")
+                                     f"k_himeno<{'float' if dt == hip.F32 else 'double'}, 0, {d}, {ry}>")
         else:
             # defect correction, per sweep on average over one outer cycle of K sweeps
             # (the library's default K = 16): K-1 FP32 inner sweeps (11 coefficients, read
@@ -494,7 +496,8 @@ def bench_himeno(ctx, hip, nx=257, ny=257, nz=513, iters=200):
             # 8 + 4 + 4 B: 124 B/LUP)
             K = HIMENO_DC_OUTER
             r["roofline"] = roofline(((K - 1) * 56.0 + 124.0) / K * pts, ms / iters,
-                                     {f"k_himeno<float, 2, {d}>": (K - 1) / K, f"k_himeno<double, 1, {d}>": 1 / K})
+                                     {f"k_himeno<float, 2, {d}, {ry}>": (K - 1) / K,
+                                      f"k_himeno<double, 1, {d}, {ry}>": 1 / K})
         out[name] = r
         for a in arrs:
             a.free()
