@@ -569,7 +569,8 @@ int run_dma_v(appfis_ctx* ctx, void* b0, void* b1, int nx, int ny, int nz, const
 
 // Candidate tilings (VX x-points per thread, TY thread rows, RY rows per thread, ZC
 // planes per block, D-deep prefetch, U-step unroll: 18 rotates the queue).  Index 0 of
-// each table is the PLAIN default: the fastest measured tiling that fits any shape.
+// each table is the PLAIN default: the fastest measured tiling that fits any shape;
+// FP32 PLAIN takes the first "dma" row instead where nx + 8 and the buffers allow it.
 static const FdtdVariant kF32[] = {
     {"vx1 ty16 ry2 zc256 d4 u4", run_v<float, 1, 16, 2, 256, 4, 4>, false, 1},
     {"vx1 ty8 ry4 zc256 d2 u2", run_v<float, 1, 8, 4, 256, 2, 2>, false, 1},
@@ -678,7 +679,17 @@ extern "C" int appfis_fdtd3d(appfis_ctx* ctx, int dtype, void* buf0, void* buf1,
         return steps == 0 ? APPFIS_OK : tab[i].fn(ctx, buf0, buf1, nx, ny, nz, coeff5, steps);
     }
     if (steps == 0) return APPFIS_OK;
-    if (variant == APPFIS_PLAIN) return tab[0].fn(ctx, buf0, buf1, nx, ny, nz, coeff5, steps);
+    if (variant == APPFIS_PLAIN) {
+        // a fixed tiling: the table's first direct-to-LDS row where the rows and buffers
+        // admit it (FP32), else row 0 -- one launch per sweep either way
+        int i0 = 0;
+        for (int i = 0; i < ntab; i++)
+            if (tab[i].wmod && fits(tab[i])) {
+                i0 = i;
+                break;
+            }
+        return tab[i0].fn(ctx, buf0, buf1, nx, ny, nz, coeff5, steps);
+    }
     // CHOSEN: the fastest tiling for this shape, timed once on scratch buffers.  Every
     // candidate evaluates the same expression in the same order: results are identical.
     std::vector<int> cand;
