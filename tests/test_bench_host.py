@@ -43,7 +43,9 @@ def test_kernel_name_normalisation():
     assert s("void gol::k_gol_plain<2, 16>(unsigned char const*, unsigned char*, int, int, int, int)") == \
         "k_gol_plain<2, 16>"
     assert s("void golbits::k_step_tb<10, 4, 16>(unsigned int const*, unsigned int*, int)") == "k_step_tb<10, 4, 16>"
-    assert s("(anonymous namespace)::k_himeno_rebase(double const*, float const*)") == "k_himeno_rebase"
+    assert s("void (anonymous namespace)::k_himeno<double, 1, 2, 1>(double const*, double*, (anonymous "
+             "namespace)::Coef, int)") == "k_himeno<double, 1, 2, 1>"
+    assert s("(anonymous namespace)::k_add_correction(double const*, float const*)") == "k_add_correction"
     assert s("k_copy16(uint4 const*, uint4*, unsigned long)") == "k_copy16"
 
 
