@@ -433,55 +433,51 @@ template <> struct WordVec<4> {
     __device__ static V put(const uint32_t (&w)[4]) { return make_uint4(w[0], w[1], w[2], w[3]); }
 };
 
-template <int WPL, int K, int G, int NWAVES>
-__global__ __launch_bounds__(64 * NWAVES) void k_step_tw(const uint32_t* __restrict__ P, uint32_t* __restrict__ Q,
-                                                          int NW, int ny, int WX, int OUT, int gens)
+// The generation loop of the full-row form: `gens` generations of the K rows in R
+// (window rows wv*K .. wv*K+K-1 of the block); with KEEP, Rp receives the rows as they
+// were before the last generation.
+template <int WPL, int K, int G, int NWAVES, bool KEEP>
+__device__ __forceinline__ void tw_generations(uint32_t (&R)[K][WPL], uint32_t (&Rp)[K][WPL],
+                                               uint32_t (*hs)[NWAVES][2][2][64 * WPL], int lane, int wv, int gens)
 {
     using VW = WordVec<WPL>;
     using V = typename VW::V;
-    (void)WX;
-    // [parity][wave][first / last row][bit-plane][word of the row]
-    __shared__ uint32_t hs[2][NWAVES][2][2][64 * WPL];
-    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-    const int band = (int)xcd_tile(blockIdx.x, gridDim.x);  // XCD-contiguous band order
-    const int y0 = band * OUT - G + wv * K;  // global row of register row 0
-    int yw = y0 % ny;
-    if (yw < 0) yw += ny;
-    uint32_t R[K][WPL];
-#pragma unroll
-    for (int k = 0; k < K; k++) {
-        int y = yw + k;  // periodic rows (the window may wrap more than once when ny < K)
-        while (y >= ny) y -= ny;
-        VW::get(reinterpret_cast<const V*>(P + (size_t)y * NW)[lane], R[k]);
-    }
     const int top = wv * K, bot = wv * K + K - 1, last = NWAVES * K - 1;  // window rows of the wave
     for (int g = 0; g < gens; g++) {
         const int b = g & 1;
         const bool sums = bot >= g && top <= last - g;        // some row valid before generation g
         const bool rule = bot >= g + 1 && top <= last - 1 - g;  // some row valid after it
+        if constexpr (KEEP) {
+            if (g == gens - 1) {
+#pragma unroll
+                for (int k = 0; k < K; k++)
+#pragma unroll
+                    for (int w = 0; w < WPL; w++) Rp[k][w] = R[k][w];
+            }
+        }
         uint32_t h0[K + 2][WPL], h1[K + 2][WPL];
         if (sums) {
 #pragma unroll
-        for (int k = 0; k < K; k++) {
-            // the words left of the lane's first and right of its last (periodic row; a
-            // rotation has a source for every lane)
-            const uint32_t lft = (uint32_t)__builtin_amdgcn_mov_dpp((int)R[k][WPL - 1], 0x13C, 0xF, 0xF, true);
-            const uint32_t rgt = (uint32_t)__builtin_amdgcn_mov_dpp((int)R[k][0], 0x134, 0xF, 0xF, true);
+            for (int k = 0; k < K; k++) {
+                // the words left of the lane's first and right of its last (periodic row;
+                // a rotation has a source for every lane)
+                const uint32_t lft = (uint32_t)__builtin_amdgcn_mov_dpp((int)R[k][WPL - 1], 0x13C, 0xF, 0xF, true);
+                const uint32_t rgt = (uint32_t)__builtin_amdgcn_mov_dpp((int)R[k][0], 0x134, 0xF, 0xF, true);
 #pragma unroll
-            for (int w = 0; w < WPL; w++) {
-                const uint32_t c = R[k][w];
-                const uint32_t l = w == 0 ? lft : R[k][w - 1];
-                const uint32_t r = w == WPL - 1 ? rgt : R[k][w + 1];
-                const uint32_t Wc = __builtin_amdgcn_alignbit(c, l, 31);
-                const uint32_t Ec = __builtin_amdgcn_alignbit(r, c, 1);
-                h0[k + 1][w] = xor3(Wc, c, Ec);
-                h1[k + 1][w] = maj(Wc, c, Ec);
+                for (int w = 0; w < WPL; w++) {
+                    const uint32_t c = R[k][w];
+                    const uint32_t l = w == 0 ? lft : R[k][w - 1];
+                    const uint32_t r = w == WPL - 1 ? rgt : R[k][w + 1];
+                    const uint32_t Wc = __builtin_amdgcn_alignbit(c, l, 31);
+                    const uint32_t Ec = __builtin_amdgcn_alignbit(r, c, 1);
+                    h0[k + 1][w] = xor3(Wc, c, Ec);
+                    h1[k + 1][w] = maj(Wc, c, Ec);
+                }
             }
-        }
-        reinterpret_cast<V*>(hs[b][wv][0][0])[lane] = VW::put(h0[1]);
-        reinterpret_cast<V*>(hs[b][wv][0][1])[lane] = VW::put(h1[1]);
-        reinterpret_cast<V*>(hs[b][wv][1][0])[lane] = VW::put(h0[K]);
-        reinterpret_cast<V*>(hs[b][wv][1][1])[lane] = VW::put(h1[K]);
+            reinterpret_cast<V*>(hs[b][wv][0][0])[lane] = VW::put(h0[1]);
+            reinterpret_cast<V*>(hs[b][wv][0][1])[lane] = VW::put(h1[1]);
+            reinterpret_cast<V*>(hs[b][wv][1][0])[lane] = VW::put(h0[K]);
+            reinterpret_cast<V*>(hs[b][wv][1][1])[lane] = VW::put(h1[K]);
         }
         __syncthreads();
         if (!rule) continue;
@@ -506,6 +502,30 @@ __global__ __launch_bounds__(64 * NWAVES) void k_step_tw(const uint32_t* __restr
                 R[k][w] = life_rule(s0, c0, v1, c1, R[k][w]);
             }
     }
+}
+
+template <int WPL, int K, int G, int NWAVES>
+__global__ __launch_bounds__(64 * NWAVES) void k_step_tw(const uint32_t* __restrict__ P, uint32_t* __restrict__ Q,
+                                                          int NW, int ny, int WX, int OUT, int gens)
+{
+    using VW = WordVec<WPL>;
+    using V = typename VW::V;
+    (void)WX;
+    // [parity][wave][first / last row][bit-plane][word of the row]
+    __shared__ uint32_t hs[2][NWAVES][2][2][64 * WPL];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int band = (int)xcd_tile(blockIdx.x, gridDim.x);  // XCD-contiguous band order
+    const int y0 = band * OUT - G + wv * K;  // global row of register row 0
+    int yw = y0 % ny;
+    if (yw < 0) yw += ny;
+    uint32_t R[K][WPL], Rp[K][WPL];
+#pragma unroll
+    for (int k = 0; k < K; k++) {
+        int y = yw + k;  // periodic rows (the window may wrap more than once when ny < K)
+        while (y >= ny) y -= ny;
+        VW::get(reinterpret_cast<const V*>(P + (size_t)y * NW)[lane], R[k]);
+    }
+    tw_generations<WPL, K, G, NWAVES, false>(R, Rp, hs, lane, wv, gens);
     // output rows: window rows G .. G+OUT-1 of the block, global rows band*OUT + r
     const int ybase = band * OUT - G + wv * K;
 #pragma unroll
@@ -516,12 +536,116 @@ __global__ __launch_bounds__(64 * NWAVES) void k_step_tw(const uint32_t* __restr
     }
 }
 
+// 16 cells of a padded uint8 row from 16 bits (bit j = cell j), as one 16-byte store
+// (the compiler merges the byte stores), plus the ghost cell the periodic exchange
+// copies from it: chunk 0 holds cell 0, which ghost column nx+1 takes; the last chunk
+// holds cell nx-1, which ghost column 0 takes.
+typedef unsigned int u32x4_u __attribute__((ext_vector_type(4), aligned(1)));  // 16 bytes, any address
+__device__ __forceinline__ void put_chunk(uint8_t* __restrict__ row, size_t W, int c, int nchunks, uint32_t h)
+{
+    u32x4_u d;
+#pragma unroll
+    for (int q = 0; q < 4; q++) d[q] = (((h >> (4 * q)) & 0xFu) * 0x00204081u) & 0x01010101u;  // bit j -> byte j
+    *reinterpret_cast<u32x4_u*>(row + 1 + 16 * c) = d;
+    if (c == 0) row[W - 1] = (uint8_t)(h & 1u);
+    if (c == nchunks - 1) row[0] = (uint8_t)((h >> 15) & 1u);
+}
+
+// The first and last launches of a call, fused with the uint8 conversions of k_pack and
+// k_step_unpack.  IN_U8: the window comes from the padded uint8 field u (interior cells
+// (x, y) at (y+1)*(nx+2) + 1 + x, 0 or 1) and is packed in registers; else from P.
+// OUT_U8: the output rows go out as uint8 -- generation n (the launch's last) into un,
+// generation n-1 into uprev, with the periodic ring as the exchange leaves it (the
+// block holding interior row ny-1 also writes ghost row 0, the one holding row 0 ghost
+// row ny+1) -- else into Q, packed.  The uint8 rows move in 16-byte chunks, chunk
+// 64p + l of a row by lane l in pass p, so that each access instruction covers 1 KB of
+// the row; the chunks' 16-bit halves of the lanes' words pass through a per-wave row in
+// LDS (each lane owns WPL consecutive words).  A launch that reads u must not write it:
+// the host pairs IN_U8 and OUT_U8 only across two launches.
+template <int WPL, int K, int G, int NWAVES, bool IN_U8, bool OUT_U8>
+__global__ __launch_bounds__(64 * NWAVES) void k_step_tw_io(const uint8_t* __restrict__ u, const uint32_t* __restrict__ P,
+                                                             uint32_t* __restrict__ Q, uint8_t* __restrict__ un,
+                                                             uint8_t* __restrict__ uprev, int nx, int ny, int OUT,
+                                                             int gens)
+{
+    using VW = WordVec<WPL>;
+    using V = typename VW::V;
+    constexpr int NW = 64 * WPL;  // words of a row
+    constexpr int NC = 2 * NW;    // 16-cell chunks of a row
+    const size_t W = (size_t)nx + 2;
+    __shared__ uint32_t hs[2][NWAVES][2][2][NW];
+    __shared__ uint32_t stage[NWAVES][NW];  // one row of the wave, as words
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    // chunk c = half c of the row (little-endian); may_alias: the same LDS is also read
+    // and written as words
+    typedef uint16_t __attribute__((may_alias)) half_t;
+    half_t* st16 = reinterpret_cast<half_t*>(stage[wv]);
+    const int band = (int)xcd_tile(blockIdx.x, gridDim.x);
+    const int y0 = band * OUT - G + wv * K;
+    int yw = y0 % ny;
+    if (yw < 0) yw += ny;
+    uint32_t R[K][WPL], Rp[K][WPL];
+#pragma unroll
+    for (int k = 0; k < K; k++) {
+        int y = yw + k;
+        while (y >= ny) y -= ny;
+        if constexpr (IN_U8) {
+            const uint8_t* row = u + (size_t)(y + 1) * W + 1;
+#pragma unroll
+            for (int p = 0; p < NC / 64; p++) {
+                const int c = 64 * p + lane;
+                const u32x4_u b = *reinterpret_cast<const u32x4_u*>(row + 16 * c);
+                uint32_t h = 0;
+#pragma unroll
+                for (int q = 0; q < 4; q++)
+                    h |= (((b[q] & 0x01010101u) * 0x01020408u) >> 24) << (4 * q);  // byte j -> bit j, as in k_pack
+                st16[c] = (uint16_t)h;
+            }
+#pragma unroll
+            for (int w = 0; w < WPL; w++) R[k][w] = stage[wv][WPL * lane + w];
+        } else {
+            VW::get(reinterpret_cast<const V*>(P + (size_t)y * NW)[lane], R[k]);
+        }
+    }
+    tw_generations<WPL, K, G, NWAVES, OUT_U8>(R, Rp, hs, lane, wv, gens);
+    const int ybase = band * OUT - G + wv * K;
+#pragma unroll
+    for (int k = 0; k < K; k++) {
+        const int r = wv * K + k - G;
+        const int y = ybase + k;
+        if (r < 0 || r >= OUT || y >= ny) continue;
+        if constexpr (OUT_U8) {
+#pragma unroll
+            for (int f = 0; f < 2; f++) {
+                uint8_t* const dst = f == 0 ? un : uprev;
+#pragma unroll
+                for (int w = 0; w < WPL; w++) stage[wv][WPL * lane + w] = f == 0 ? R[k][w] : Rp[k][w];
+#pragma unroll
+                for (int p = 0; p < NC / 64; p++) {
+                    const int c = 64 * p + lane;
+                    const uint32_t h = st16[c];
+                    put_chunk(dst + (size_t)(y + 1) * W, W, c, NC, h);
+                    if (y == ny - 1) put_chunk(dst, W, c, NC, h);
+                    if (y == 0) put_chunk(dst + (size_t)(ny + 1) * W, W, c, NC, h);
+                }
+            }
+        } else {
+            reinterpret_cast<V*>(Q + (size_t)y * NW)[lane] = VW::put(R[k]);
+        }
+    }
+}
+
+typedef void (*tw_io_kernel)(const uint8_t*, const uint32_t*, uint32_t*, uint8_t*, uint8_t*, int, int, int, int);
 struct TbVariant {
     const char* name;
     void (*kernel)(const uint32_t*, uint32_t*, int, int, int, int, int);
     int k, g, nwaves;
     int wpl;  // full-row form: words per lane (the wave spans the row); 0: WX waves per row
+    // full-row form: the first launch of a call reading the uint8 field, and the last one
+    // writing both uint8 buffers (k_step_tw_io); null for the other forms
+    tw_io_kernel kin = nullptr, kout = nullptr;
 };
+#define TW_IO(WPL, K, G, NWAVES) k_step_tw_io<WPL, K, G, NWAVES, true, false>, k_step_tw_io<WPL, K, G, NWAVES, false, true>
 
 static const TbVariant kTb[] = {
     {"tb k16 g8 w16", k_step_tb<16, 8, 16>, 16, 8, 16, 0}, {"tb k10 g4 w16", k_step_tb<10, 4, 16>, 10, 4, 16, 0},
@@ -537,9 +661,10 @@ static const TbVariant kTb[] = {
     // rows), with more generations per launch to spread the launch's fixed load / store
     {"tb1 k11 g6 w16", k_step_tb1<11, 6, 16>, 11, 6, 16, 0}, {"tb1 k12 g8 w16", k_step_tb1<12, 8, 16>, 12, 8, 16, 0},
     // full-row waves (nx = 2048 * WPL), also 32 output rows per block at 8192 rows
-    {"tw4 k3 g8 w16", k_step_tw<4, 3, 8, 16>, 3, 8, 16, 4}, {"tw4 k4 g8 w12", k_step_tw<4, 4, 8, 12>, 4, 8, 12, 4},
-    {"tw2 k3 g8 w16", k_step_tw<2, 3, 8, 16>, 3, 8, 16, 2},
-    {"tw1 k3 g8 w16", k_step_tw<1, 3, 8, 16>, 3, 8, 16, 1},
+    {"tw4 k3 g8 w16", k_step_tw<4, 3, 8, 16>, 3, 8, 16, 4, TW_IO(4, 3, 8, 16)},
+    {"tw4 k4 g8 w12", k_step_tw<4, 4, 8, 12>, 4, 8, 12, 4, TW_IO(4, 4, 8, 12)},
+    {"tw2 k3 g8 w16", k_step_tw<2, 3, 8, 16>, 3, 8, 16, 2, TW_IO(2, 3, 8, 16)},
+    {"tw1 k3 g8 w16", k_step_tw<1, 3, 8, 16>, 3, 8, 16, 1, TW_IO(1, 3, 8, 16)},
 };
 
 }  // namespace golbits
@@ -596,10 +721,12 @@ int run_tb(appfis_ctx* ctx, const golbits::TbVariant& v, uint32_t* const P[2], i
 
 // Runs `gens` >= 1 generations of the nx x ny periodic field (nx % 32 == 0) held in
 // buf0; see the file comment for what the two buffers hold on return.  When the packed
-// row width is a multiple of 64 words (and at most 1024), generations 1 .. gens-1 run
-// temporally blocked
-// (tile height and depth autotuned per shape), and the last one with the
-// one-generation kernel, so that generation gens-1 is still at hand to unpack.
+// row width is a multiple of 64 words (and at most 1024), the generations run
+// temporally blocked (tile height and depth autotuned per shape).  With a full-row
+// variant and more generations than one launch holds, all of them run in its launches,
+// the first reading the uint8 field and the last writing both uint8 buffers; otherwise
+// k_pack packs the field, generations 1 .. gens-1 run blocked, and the last one runs
+// fused with the unpacking (k_step_unpack), so that generation gens-1 is still at hand.
 int gol_bits_run(appfis_ctx* ctx, uint8_t* b0, uint8_t* b1, int nx, int ny, int gens, int force)
 {
     APPFIS_REQUIRE(nx % 32 == 0, "gol: the bit-packed form needs nx %% 32 == 0 (nx = %d)", nx);
@@ -638,6 +765,32 @@ int gol_bits_run(appfis_ctx* ctx, uint8_t* b0, uint8_t* b1, int nx, int ny, int 
         if (st) return st;
         pick = cand[c];
     }
+    uint8_t* buf[2] = {b0, b1};
+    if (pick >= 0 && golbits::kTb[pick].kin && gens > golbits::kTb[pick].g) {
+        // full-row form over two or more launches: the first packs the uint8 field as it
+        // loads its window, the last writes generations gens and gens-1 back as uint8
+        // (rings included) -- after the first has finished reading b0, which it may
+        // overwrite
+        const golbits::TbVariant& v = golbits::kTb[pick];
+        const int OUT = tb_rows(v, NW) - 2 * v.g;
+        const dim3 grid(ceil_div((size_t)ny, (size_t)OUT)), block(64 * v.nwaves);
+        const int L = (gens + v.g - 1) / v.g;
+        st = ktime_begin(ctx);
+        if (st) return st;
+        hipLaunchKernelGGL(v.kin, grid, block, 0, ctx->stream, b0, nullptr, P[0], nullptr, nullptr, nx, ny, OUT,
+                           v.g);
+        APPFIS_LAUNCH_CHECK();
+        int cur = 0;
+        for (int l = 1; l < L - 1; l++) {
+            hipLaunchKernelGGL(v.kernel, grid, block, 0, ctx->stream, P[cur], P[cur ^ 1], NW, ny, NW / 64, OUT, v.g);
+            APPFIS_LAUNCH_CHECK();
+            cur ^= 1;
+        }
+        hipLaunchKernelGGL(v.kout, grid, block, 0, ctx->stream, nullptr, P[cur], nullptr, buf[gens & 1],
+                           buf[(gens & 1) ^ 1], nx, ny, OUT, gens - (L - 1) * v.g);
+        APPFIS_LAUNCH_CHECK();
+        return ktime_end(ctx, L, gens);
+    }
     const unsigned pblocks = ceil_div(words, 256);
     hipLaunchKernelGGL(golbits::k_pack, dim3(pblocks), dim3(256), 0, ctx->stream, b0, P[0], nx, ny, NW);
     APPFIS_LAUNCH_CHECK();
@@ -657,7 +810,6 @@ int gol_bits_run(appfis_ctx* ctx, uint8_t* b0, uint8_t* b1, int nx, int ny, int 
     // generation gens-1 is in P[cur]: the last generation runs fused with the unpacking
     // of generations gens (into buf[gens % 2]) and gens-1 (the other buffer), rings
     // exchanged
-    uint8_t* buf[2] = {b0, b1};
     const int nbands = (ny + golbits::KROWS - 1) / golbits::KROWS;
     const unsigned nitems = (unsigned)NW * (unsigned)nbands;
     hipLaunchKernelGGL(golbits::k_step_unpack<golbits::KROWS>, dim3(ceil_div(nitems, 256)), dim3(256), 0,

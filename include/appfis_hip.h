@@ -106,11 +106,14 @@ int appfis_timer_stop(appfis_ctx* ctx, float* ms); /* waits for the stop event *
 /* Timing of a call's generation kernel (a measurement aid; no reference counterpart).
  * With on != 0, each later appfis_gol call records a hipEvent on the context stream
  * before the first and after the last launch of its generation kernel -- the byte
- * kernel of the PLAIN form, the temporally blocked (or one-generation) kernel of the
- * bit-packed form, which advances generations 1 .. gens-1 -- and appfis_kernel_time
- * gives, for the most recent such call, the time between the two events, the number
- * of launches and the generations they advanced (it waits for the second event).
- * Nothing is recorded while the stream is being captured. */
+ * kernel of the PLAIN form; for the bit-packed form the temporally blocked (or
+ * one-generation) kernel, which advances generations 1 .. gens-1 between the packing
+ * and the fused last generation, or, for a full-row variant over two or more launches,
+ * all gens generations, its first launch packing and its last unpacking the uint8
+ * field -- and appfis_kernel_time gives, for the most recent such call, the time
+ * between the two events, the number of launches and the generations they advanced
+ * (it waits for the second event).  Nothing is recorded while the stream is being
+ * captured. */
 int appfis_kernel_timing(appfis_ctx* ctx, int on);
 int appfis_kernel_time(appfis_ctx* ctx, double* ms, int* launches, int* generations);
 

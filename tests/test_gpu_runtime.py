@@ -192,11 +192,22 @@ def test_kernel_timing_of_gol_calls(ctx):
     ctx.gol(d0, d1, nx, ny, 5, hip.PLAIN)
     ms, launches, gens = ctx.kernel_time()
     assert (launches, gens) == (5, 5) and ms > 0.0
+    ref = oracle.gol_u8(a.copy(), np.zeros_like(a), nx, ny, 20)
+    names = hip.gol_candidates()
+    # a full-row variant runs all 20 generations in ceil(20 / 8) launches, the first
+    # packing the uint8 field as it loads, the last writing both uint8 buffers
     d0.upload(a)
-    ctx.gol(d0, d1, nx, ny, 20, hip.CHOSEN)
+    ctx.gol(d0, d1, nx, ny, 20, hip.CANDIDATE0 + names.index("tw1 k3 g8 w16"))
     ms, launches, gens = ctx.kernel_time()
-    assert gens == 19 and 1 <= launches <= 19 and ms > 0.0
-    assert np.array_equal(d0.download(), oracle.gol_u8(a.copy(), np.zeros_like(a), nx, ny, 20))
+    assert (launches, gens) == (3, 20) and ms > 0.0
+    assert np.array_equal(d0.download(), ref)
+    # the other forms: k_pack, generations 1 .. 19 blocked (ceil(19 / 4) launches), the
+    # last one fused with the unpacking
+    d0.upload(a)
+    ctx.gol(d0, d1, nx, ny, 20, hip.CANDIDATE0 + names.index("tb k10 g4 w16"))
+    ms, launches, gens = ctx.kernel_time()
+    assert (launches, gens) == (5, 19) and ms > 0.0
+    assert np.array_equal(d0.download(), ref)
     ctx.kernel_timing(False)
     ctx.gol(d0, d1, nx, ny, 2, hip.PLAIN)
     with pytest.raises(hip.AppfisError):
