@@ -306,8 +306,10 @@ int appfis_ctx_create(int device, appfis_ctx** out)
     APPFIS_REQUIRE(c, "host allocation failed");
     c->device = device;
     hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
-    if (e == hipSuccess) e = hipEventCreate(&c->ev_start);
-    if (e == hipSuccess) e = hipEventCreate(&c->ev_stop);
+    // timer events are timing markers only: no system-scope fence (cache writeback and
+    // invalidate) when they are recorded, which would add to the interval they measure
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&c->ev_start, hipEventDisableSystemFence);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&c->ev_stop, hipEventDisableSystemFence);
     if (e == hipSuccess) e = hipDeviceGetAttribute(&c->num_cu, hipDeviceAttributeMultiprocessorCount, device);
     if (e != hipSuccess) {
         appfis_ctx_destroy(c);
@@ -471,8 +473,10 @@ int appfis_kernel_timing(appfis_ctx* ctx, int on)
     APPFIS_REQUIRE(ctx, "ctx is NULL");
     if (on && !ctx->kt0) {
         APPFIS_NOT_CAPTURING(ctx, "appfis_kernel_timing");
-        APPFIS_HIP_CHECK(hipEventCreate(&ctx->kt0));
-        APPFIS_HIP_CHECK(hipEventCreate(&ctx->kt1));
+        // timing markers only: no system-scope fence (cache writeback and invalidate)
+        // when they are recorded, so they do not slow the work they bracket
+        APPFIS_HIP_CHECK(hipEventCreateWithFlags(&ctx->kt0, hipEventDisableSystemFence));
+        APPFIS_HIP_CHECK(hipEventCreateWithFlags(&ctx->kt1, hipEventDisableSystemFence));
     }
     ctx->ktime = on != 0;
     ctx->kt_launches = -1;

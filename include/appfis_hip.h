@@ -101,6 +101,9 @@ int appfis_memcpy_d2d(appfis_ctx* ctx, void* dst, const void* src, size_t bytes)
 int appfis_memset(appfis_ctx* ctx, void* dst, int value, size_t bytes);
 
 /* ---- timing on the context stream (hipEvents) ---------------------------------- */
+/* The two events carry no system-scope fence: the interval is the stream's work alone,
+ * without a cache writeback at either end (read results through appfis_memcpy_d2h,
+ * which synchronises on its own). */
 int appfis_timer_start(appfis_ctx* ctx);
 int appfis_timer_stop(appfis_ctx* ctx, float* ms); /* waits for the stop event */
 /* Timing of a call's generation kernel (a measurement aid; no reference counterpart).
