@@ -723,7 +723,7 @@ int run_tb(appfis_ctx* ctx, const golbits::TbVariant& v, uint32_t* const P[2], i
 // buf0; see the file comment for what the two buffers hold on return.  When the packed
 // row width is a multiple of 64 words (and at most 1024), the generations run
 // temporally blocked (tile height and depth autotuned per shape).  With a full-row
-// variant and more generations than one launch holds, all of them run in its launches,
+// variant and two or more generations, all of them run in its launches (at least two),
 // the first reading the uint8 field and the last writing both uint8 buffers; otherwise
 // k_pack packs the field, generations 1 .. gens-1 run blocked, and the last one runs
 // fused with the unpacking (k_step_unpack), so that generation gens-1 is still at hand.
@@ -766,19 +766,22 @@ int gol_bits_run(appfis_ctx* ctx, uint8_t* b0, uint8_t* b1, int nx, int ny, int 
         pick = cand[c];
     }
     uint8_t* buf[2] = {b0, b1};
-    if (pick >= 0 && golbits::kTb[pick].kin && gens > golbits::kTb[pick].g) {
+    if (pick >= 0 && golbits::kTb[pick].kin && gens >= 2) {
         // full-row form over two or more launches: the first packs the uint8 field as it
         // loads its window, the last writes generations gens and gens-1 back as uint8
         // (rings included) -- after the first has finished reading b0, which it may
-        // overwrite
+        // overwrite.  Up to G generations: gens-1 in the first launch, 1 in the last;
+        // beyond: G per launch, the rest in the last.
         const golbits::TbVariant& v = golbits::kTb[pick];
         const int OUT = tb_rows(v, NW) - 2 * v.g;
         const dim3 grid(ceil_div((size_t)ny, (size_t)OUT)), block(64 * v.nwaves);
-        const int L = (gens + v.g - 1) / v.g;
+        const int L = gens <= v.g ? 2 : (gens + v.g - 1) / v.g;
+        const int first = gens <= v.g ? gens - 1 : v.g;
+        const int last = gens - first - (L - 2) * v.g;
         st = ktime_begin(ctx);
         if (st) return st;
         hipLaunchKernelGGL(v.kin, grid, block, 0, ctx->stream, b0, nullptr, P[0], nullptr, nullptr, nx, ny, OUT,
-                           v.g);
+                           first);
         APPFIS_LAUNCH_CHECK();
         int cur = 0;
         for (int l = 1; l < L - 1; l++) {
@@ -787,7 +790,7 @@ int gol_bits_run(appfis_ctx* ctx, uint8_t* b0, uint8_t* b1, int nx, int ny, int 
             cur ^= 1;
         }
         hipLaunchKernelGGL(v.kout, grid, block, 0, ctx->stream, nullptr, P[cur], nullptr, buf[gens & 1],
-                           buf[(gens & 1) ^ 1], nx, ny, OUT, gens - (L - 1) * v.g);
+                           buf[(gens & 1) ^ 1], nx, ny, OUT, last);
         APPFIS_LAUNCH_CHECK();
         return ktime_end(ctx, L, gens);
     }

@@ -111,9 +111,9 @@ int appfis_timer_stop(appfis_ctx* ctx, float* ms); /* waits for the stop event *
  * before the first and after the last launch of its generation kernel -- the byte
  * kernel of the PLAIN form; for the bit-packed form the temporally blocked (or
  * one-generation) kernel, which advances generations 1 .. gens-1 between the packing
- * and the fused last generation, or, for a full-row variant over two or more launches,
- * all gens generations, its first launch packing and its last unpacking the uint8
- * field -- and appfis_kernel_time gives, for the most recent such call, the time
+ * and the fused last generation, or, for a full-row variant and gens >= 2, all gens
+ * generations in two or more launches, the first packing and the last unpacking the
+ * uint8 field -- and appfis_kernel_time gives, for the most recent such call, the time
  * between the two events, the number of launches and the generations they advanced
  * (it waits for the second event).  Nothing is recorded while the stream is being
  * captured. */
