@@ -540,7 +540,8 @@ __global__ __launch_bounds__(64 * NWAVES) void k_step_tw(const uint32_t* __restr
 // (the compiler merges the byte stores), plus the ghost cell the periodic exchange
 // copies from it: chunk 0 holds cell 0, which ghost column nx+1 takes; the last chunk
 // holds cell nx-1, which ghost column 0 takes.
-typedef unsigned int u32x4_u __attribute__((ext_vector_type(4), aligned(1)));  // 16 bytes, any address
+// 16 bytes at any address, allowed to alias the uint8 field it reads and writes
+typedef unsigned int u32x4_u __attribute__((ext_vector_type(4), aligned(1), may_alias));
 __device__ __forceinline__ void put_chunk(uint8_t* __restrict__ row, size_t W, int c, int nchunks, uint32_t h)
 {
     u32x4_u d;
