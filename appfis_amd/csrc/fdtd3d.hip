@@ -481,6 +481,9 @@ __global__ __launch_bounds__(TXT* TY) void k_fdtd3d_dma(const S* __restrict__ u,
                 else __builtin_amdgcn_s_waitcnt(E2);
             }
             __builtin_amdgcn_s_barrier();
+            // the wait and the barrier do not count as memory accesses for the compiler:
+            // this keeps the ring reads below them (it emits no instruction)
+            __atomic_signal_fence(__ATOMIC_SEQ_CST);
             const C* cur = ring[k];
             const C* nxt4 = ring[(k + 4) % NB];
 #pragma unroll
