@@ -556,6 +556,7 @@ struct FdtdVariant {
     int vx;        // x points per thread: needs nx % vx == 0 (a lane stores all vx or none)
     int wmod = 0;  // tile rows loaded as 16-byte chunks: nx + 8 a multiple of wmod, 16-byte
                    // aligned buffers (0: no such needs)
+    bool plain = false;  // PLAIN's fixed tiling where it fits (else row 0)
 };
 
 template <typename S, int VX, int TY, int RY, int ZC, int D, int U>
@@ -573,7 +574,7 @@ int run_dma_v(appfis_ctx* ctx, void* b0, void* b1, int nx, int ny, int nz, const
 // Candidate tilings (VX x-points per thread, TY thread rows, RY rows per thread, ZC
 // planes per block, D-deep prefetch, U-step unroll: 18 rotates the queue).  Index 0 of
 // each table is the PLAIN default: the fastest measured tiling that fits any shape;
-// FP32 PLAIN takes the first "dma" row instead where nx + 8 and the buffers allow it.
+// PLAIN takes the row marked plain instead where the shape and buffers allow it.
 static const FdtdVariant kF32[] = {
     {"vx1 ty16 ry2 zc256 d4 u4", run_v<float, 1, 16, 2, 256, 4, 4>, false, 1},
     {"vx1 ty8 ry4 zc256 d2 u2", run_v<float, 1, 8, 4, 256, 2, 2>, false, 1},
@@ -595,7 +596,7 @@ static const FdtdVariant kF32[] = {
     {"vx2 ty16 ry1 zc256 d4 u4", run_v<float, 2, 16, 1, 256, 4, 4>, false, 2},
     {"vx2 ty16 ry1 zc256 d6 u6", run_v<float, 2, 16, 1, 256, 6, 6>, false, 2},
     // tiles loaded straight into LDS (k_fdtd3d_dma): "dma ty.. ry.. zc.. d.."
-    {"dma ty16 ry2 zc256 d2", run_dma_v<float, 16, 2, 256, 2>, false, 1, 4},
+    {"dma ty16 ry2 zc256 d2", run_dma_v<float, 16, 2, 256, 2>, false, 1, 4, true},
     {"dma ty16 ry2 zc256 d3", run_dma_v<float, 16, 2, 256, 3>, false, 1, 4},
     {"dma ty8 ry4 zc256 d2", run_dma_v<float, 8, 4, 256, 2>, false, 1, 4},
     {"dma ty16 ry2 zc128 d2", run_dma_v<float, 16, 2, 128, 2>, false, 1, 4},
@@ -613,7 +614,7 @@ static const FdtdVariant kF64[] = {
     {"vx1 ty8 ry2 zc128 d2 u18", run_v<double, 1, 8, 2, 128, 2, 18>, false, 1},
     {"vx1 ty4 ry4 zc256 d4 u4", run_v<double, 1, 4, 4, 256, 4, 4>, false, 1},
     {"vx2 ty8 ry2 zc256 d2 u2", run_v<double, 2, 8, 2, 256, 2, 2>, false, 2},
-    {"vx2 ty16 ry1 zc256 d2 u2", run_v<double, 2, 16, 1, 256, 2, 2>, false, 2},
+    {"vx2 ty16 ry1 zc256 d2 u2", run_v<double, 2, 16, 1, 256, 2, 2>, false, 2, 0, true},
     {"vx1 ty16 ry1 zc256 d4 u4", run_v<double, 1, 16, 1, 256, 4, 4>, false, 1},
 };
 #define FDTD16_VARIANTS(S)                                                 \
@@ -625,7 +626,7 @@ static const FdtdVariant kF64[] = {
     {"vx2 ty4 ry2 zc64 d2 u18", run_v<S, 2, 4, 2, 64, 2, 18>, true, 2},       \
     {"vx2 ty8 ry2 zc64 d2 u18", run_v<S, 2, 8, 2, 64, 2, 18>, true, 2},       \
     {"vx2 ty4 ry4 zc64 d2 u2", run_v<S, 2, 4, 4, 64, 2, 2>, true, 2},        \
-    {"vx2 ty8 ry2 zc128 d2 u18", run_v<S, 2, 8, 2, 128, 2, 18>, true, 2}
+    {"vx2 ty8 ry2 zc128 d2 u18", run_v<S, 2, 8, 2, 128, 2, 18>, true, 2, 0, true}
 static const FdtdVariant kF16[] = {FDTD16_VARIANTS(__half)};
 static const FdtdVariant kBF16[] = {FDTD16_VARIANTS(__hip_bfloat16)};
 
@@ -683,11 +684,11 @@ extern "C" int appfis_fdtd3d(appfis_ctx* ctx, int dtype, void* buf0, void* buf1,
     }
     if (steps == 0) return APPFIS_OK;
     if (variant == APPFIS_PLAIN) {
-        // a fixed tiling: the table's first direct-to-LDS row where the rows and buffers
-        // admit it (FP32), else row 0 -- one launch per sweep either way
+        // a fixed tiling: the table's row marked plain where the shape and the buffers
+        // admit it, else row 0 -- one launch per sweep either way
         int i0 = 0;
         for (int i = 0; i < ntab; i++)
-            if (tab[i].wmod && fits(tab[i])) {
+            if (tab[i].plain && fits(tab[i])) {
                 i0 = i;
                 break;
             }

@@ -398,12 +398,17 @@ def _plan(iters, ts):
 
 
 
-def _fdtd_plain(hip, dt, n):
-    """The PLAIN form's fixed tiling: the first direct-to-LDS row where n + 8 is a
-    multiple of 4 (the bench's buffers are 256-byte aligned), else the table's row 0."""
-    names = hip.fdtd3d_candidates(dt)
-    dma = [x for x in names if x.startswith("dma ")]
-    return dma[0] if dma and (n + 8) % 4 == 0 else names[0]
+# the PLAIN form's preferred fixed tilings (the library's rows marked plain), used where
+# the shape admits them: the bench's buffers are 256-byte aligned and n = 512
+FDTD_PLAIN_PREFERRED = {"f32": "dma ty16 ry2 zc256 d2", "f64": "vx2 ty16 ry1 zc256 d2 u2",
+                        "f16": "vx2 ty8 ry2 zc128 d2 u18", "bf16": "vx2 ty8 ry2 zc128 d2 u18"}
+
+
+def _fdtd_plain(hip, name, dt, n):
+    """The PLAIN form's fixed tiling at n^3: the preferred row where it fits (F32: n + 8
+    a multiple of 4; the others: n even), else the table's row 0."""
+    fits = (n + 8) % 4 == 0 if name == "f32" else n % 2 == 0
+    return FDTD_PLAIN_PREFERRED[name] if fits else hip.fdtd3d_candidates(dt)[0]
 
 
 FDTD_CTYPE = {"f32": "float", "f64": "double", "f16": "__half", "bf16": "__hip_bfloat16"}
@@ -422,7 +427,7 @@ def bench_fdtd(ctx, hip, n=512, steps=100):
             ctx.fdtd3d(a, b, n, n, n, coeff, 2, variant)  # warm-up (and the one-time autotuning)
             ctx.fill_uniform(a, 4, 1004)
             ms, _ = timed(ctx, lambda: ctx.fdtd3d(a, b, n, n, n, coeff, steps, variant))
-            tname = _fdtd_plain(hip, dt, n) if variant == hip.PLAIN else _tuned(ctx, 3, dt)
+            tname = _fdtd_plain(hip, name, dt, n) if variant == hip.PLAIN else _tuned(ctx, 3, dt)
             if tname.startswith("dma "):  # direct-to-LDS form: "dma ty16 ry2 zc256 d2"
                 ty, ry, zc, dd = (int(t[2:]) if t[:2] in ("ty", "ry", "zc") else int(t[1:])
                                   for t in tname.split()[1:])

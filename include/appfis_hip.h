@@ -230,9 +230,11 @@ int appfis_gol_candidates(char* names, int len);
 /* ---- FDTD3d radius-4 star (config form of FDTD3d/src/fdtd3d_appfis.hpp:129-143):
  * u' = c[0]*u + sum_{r=1..4} c[r]*(6 neighbours at distance r); ghost 4 held at 0.
  * dtype F32 / F64 (arithmetic in that type) or F16 / BF16 (storage only, FP32
- * arithmetic).  Field in bufs[steps % 2] at exit.  PLAIN runs a fixed tiling (F32: the
- * direct-to-LDS one when (nx + 8) % 4 == 0 and both buffers are 16-byte aligned, else
- * the default), one launch per sweep;
+ * arithmetic).  Field in bufs[steps % 2] at exit.  PLAIN runs a fixed tiling, one
+ * launch per sweep: per dtype a preferred one where the shape and buffers admit it (F32:
+ * direct-to-LDS, needs (nx + 8) % 4 == 0 and 16-byte aligned buffers; F64: two points
+ * per lane, nx even; F16 / BF16: two points per lane in 32-bit pairs, nx even and
+ * 4-byte aligned buffers), else one that fits any shape;
  * CHOSEN runs the tiling the autotuner picked for this dtype and shape (timed once on
  * scratch buffers, cached in the context, among the candidates the shape and the
  * buffers' alignment admit); APPFIS_CANDIDATE0 + i runs candidate i of
