@@ -22,7 +22,7 @@ per launch measured with rocprofv3 --pmc (FETCH_SIZE + WRITE_SIZE, calibrated on
 kernel), read from profiles/traffic.json when present.  `valu` is the vector-ALU issue
 rate: the wave instructions rocprofv3 counted per dispatch (SQ_INSTS_VALU, same file)
 over the launch time measured here, against 256 CUs x 64 lanes x 2.4 GHz; the
-VALU-bound kernels (bit-packed GoL, flood flux) are judged by it.
+VALU-bound kernels (bit-packed GoL, flood flux) are bounded by it.
 
 cpu_baseline: the oracle (OpenMP C restatement of the reference CPU path) timed on this
 host on a bounded sample of the headline workload.
