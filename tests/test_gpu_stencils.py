@@ -336,6 +336,28 @@ def test_fdtd3d_every_candidate_bit_exact(ctx, dt, n):
     assert ran >= 2
 
 
+@pytest.mark.parametrize("dt,off", [(hip.F32, 4), (hip.F64, 8), (hip.F16, 2), (hip.BF16, 2)])
+def test_fdtd3d_on_unaligned_buffers(ctx, dt, off):
+    # buffers that start `off` bytes into an allocation, so that the preferred tilings'
+    # alignment fails (16 bytes for the direct-to-LDS rows, 4 for the 16-bit pairs):
+    # PLAIN and CHOSEN fall back to tilings that fit and stay bit-exact (nx + 8 = 72 and
+    # an even nx would otherwise admit them)
+    from types import SimpleNamespace
+    n = (64, 20, 17)
+    coeff = FDTD_COEFF if dt != hip.F16 else [c / 2.0625 for c in FDTD_COEFF]
+    a, d0, d1 = _fdtd_pair(ctx, n, dt)
+    ref = oracle.fdtd3d(a.copy(), np.zeros_like(a), *n, coeff, 3, bf16=(dt == hip.BF16))
+    big = [ctx.array((a.nbytes + 64,), hip.U8) for _ in range(2)]
+    bufs = [SimpleNamespace(ptr=b.ptr + off, dtype=dt) for b in big]
+    for variant in (hip.PLAIN, hip.CHOSEN):
+        print(f"[fdtd unaligned] dtype {dt} offset {off} variant {variant}", flush=True)
+        hip.check(hip.lib().appfis_memcpy_h2d(ctx.handle, bufs[0].ptr, a.ctypes.data, a.nbytes), "h2d")
+        ctx.fdtd3d(bufs[0], bufs[1], *n, coeff, 3, variant)
+        got = np.empty_like(a)
+        hip.check(hip.lib().appfis_memcpy_d2h(ctx.handle, got.ctypes.data, bufs[1].ptr, a.nbytes), "d2h")
+        assert np.array_equal(got.view(np.uint8), ref.view(np.uint8)), variant
+
+
 # unit roundoff of the storage formats: every sweep rounds each stored value once
 FDTD_UNIT_ROUNDOFF = {hip.F32: 2.0 ** -24, hip.F16: 2.0 ** -11, hip.BF16: 2.0 ** -8}
 
