@@ -231,6 +231,22 @@ def test_gol_every_blocked_candidate(ctx, nx, ny, gens):
     assert len(ran) >= 3
 
 
+@pytest.mark.parametrize("gens", [2, 7, 8, 9, 16, 17, 24, 25])
+def test_gol_full_row_launch_split(ctx, gens):
+    # the full-row form runs a call's generations in at least two launches, the first
+    # packing and the last unpacking the uint8 field (up to G = 8 generations: n-1 and 1;
+    # beyond: G per launch and the rest in the last): generation counts around G and 2G,
+    # both buffers against the oracle
+    nx, ny = 2048, 40
+    names = hip.gol_candidates()
+    a, d0, d1 = _gol_pair(ctx, nx, ny, seed=gens + 500)
+    ctx.gol(d0, d1, nx, ny, gens, hip.CANDIDATE0 + names.index("tw1 k3 g8 w16"))
+    ref = oracle.gol_u8(a.copy(), np.zeros_like(a), nx, ny, gens)
+    prev = oracle.gol_u8(a.copy(), np.zeros_like(a), nx, ny, gens - 1)
+    assert np.array_equal((d0, d1)[gens % 2].download(), ref)
+    assert np.array_equal((d0, d1)[(gens + 1) % 2].download(), prev)
+
+
 def test_gol_chosen_glider_period(ctx):
     # a glider on an N x N torus returns to its start after 4N generations
     n = 64
