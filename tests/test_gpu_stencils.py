@@ -326,11 +326,12 @@ def test_fdtd3d_bit_exact(ctx, dt, n, steps, variant):
 
 
 @pytest.mark.parametrize("dt", [hip.F32, hip.F64, hip.F16, hip.BF16])
-@pytest.mark.parametrize("n", [(70, 33, 40), (68, 21, 37), (71, 30, 19), (68, 9, 259)])
+@pytest.mark.parametrize("n", [(70, 33, 40), (68, 21, 37), (71, 30, 19), (68, 9, 259), (68, 1, 1), (68, 2, 3)])
 def test_fdtd3d_every_candidate_bit_exact(ctx, dt, n):
     # every tiling the autotuner may pick, forced one at a time; nx = 70, 68, 71 covers
-    # x points per lane (1, 2, 4) that do and do not divide nx, and nz = 259 gives the
-    # 64- to 256-plane z-chunks a last chunk of 3 planes (fewer than a prefetch ring)
+    # x points per lane (1, 2, 4) that do and do not divide nx, nz = 259 gives the
+    # 64- to 256-plane z-chunks a last chunk of 3 planes (fewer than a prefetch ring),
+    # and one plane of one row / three planes of two rows leave every prefetch short
     coeff = FDTD_COEFF if dt != hip.F16 else [c / 2.0625 for c in FDTD_COEFF]
     a, d0, d1 = _fdtd_pair(ctx, n, dt)
     ref = oracle.fdtd3d(a.copy(), np.zeros_like(a), *n, coeff, 3, bf16=(dt == hip.BF16))
