@@ -113,9 +113,10 @@ def bench_gol(ctx, hip, steps, warmup, variant, nx=8192, ny=8192):
     """One call of `steps` generations, timed.  Inside that call the library records HIP
     events on its stream before the first and after the last launch of the generation
     kernel (appfis_kernel_timing): their interval over the launch count is the kernel's
-    average launch duration, without the call's other work (the PLAIN form's initial ring
-    exchange; the CHOSEN form's pack and its last generation fused with the unpacking of
-    both buffers)."""
+    average launch duration.  It leaves out the PLAIN form's initial ring exchange; for
+    the bit-packed form with full-row waves it spans the whole call (its first launch
+    packs, its last unpacks), otherwise it leaves out the packing and the last generation
+    fused with the unpacking of both buffers."""
     d0 = ctx.array((ny + 2, nx + 2), hip.U8)
     d1 = ctx.array((ny + 2, nx + 2), hip.U8)
     ctx.fill_bernoulli(d0, 1, 1002, 0.25)
